@@ -1,0 +1,642 @@
+// mdmc — host side of the C ABI declared in include/mdmc.h.
+//
+// Owns the case tables in constant memory, per-context scratch in HBM, the
+// tile planner and the launches of the kernels in mdmc_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mdmc.h"
+#include "dmc_luts.h"
+#include "mdmc_fields.h"
+#include "mdmc_kernels.hpp"
+#include "mdmc_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+std::mutex g_setup_mutex;
+uint64_t g_setup_mask = 0;  // bit d: tables uploaded on device d
+mdmc_ctx *g_default_ctx = nullptr;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return fail(MDMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+int current_device(int *dev) {
+  HIP_TRY(hipGetDevice(dev));
+  return MDMC_OK;
+}
+
+bool is_setup_on(int dev) {
+  std::lock_guard<std::mutex> lk(g_setup_mutex);
+  return dev >= 0 && dev < 64 && ((g_setup_mask >> dev) & 1ull);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Planning
+
+namespace mdmc {
+
+static int env_int(const char *name, int dflt) {
+  const char *s = std::getenv(name);
+  return s ? std::atoi(s) : dflt;
+}
+
+int plan_tiling(const uint32_t dims[3], Plan *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->X = dims[0];
+  p->Y = dims[1];
+  p->Z = dims[2];
+  if (p->X < 2 || p->Y < 2 || p->Z < 2) {
+    p->empty = 1;
+    return MDMC_OK;
+  }
+  const uint64_t npts = (uint64_t)p->X * p->Y * p->Z;
+  if (npts > 0xFFFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "grid has more than 2^32-1 points");
+  p->cx = p->X - 1;
+  p->cy = p->Y - 1;
+  p->cz = p->Z - 1;
+  // x-tile: smallest 64*W covering the row, at most 1024 cells.
+  uint32_t W = 1;
+  while (W < 16 && 64u * W < p->cx) W <<= 1;
+  uint32_t R = (W >= 16) ? 4 : 8;
+  if (const char *t = std::getenv("MDMC_TILE")) {  // "W,R,Kc" override for tuning
+    int w = 0, r = 0, kc = 0;
+    if (std::sscanf(t, "%d,%d,%d", &w, &r, &kc) == 3) {
+      W = (uint32_t)w;
+      R = (uint32_t)r;
+      p->Kc = (uint32_t)kc;
+    }
+  }
+  if (!tiling_supported(W, R)) return fail(MDMC_ERR_INVALID, "unsupported tile shape");
+  p->W = W;
+  p->R = R;
+  p->cap = 64u * W;
+  p->XT = (p->cx + p->cap - 1) / p->cap;
+  p->NRB = (p->cy + R - 1) / R;
+  if (p->Kc == 0) {
+    // Enough waves to cover the chip several times over: ~8 per CU.
+    const uint64_t target = (uint64_t)env_int("MDMC_TARGET_WAVES", 2048);
+    const uint64_t cols = (uint64_t)p->XT * p->NRB;
+    uint64_t nkc = (target + cols - 1) / cols;
+    if (nkc < 1) nkc = 1;
+    uint32_t Kc = (uint32_t)((p->cz + nkc - 1) / nkc);
+    if (Kc < 4) Kc = std::min<uint32_t>(4, p->cz);
+    p->Kc = Kc;
+  }
+  p->NKC = (p->cz + p->Kc - 1) / p->Kc;
+  p->nsegs = (uint64_t)p->XT * p->cy * p->cz;
+  p->ntiles = (uint32_t)((p->nsegs + kScanTile - 1) / kScanTile);
+  p->nslots = p->nsegs * p->cap;
+  const uint64_t cells = (uint64_t)p->cx * p->cy * p->cz;
+  p->max_chunks = (cells + 63) / 64 + 1;
+  p->classify_blocks = (uint64_t)p->XT * p->NRB * p->NKC;
+  return MDMC_OK;
+}
+
+}  // namespace mdmc
+
+// ---------------------------------------------------------------------------
+// Context and scratch
+
+struct mdmc_ctx {
+  int device = 0;
+  int num_cus = 256;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  mdmc::Scratch s{};
+  mdmc::Plan plan{};
+  bool counted = false;
+  const float *counted_field = nullptr;
+  float iso = 0.f;
+  // generator tables
+  void *gen = nullptr;
+  size_t gen_bytes = 0;
+  // per-kernel event timing
+  bool profiling = false;
+  std::vector<hipEvent_t> events;  // pool, used in (start, stop) pairs
+  std::vector<int> event_kernel;   // kernel id of each used pair
+  size_t events_used = 0;          // pairs in use
+};
+
+namespace mdmc {
+
+// Brackets one launch with HIP events when the context is profiling.
+struct ProfScope {
+  mdmc_ctx *ctx;
+  hipStream_t st;
+  hipEvent_t stop = nullptr;
+  ProfScope(mdmc_ctx *c, int kernel, hipStream_t s) : ctx(c), st(s) {
+    if (!ctx || !ctx->profiling) return;
+    const size_t pair = ctx->events_used;
+    while (ctx->events.size() < 2 * (pair + 1)) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { (void)hipGetLastError(); return; }
+      ctx->events.push_back(e);
+    }
+    if (ctx->event_kernel.size() < pair + 1) ctx->event_kernel.resize(pair + 1);
+    ctx->event_kernel[pair] = kernel;
+    if (hipEventRecord(ctx->events[2 * pair], st) != hipSuccess) { (void)hipGetLastError(); return; }
+    stop = ctx->events[2 * pair + 1];
+    ctx->events_used = pair + 1;
+  }
+  ~ProfScope() {
+    if (stop) (void)hipEventRecord(stop, st);
+  }
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
+  size_t off = 0;
+  Scratch s{};
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  const size_t o_counts = take(p.nsegs * sizeof(uint2));
+  const size_t o_vq = take(p.nsegs * sizeof(uint2));
+  const size_t o_ne_seg = take((p.nsegs + 1) * sizeof(uint32_t));
+  const size_t o_ne_ap = take((p.nsegs + 1) * sizeof(uint32_t));
+  const size_t o_chunk = take(p.max_chunks * sizeof(uint32_t));
+  const size_t o_state = take((size_t)p.ntiles * 4 * sizeof(unsigned long long));
+  const size_t o_ticket = take(sizeof(uint32_t));
+  const size_t o_totals = take(sizeof(Totals));
+  const size_t o_slots = take(p.nslots * sizeof(uint64_t));
+  if (off > ctx->scratch_bytes) {
+    if (ctx->scratch) {
+      hipError_t e = hipFree(ctx->scratch);
+      (void)e;
+      ctx->scratch = nullptr;
+      ctx->scratch_bytes = 0;
+    }
+    const size_t want = off + off / 8;  // headroom for slightly larger grids
+    if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
+      (void)hipGetLastError();
+      if (hipMalloc(&ctx->scratch, off) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->scratch = nullptr;
+        return fail(MDMC_ERR_NOMEM, "scratch allocation failed");
+      }
+      ctx->scratch_bytes = off;
+    } else {
+      ctx->scratch_bytes = want;
+    }
+  }
+  char *b = static_cast<char *>(ctx->scratch);
+  s.seg_counts = reinterpret_cast<uint2 *>(b + o_counts);
+  s.seg_vq = reinterpret_cast<uint2 *>(b + o_vq);
+  s.ne_seg = reinterpret_cast<uint32_t *>(b + o_ne_seg);
+  s.ne_ap = reinterpret_cast<uint32_t *>(b + o_ne_ap);
+  s.chunk_ne = reinterpret_cast<uint32_t *>(b + o_chunk);
+  s.scan_state = reinterpret_cast<unsigned long long *>(b + o_state);
+  s.ticket = reinterpret_cast<uint32_t *>(b + o_ticket);
+  s.totals = reinterpret_cast<Totals *>(b + o_totals);
+  s.slots = reinterpret_cast<uint64_t *>(b + o_slots);
+  ctx->s = s;
+  return MDMC_OK;
+}
+
+__global__ void k_zero_counts(uint32_t *c, Totals *t) {
+  if (threadIdx.x < 4 && c) c[threadIdx.x] = 0u;
+  if (threadIdx.x < 8 && t) reinterpret_cast<uint32_t *>(t)[threadIdx.x] = 0u;
+}
+
+int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
+                 mdmc_counts *d_counts, hipStream_t st) {
+  const Scratch &s = ctx->s;
+  if (p.empty) {
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st,
+                       reinterpret_cast<uint32_t *>(d_counts), s.totals);
+    HIP_TRY(hipGetLastError());
+    return MDMC_OK;
+  }
+  ClassifyArgs ca{};
+  ca.g = FieldDesc{d_field, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
+  ca.XT = p.XT;
+  ca.NRB = p.NRB;
+  ca.NKC = p.NKC;
+  ca.Kc = p.Kc;
+  ca.seg_counts = s.seg_counts;
+  ca.slots = s.slots;
+  ca.scan_state = s.scan_state;
+  ca.scan_state_words = p.ntiles * 4;
+  ca.scan_ticket = s.ticket;
+  ca.totals = s.totals;
+  ca.counts_out = reinterpret_cast<uint32_t *>(d_counts);
+  int rc;
+  {
+    ProfScope ps(ctx, 0, st);
+    rc = launch_classify(p.W, p.R, ca, (uint32_t)p.classify_blocks, st);
+  }
+  if (rc != MDMC_OK) return fail(MDMC_ERR_INVALID, "no classify kernel for this tile shape");
+  HIP_TRY(hipGetLastError());
+
+  ScanArgs sa{};
+  sa.seg_counts = s.seg_counts;
+  sa.nsegs = (uint32_t)p.nsegs;
+  sa.ntiles = p.ntiles;
+  sa.seg_vq = s.seg_vq;
+  sa.ne_seg = s.ne_seg;
+  sa.ne_ap = s.ne_ap;
+  sa.chunk_ne = s.chunk_ne;
+  sa.state = s.scan_state;
+  sa.ticket = s.ticket;
+  sa.totals = s.totals;
+  sa.counts_out = reinterpret_cast<uint32_t *>(d_counts);
+  sa.max_chunks = p.max_chunks;
+  {
+    ProfScope ps(ctx, 1, st);
+    hipLaunchKernelGGL(k_scan, dim3(p.ntiles), dim3(kScanThreads), 0, st, sa);
+  }
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+uint32_t emit_blocks(const mdmc_ctx *ctx, const Plan &p) {
+  const uint64_t want = (p.max_chunks + 3) / 4;
+  const uint64_t cap = (uint64_t)ctx->num_cus * 8;
+  return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
+                const float mn[3], const float mx[3], float *d_vertices, uint64_t vcap,
+                int32_t *d_quads, uint64_t qcap, mdmc_counts *d_counts, hipStream_t st) {
+  if (p.empty) return MDMC_OK;
+  const Scratch &s = ctx->s;
+  EmitArgs ea{};
+  ea.g = FieldDesc{d_field, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
+  ea.mnx = mn[0];
+  ea.mny = mn[1];
+  ea.mnz = mn[2];
+  // xyz_range = xyz_max - xyz_min in float (dmc.cu:2149)
+  ea.rgx = mx[0] - mn[0];
+  ea.rgy = mx[1] - mn[1];
+  ea.rgz = mx[2] - mn[2];
+  ea.XT = p.XT;
+  ea.cap = p.cap;
+  ea.nsegs = (uint32_t)p.nsegs;
+  ea.max_chunks = p.max_chunks;
+  ea.seg_counts = s.seg_counts;
+  ea.seg_vq = s.seg_vq;
+  ea.slots = s.slots;
+  ea.ne_seg = s.ne_seg;
+  ea.ne_ap = s.ne_ap;
+  ea.chunk_ne = s.chunk_ne;
+  ea.totals = s.totals;
+  ea.verts = d_vertices;
+  ea.vcap = d_vertices ? vcap : 0;
+  ea.quads = d_quads;
+  ea.qcap = d_quads ? qcap : 0;
+  ea.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
+  {
+    ProfScope ps(ctx, 2, st);
+    hipLaunchKernelGGL(k_emit, dim3(emit_blocks(ctx, p)), dim3(kEmitThreads), 0, st, ea);
+  }
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+}  // namespace mdmc
+
+using namespace mdmc;
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char *mdmc_status_string(int status) {
+  switch (status) {
+    case MDMC_OK: return "ok";
+    case MDMC_ERR_INVALID: return "invalid argument";
+    case MDMC_ERR_HIP: return "HIP runtime error";
+    case MDMC_ERR_NOMEM: return "out of memory";
+    case MDMC_ERR_CAPACITY: return "output capacity too small";
+    case MDMC_ERR_TOO_LARGE: return "grid too large";
+    case MDMC_ERR_NOT_SETUP: return "mdmc_setup() not called";
+    case MDMC_ERR_INTERNAL: return "device invariant check failed";
+    case MDMC_ERR_REJECTED: return "request rejected by admission";
+    default: return "unknown status";
+  }
+}
+
+const char *mdmc_last_error(void) { return g_last_error.c_str(); }
+
+int mdmc_setup(void) {
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  if (dev < 0 || dev >= 64) return fail(MDMC_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_setup_mutex);
+  if ((g_setup_mask >> dev) & 1ull) return MDMC_OK;
+  mdmc_luts L;
+  mdmc_build_luts(&L);
+  static uint64_t packed[2][256];
+  static uint16_t info[256];
+  mdmc_pack_luts(&L, packed, info);
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_label), packed, sizeof(packed)));
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_cfg_info), info, sizeof(info)));
+  g_setup_mask |= 1ull << dev;
+  return MDMC_OK;
+}
+
+int mdmc_cleanup(void) {
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  if (g_default_ctx) {
+    mdmc_ctx_destroy(g_default_ctx);
+    g_default_ctx = nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_setup_mutex);
+  if (dev >= 0 && dev < 64) g_setup_mask &= ~(1ull << dev);
+  return MDMC_OK;
+}
+
+int mdmc_ctx_create(mdmc_ctx **out) {
+  if (!out) return fail(MDMC_ERR_INVALID, "null ctx pointer");
+  *out = nullptr;
+  int rc = mdmc_setup();
+  if (rc != MDMC_OK) return rc;
+  mdmc_ctx *ctx = new (std::nothrow) mdmc_ctx();
+  if (!ctx) return fail(MDMC_ERR_NOMEM, "ctx allocation failed");
+  HIP_TRY(hipGetDevice(&ctx->device));
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess &&
+      cus > 0)
+    ctx->num_cus = cus;
+  *out = ctx;
+  return MDMC_OK;
+}
+
+int mdmc_ctx_profile(mdmc_ctx *ctx, int enable) {
+  if (!ctx) return fail(MDMC_ERR_INVALID, "null ctx");
+  ctx->profiling = enable != 0;
+  return MDMC_OK;
+}
+
+int mdmc_ctx_profile_read(mdmc_ctx *ctx, double ms_total[4], uint32_t launches[4]) {
+  if (!ctx || !ms_total || !launches) return fail(MDMC_ERR_INVALID, "null argument");
+  for (int i = 0; i < 4; ++i) { ms_total[i] = 0.0; launches[i] = 0; }
+  for (size_t pr = 0; pr < ctx->events_used; ++pr) {
+    HIP_TRY(hipEventSynchronize(ctx->events[2 * pr + 1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->events[2 * pr], ctx->events[2 * pr + 1]));
+    const int k = std::min(3, std::max(0, ctx->event_kernel[pr]));
+    ms_total[k] += ms;
+    launches[k] += 1;
+  }
+  ctx->events_used = 0;
+  return MDMC_OK;
+}
+
+void mdmc_ctx_destroy(mdmc_ctx *ctx) {
+  if (!ctx) return;
+  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->gen) (void)hipFree(ctx->gen);
+  delete ctx;
+}
+
+int mdmc_count_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3], float iso,
+                      mdmc_counts *d_counts, void *stream) {
+  if (!ctx || !dims || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
+  if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  Plan p;
+  int rc = plan_tiling(dims, &p);
+  if (rc != MDMC_OK) return rc;
+  if (!p.empty && !d_field) return fail(MDMC_ERR_INVALID, "null field");
+  rc = ensure_scratch(ctx, p);
+  if (rc != MDMC_OK) return rc;
+  rc = launch_count(ctx, p, d_field, iso, d_counts, static_cast<hipStream_t>(stream));
+  if (rc != MDMC_OK) return rc;
+  ctx->plan = p;
+  ctx->counted = true;
+  ctx->counted_field = d_field;
+  ctx->iso = iso;
+  return MDMC_OK;
+}
+
+int mdmc_emit_device(mdmc_ctx *ctx, const float *d_field, const float xyz_min[3],
+                     const float xyz_max[3], float *d_vertices, uint64_t vertex_capacity,
+                     int32_t *d_quads, uint64_t quad_capacity, mdmc_counts *d_counts,
+                     void *stream) {
+  if (!ctx || !xyz_min || !xyz_max || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
+  if (!ctx->counted) return fail(MDMC_ERR_INVALID, "emit without a preceding count");
+  if (d_field != ctx->counted_field) return fail(MDMC_ERR_INVALID, "emit field differs from counted field");
+  return launch_emit(ctx, ctx->plan, d_field, ctx->iso, xyz_min, xyz_max, d_vertices, vertex_capacity,
+                     d_quads, quad_capacity, d_counts, static_cast<hipStream_t>(stream));
+}
+
+int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3],
+                        const float xyz_min[3], const float xyz_max[3], float iso,
+                        float *d_vertices, uint64_t vertex_capacity, int32_t *d_quads,
+                        uint64_t quad_capacity, mdmc_counts *d_counts, void *stream) {
+  int rc = mdmc_count_device(ctx, d_field, dims, iso, d_counts, stream);
+  if (rc != MDMC_OK) return rc;
+  return mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_vertices, vertex_capacity, d_quads,
+                          quad_capacity, d_counts, stream);
+}
+
+int mdmc_cells_device(mdmc_ctx *ctx, uint32_t *d_cell_index, uint8_t *d_cfg, uint8_t *d_flags,
+                      uint32_t *d_vertex_begin, uint64_t capacity, void *stream) {
+  if (!ctx || !d_cell_index || !d_cfg || !d_flags || !d_vertex_begin)
+    return fail(MDMC_ERR_INVALID, "null argument");
+  if (!ctx->counted) return fail(MDMC_ERR_INVALID, "no counted extraction on this context");
+  const Plan &p = ctx->plan;
+  if (p.empty) return MDMC_OK;
+  CellsArgs ca{};
+  ca.XT = p.XT;
+  ca.cap = p.cap;
+  ca.cx = p.cx;
+  ca.cy = p.cy;
+  ca.nsegs = (uint32_t)p.nsegs;
+  ca.max_chunks = p.max_chunks;
+  ca.seg_counts = ctx->s.seg_counts;
+  ca.slots = ctx->s.slots;
+  ca.seg_vq = ctx->s.seg_vq;
+  ca.ne_seg = ctx->s.ne_seg;
+  ca.ne_ap = ctx->s.ne_ap;
+  ca.chunk_ne = ctx->s.chunk_ne;
+  ca.totals = ctx->s.totals;
+  ca.cell_index = d_cell_index;
+  ca.cfg = d_cfg;
+  ca.flags = d_flags;
+  ca.vbegin = d_vertex_begin;
+  ca.capacity = capacity;
+  hipLaunchKernelGGL(k_cells, dim3(emit_blocks(ctx, p)), dim3(256), 0, static_cast<hipStream_t>(stream), ca);
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
+             const float xyz_max[3], float iso, uint32_t num_smooth, mdmc_mesh *out) {
+  if (!out || !dims || !xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  if (num_smooth != 0) return fail(MDMC_ERR_INVALID, "smoothing (num_smooth > 0) is not provided");
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  if (!is_setup_on(dev)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  if (!g_default_ctx) {
+    rc = mdmc_ctx_create(&g_default_ctx);
+    if (rc != MDMC_OK) return rc;
+  }
+  mdmc_ctx *ctx = g_default_ctx;
+  Plan p;
+  rc = plan_tiling(dims, &p);
+  if (rc != MDMC_OK) return rc;
+  if (p.empty) return MDMC_OK;
+  if (!field) return fail(MDMC_ERR_INVALID, "null field");
+  const size_t npts = (size_t)p.X * p.Y * p.Z;
+  float *d_field = nullptr;
+  mdmc_counts *d_counts = nullptr;
+  float *d_v = nullptr;
+  int32_t *d_q = nullptr;
+  mdmc_counts h{};
+  auto cleanup = [&]() {
+    if (d_field) (void)hipFree(d_field);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_v) (void)hipFree(d_v);
+    if (d_q) (void)hipFree(d_q);
+  };
+#define RUN_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      cleanup();                                                                        \
+      return fail(MDMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    }                                                                                   \
+  } while (0)
+  RUN_TRY(hipMalloc(&d_field, npts * sizeof(float)));
+  RUN_TRY(hipMalloc(&d_counts, sizeof(mdmc_counts)));
+  RUN_TRY(hipMemcpy(d_field, field, npts * sizeof(float), hipMemcpyHostToDevice));
+  rc = mdmc_count_device(ctx, d_field, dims, iso, d_counts, nullptr);
+  if (rc != MDMC_OK) { cleanup(); return rc; }
+  RUN_TRY(hipMemcpy(&h, d_counts, sizeof(h), hipMemcpyDeviceToHost));
+  if (h.status != 0) { cleanup(); return fail(MDMC_ERR_INTERNAL, "device status after count"); }
+  if (h.num_vertices) RUN_TRY(hipMalloc(&d_v, (size_t)h.num_vertices * 3 * sizeof(float)));
+  if (h.num_quads) RUN_TRY(hipMalloc(&d_q, (size_t)h.num_quads * 4 * sizeof(int32_t)));
+  rc = mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_v, h.num_vertices, d_q, h.num_quads,
+                        d_counts, nullptr);
+  if (rc != MDMC_OK) { cleanup(); return rc; }
+  mdmc_counts h2{};
+  RUN_TRY(hipMemcpy(&h2, d_counts, sizeof(h2), hipMemcpyDeviceToHost));
+  if (h2.status != 0) { cleanup(); return fail(MDMC_ERR_INTERNAL, "device status after emit"); }
+  out->num_active = h.num_active;
+  out->num_vertices = h.num_vertices;
+  out->num_quads = h.num_quads;
+  out->vertices = static_cast<float *>(std::malloc(std::max<size_t>(1, (size_t)h.num_vertices * 3 * sizeof(float))));
+  out->quads = static_cast<int32_t *>(std::malloc(std::max<size_t>(1, (size_t)h.num_quads * 4 * sizeof(int32_t))));
+  if (!out->vertices || !out->quads) {
+    cleanup();
+    mdmc_mesh_free(out);
+    return fail(MDMC_ERR_NOMEM, "host mesh allocation failed");
+  }
+  if (h.num_vertices)
+    RUN_TRY(hipMemcpy(out->vertices, d_v, (size_t)h.num_vertices * 3 * sizeof(float), hipMemcpyDeviceToHost));
+  if (h.num_quads)
+    RUN_TRY(hipMemcpy(out->quads, d_q, (size_t)h.num_quads * 4 * sizeof(int32_t), hipMemcpyDeviceToHost));
+#undef RUN_TRY
+  cleanup();
+  return MDMC_OK;
+}
+
+void mdmc_mesh_free(mdmc_mesh *mesh) {
+  if (!mesh) return;
+  std::free(mesh->vertices);
+  std::free(mesh->quads);
+  std::memset(mesh, 0, sizeof(*mesh));
+}
+
+// ---- synthetic fields ------------------------------------------------------
+
+static int ensure_gen(mdmc_ctx *ctx, size_t bytes) {
+  if (bytes <= ctx->gen_bytes) return MDMC_OK;
+  if (ctx->gen) (void)hipFree(ctx->gen);
+  ctx->gen = nullptr;
+  ctx->gen_bytes = 0;
+  if (hipMalloc(&ctx->gen, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(MDMC_ERR_NOMEM, "generator table allocation failed");
+  }
+  ctx->gen_bytes = bytes;
+  return MDMC_OK;
+}
+
+int mdmc_field_blobs_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], uint64_t seed,
+                            uint32_t nblobs, double sigma_lo, double sigma_hi, double amp_lo,
+                            double amp_hi, void *stream) {
+  if (!ctx || !d_out || !dims) return fail(MDMC_ERR_INVALID, "null argument");
+  const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
+  if (!X || !Y || !Z) return MDMC_OK;
+  std::vector<double> h((size_t)nblobs * (X + Y + Z + 1));
+  double *ex = h.data(), *ey = ex + (size_t)nblobs * X, *ez = ey + (size_t)nblobs * Y,
+         *amp = ez + (size_t)nblobs * Z;
+  for (uint32_t b = 0; b < nblobs; ++b) {
+    mdmc_blob bp = mdmc_blob_params(seed, b, dims, sigma_lo, sigma_hi, amp_lo, amp_hi);
+    mdmc_gauss_1d(ex + (size_t)b * X, X, bp.cx, bp.sigma);
+    mdmc_gauss_1d(ey + (size_t)b * Y, Y, bp.cy, bp.sigma);
+    mdmc_gauss_1d(ez + (size_t)b * Z, Z, bp.cz, bp.sigma);
+    amp[b] = bp.amp;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamSynchronize(st));  // the table buffer may still be in use
+  int rc = ensure_gen(ctx, std::max<size_t>(8, h.size() * sizeof(double)));
+  if (rc != MDMC_OK) return rc;
+  HIP_TRY(hipMemcpy(ctx->gen, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  double *d = static_cast<double *>(ctx->gen);
+  BlobArgs ba{d_out, X, Y, Z, nblobs, d, d + (size_t)nblobs * X, d + (size_t)nblobs * (X + Y),
+              d + (size_t)nblobs * (X + Y + Z)};
+  const uint64_t n = (uint64_t)X * Y * Z;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_field_blobs, dim3(blocks), dim3(256), 0, st, ba);
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+int mdmc_field_gyroid_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], double period,
+                             uint64_t seed, double noise_amp, void *stream) {
+  if (!ctx || !d_out || !dims) return fail(MDMC_ERR_INVALID, "null argument");
+  const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
+  if (!X || !Y || !Z) return MDMC_OK;
+  std::vector<double> h(2 * ((size_t)X + Y + Z));
+  double *sx = h.data(), *cx = sx + X, *sy = cx + X, *cy = sy + Y, *sz = cy + Y, *cz = sz + Z;
+  mdmc_trig_1d(sx, cx, X, period);
+  mdmc_trig_1d(sy, cy, Y, period);
+  mdmc_trig_1d(sz, cz, Z, period);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipStreamSynchronize(st));
+  int rc = ensure_gen(ctx, h.size() * sizeof(double));
+  if (rc != MDMC_OK) return rc;
+  HIP_TRY(hipMemcpy(ctx->gen, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  double *d = static_cast<double *>(ctx->gen);
+  GyroidArgs ga{d_out, X, Y, Z, d, d + X, d + 2 * X, d + 2 * X + Y, d + 2 * X + 2 * Y,
+                d + 2 * X + 2 * Y + Z, seed, noise_amp};
+  const uint64_t n = (uint64_t)X * Y * Z;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(k_field_gyroid, dim3(blocks), dim3(256), 0, st, ga);
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+}  // extern "C"

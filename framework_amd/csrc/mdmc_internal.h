@@ -1,0 +1,73 @@
+// Internal types of the mdmc host code: tile plan, scratch layout and the
+// dispatch of the classify kernel over its compile-time tile shapes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mdmc_kernels.hpp"
+
+namespace mdmc {
+
+// Tiling of one volume.  Cells are split into segments: one cell row of one
+// x-tile (64*W cells) in one plane, numbered xt + XT*(y + cy*z) so that
+// segment order is the reference's cell order (x fastest).
+struct Plan {
+  uint32_t X, Y, Z, cx, cy, cz;
+  uint32_t W, R, Kc, cap, XT, NRB, NKC, ntiles;
+  uint64_t nsegs, nslots, max_chunks, classify_blocks;
+  int empty;
+};
+
+struct Scratch {
+  uint2 *seg_counts;
+  uint2 *seg_vq;
+  uint32_t *ne_seg;
+  uint32_t *ne_ap;
+  uint32_t *chunk_ne;
+  unsigned long long *scan_state;
+  uint32_t *ticket;
+  Totals *totals;
+  uint64_t *slots;
+};
+
+int plan_tiling(const uint32_t dims[3], Plan *p);
+
+// Tile shapes with a compiled classify kernel.
+inline bool tiling_supported(uint32_t W, uint32_t R) {
+  const bool w_ok = W == 1 || W == 2 || W == 4 || W == 8 || W == 16;
+  const bool r_ok = R == 2 || R == 4 || R == 8 || (R == 12 && W <= 8);
+  return w_ok && r_ok;
+}
+
+template <int W, int R>
+inline void launch_classify_t(const ClassifyArgs &a, uint32_t blocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_classify<W, R>), dim3(blocks), dim3(64), 0, st, a);
+}
+
+template <int W>
+inline int launch_classify_w(uint32_t R, const ClassifyArgs &a, uint32_t blocks, hipStream_t st) {
+  switch (R) {
+    case 2: launch_classify_t<W, 2>(a, blocks, st); return 0;
+    case 4: launch_classify_t<W, 4>(a, blocks, st); return 0;
+    case 8: launch_classify_t<W, 8>(a, blocks, st); return 0;
+    case 12:
+      if constexpr (W <= 8) { launch_classify_t<W, 12>(a, blocks, st); return 0; }
+      return -1;
+    default: return -1;
+  }
+}
+
+inline int launch_classify(uint32_t W, uint32_t R, const ClassifyArgs &a, uint32_t blocks,
+                           hipStream_t st) {
+  switch (W) {
+    case 1: return launch_classify_w<1>(R, a, blocks, st);
+    case 2: return launch_classify_w<2>(R, a, blocks, st);
+    case 4: return launch_classify_w<4>(R, a, blocks, st);
+    case 8: return launch_classify_w<8>(R, a, blocks, st);
+    case 16: return launch_classify_w<16>(R, a, blocks, st);
+    default: return -1;
+  }
+}
+
+}  // namespace mdmc

@@ -1,0 +1,830 @@
+// Dual Marching Cubes kernels for gfx950 (MI355X).
+//
+// Pipeline for one volume (three stream-ordered launches, no host sync):
+//
+//   k_classify  dense pass over the field.  One wave owns a tile of R cell
+//               rows x one x-tile (64*W cells) and marches a z-chunk of planes.
+//               Each point is loaded once per tile (halo: one extra point row
+//               per R rows, one extra plane per chunk).  Lanes keep the sign
+//               bits (v < iso) of their point column for the tile's rows at
+//               planes k and k+1 in one 32-bit register; the neighbour column
+//               comes from lane+1.  A chunk whose 64 columns are uniformly in
+//               or out is skipped with two ballots.  Active cells get their
+//               2B/3B table variant, vertex and quad counts, and a 64-bit
+//               record written into a per-segment slot range (segment = one
+//               cell row of one x-tile in one plane), ranked with ballot +
+//               mbcnt and wave prefix sums.  Per-segment totals are written.
+//   k_scan      single-pass decoupled look-back scan (ticketed tiles) over the
+//               segment totals: vertex/quad offsets per segment, the list of
+//               non-empty segments and where each 64-record chunk starts.
+//   k_emit      sparse pass, one lane per active cell: edge crossing points,
+//               dual vertices (FP32, reference operation order) and quads.
+//               Ring neighbours are found by binary search in the neighbour
+//               segment's sorted records.
+//
+// Reference semantics: DMC_CUDA/DMC_CUDA/src/dmc.cu:1018-2208 (dmc::run_dmc).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace mdmc {
+
+constexpr int kWave = 64;
+constexpr uint32_t kStatusVertexOverflow = 1, kStatusQuadOverflow = 2, kStatusLookup = 4,
+                   kStatusScanTimeout = 8, kStatusInternal = 16;
+
+// Case tables in constant memory (uploaded by mdmc_setup):
+//   c_label[v][cfg]   12 nibbles, edge e -> iso-vertex label or 0xF
+//   c_cfg_info[cfg]   bits 0-2 NV variant 0, 3-5 NV variant 1, 6 = 2B/3B,
+//                     7-9 check direction (POS_X=0 .. NEG_Z=5)
+__constant__ uint64_t c_label[2][256];
+__constant__ uint16_t c_cfg_info[256];
+
+struct Totals {
+  uint32_t num_active, num_vertices, num_quads, num_nonempty;
+  uint32_t num_chunks, pad0, pad1, pad2;
+};
+
+struct FieldDesc {
+  const float *f;
+  uint32_t X, Y, Z;     // points
+  uint32_t cx, cy, cz;  // cells
+  float iso;
+};
+
+// Record of an active cell (64 bits):
+//   bits  0-9   x offset inside its x-tile
+//   bits 10-17  configuration
+//   bit  18     uses the variant-1 table (2B/3B correction)
+//   bits 19-31  first vertex, relative to the segment
+//   bits 32-43  first quad, relative to the segment
+__device__ __forceinline__ uint64_t make_record(uint32_t xl, uint32_t cfg, uint32_t lut2,
+                                                uint32_t lvb, uint32_t lqb) {
+  return (uint64_t)(xl | (cfg << 10) | (lut2 << 18) | (lvb << 19)) | ((uint64_t)lqb << 32);
+}
+__device__ __forceinline__ uint32_t rec_x(uint64_t r) { return (uint32_t)r & 0x3FFu; }
+__device__ __forceinline__ uint32_t rec_cfg(uint64_t r) { return ((uint32_t)r >> 10) & 0xFFu; }
+__device__ __forceinline__ uint32_t rec_lut2(uint64_t r) { return ((uint32_t)r >> 18) & 1u; }
+__device__ __forceinline__ uint32_t rec_lvb(uint64_t r) { return ((uint32_t)r >> 19) & 0x1FFFu; }
+__device__ __forceinline__ uint32_t rec_lqb(uint64_t r) { return (uint32_t)(r >> 32) & 0xFFFu; }
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
+// Remap so that each XCD works on a contiguous range of tiles: neighbouring
+// tiles (which share halo rows) then hit the same L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+  uint32_t q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = (int)lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, kWave);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Configuration of cell (x, y, z) read straight from the field (used for the
+// rare 2B/3B neighbour checks whose neighbour lies outside the tile window).
+__device__ __forceinline__ uint32_t cell_cfg_global(const FieldDesc &g, uint32_t x, uint32_t y,
+                                                    uint32_t z) {
+  const size_t sy = g.X, sz = (size_t)g.X * g.Y;
+  const float *p = g.f + (size_t)z * sz + (size_t)y * sy + x;
+  const float iso = g.iso;
+  uint32_t c = 0;
+  c |= (uint32_t)(p[0] < iso);
+  c |= (uint32_t)(p[1] < iso) << 1;
+  c |= (uint32_t)(p[sy + 1] < iso) << 2;
+  c |= (uint32_t)(p[sy] < iso) << 3;
+  c |= (uint32_t)(p[sz] < iso) << 4;
+  c |= (uint32_t)(p[sz + 1] < iso) << 5;
+  c |= (uint32_t)(p[sz + sy + 1] < iso) << 6;
+  c |= (uint32_t)(p[sz + sy] < iso) << 7;
+  return c;
+}
+
+// 2B/3B correction (dmc.cu:1345-1371): a cell whose configuration is 2B/3B
+// uses the variant-1 table when its in-grid neighbour across the listed face
+// is also 2B/3B.
+__device__ __forceinline__ uint32_t use_variant1(const FieldDesc &g, const uint16_t *info_tab,
+                                                 uint32_t info, uint32_t x, uint32_t y,
+                                                 uint32_t z) {
+  if (!(info & 64u)) return 0;
+  uint32_t dir = (info >> 7) & 7u;
+  int nx = (int)x, ny = (int)y, nz = (int)z;
+  switch (dir) {
+    case 0: nx += 1; break;
+    case 1: nx -= 1; break;
+    case 2: ny += 1; break;
+    case 3: ny -= 1; break;
+    case 4: nz += 1; break;
+    default: nz -= 1; break;
+  }
+  if (nx < 0 || ny < 0 || nz < 0 || nx >= (int)g.cx || ny >= (int)g.cy || nz >= (int)g.cz) return 0;
+  uint32_t ncfg = cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz);
+  return (info_tab[ncfg] >> 6) & 1u;
+}
+
+struct ClassifyArgs {
+  FieldDesc g;
+  uint32_t XT, NRB, NKC, Kc;
+  uint2 *seg_counts;      // per segment: {nact | nquad << 16, nvert}
+  uint64_t *slots;        // per segment: 64*W record slots
+  unsigned long long *scan_state;  // zeroed here for k_scan
+  uint32_t scan_state_words;
+  uint32_t *scan_ticket;
+  Totals *totals;
+  uint32_t *counts_out;   // user mdmc_counts (4 words), zeroed here
+};
+
+// One wave per tile; see the file header.
+template <int W, int R>
+__global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
+  static_assert(R >= 1 && R <= 15, "rows per tile must fit the 16-bit plane fields");
+  static_assert(W >= 1 && W <= 16, "x-tile is at most 1024 cells");
+  __shared__ uint16_t s_info[256];
+  const uint32_t lane = lane_id();
+  for (uint32_t i = lane; i < 256; i += kWave) s_info[i] = c_cfg_info[i];
+  if (blockIdx.x == 0) {
+    for (uint32_t i = lane; i < a.scan_state_words; i += kWave) a.scan_state[i] = 0ull;
+    if (lane < 8) reinterpret_cast<uint32_t *>(a.totals)[lane] = 0u;
+    if (lane == 0) *a.scan_ticket = 0u;
+    if (a.counts_out != nullptr && lane < 4) a.counts_out[lane] = 0u;
+  }
+  __syncthreads();
+
+  const FieldDesc g = a.g;
+  const float iso = g.iso;
+  constexpr uint32_t CAP = 64u * W;
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t xt = tile % a.XT;
+  const uint32_t rest = tile / a.XT;
+  const uint32_t rb = rest % a.NRB;
+  const uint32_t kc = rest / a.NRB;
+  const uint32_t x0 = xt * CAP;
+  const uint32_t j0 = rb * R;
+  const uint32_t k0 = kc * a.Kc;
+  const uint32_t k1 = min(k0 + a.Kc, g.cz);
+  if (k0 >= g.cz) return;
+  const size_t plane = (size_t)g.X * g.Y;
+  const bool has_extra = x0 + CAP < g.X;  // first point of the next x-tile
+  const float qnan = __int_as_float(0x7FC00000);
+
+  float vals[R + 1][W];
+  float vext[R + 1];
+
+  auto load_plane = [&](uint32_t z) {
+#pragma unroll
+    for (int r = 0; r <= R; ++r) {
+      const uint32_t y = j0 + r;
+      const bool row_ok = (z < g.Z) && (y < g.Y);
+      const float *rowp = g.f + (size_t)z * plane + (size_t)y * g.X;
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const uint32_t x = x0 + 64u * c + lane;
+        vals[r][c] = (row_ok && x < g.X) ? rowp[x] : qnan;
+      }
+      vext[r] = (row_ok && has_extra) ? rowp[x0 + CAP] : qnan;
+    }
+  };
+  auto plane_bits = [&](int c) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int r = 0; r <= R; ++r) b |= (uint32_t)(vals[r][c] < iso) << r;
+    return b;
+  };
+  auto ext_bits = [&]() {
+    uint32_t b = 0;
+#pragma unroll
+    for (int r = 0; r <= R; ++r) b |= (uint32_t)(vext[r] < iso) << r;
+    return b;
+  };
+
+  uint32_t B[W];
+  uint32_t E;
+  load_plane(k0);
+#pragma unroll
+  for (int c = 0; c < W; ++c) B[c] = plane_bits(c) << 16;
+  E = ext_bits() << 16;
+  load_plane(k0 + 1);
+
+  constexpr uint32_t rowmask = (1u << (R + 1)) - 1u;
+  constexpr uint32_t fullmask = rowmask | (rowmask << 16);
+
+  for (uint32_t k = k0; k < k1; ++k) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) B[c] = (B[c] >> 16) | (plane_bits(c) << 16);
+    E = (E >> 16) | (ext_bits() << 16);
+    if (k + 1 < k1) load_plane(k + 2);  // prefetch the plane the next step needs
+
+    // Neighbour columns (lane+1; lane 63 takes the next chunk's lane 0) and
+    // the chunks that are not uniformly in or out over all rows.
+    uint32_t N[W];
+    uint32_t live = 0;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      const uint32_t own = B[c];
+      uint32_t nb = __shfl_down(own, 1, kWave);
+      const uint32_t nxt = (c + 1 < W) ? (uint32_t)__shfl(B[(c + 1 < W) ? c + 1 : c], 0, kWave) : E;
+      if (lane == kWave - 1) nb = nxt;
+      N[c] = nb;
+      const uint32_t uo = own & fullmask, un = nb & fullmask;
+      const bool all_out = (uo | un) == 0u;
+      const bool all_in = (uo & un) == fullmask;
+      if (__ballot(!all_out) != 0ull && __ballot(!all_in) != 0ull) live |= 1u << c;
+    }
+
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+      const uint32_t y = j0 + r;
+      if (y >= g.cy) break;
+      uint32_t cnt_a = 0, cnt_v = 0, cnt_q = 0;
+      const uint32_t seg = xt + a.XT * (y + g.cy * k);
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        if (!((live >> c) & 1u)) continue;
+        const uint32_t own = B[c], nb = N[c];
+        const uint32_t x = x0 + 64u * c + lane;
+        const uint32_t t = ((own >> r) & 0x30003u) | (((nb >> r) & 0x30003u) << 2);
+        const uint32_t key = (t & 0xFu) | ((t >> 12) & 0xF0u);
+        const uint32_t cfg = (key & 0x11u) | ((key >> 1) & 0x66u) | ((key << 2) & 0x88u);
+        const bool active = (x < g.cx) && (cfg != 0u) && (cfg != 0xFFu);
+        const uint64_t bal = __ballot(active);
+        if (bal == 0ull) continue;
+        uint32_t nv = 0, nq = 0, lut2 = 0;
+        if (active) {
+          const uint32_t info = s_info[cfg];
+          lut2 = use_variant1(g, s_info, info, x, y, k);
+          const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
+          const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
+          const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
+          nv = (lut2 ? ((info >> 3) & 7u) : (info & 7u)) + b6 + b9 + b10;
+          const bool xin = x + 1 < g.cx, yin = y + 1 < g.cy, zin = k + 1 < g.cz;
+          nq = (b6 & (uint32_t)(xin && yin)) + (b9 & (uint32_t)(xin && zin)) +
+               (b10 & (uint32_t)(yin && zin));
+        }
+        const uint32_t packed = nv | (nq << 16);
+        const uint32_t incl = wave_incl_scan(packed);
+        const uint32_t excl = incl - packed;
+        const uint32_t tot = __shfl(incl, kWave - 1, kWave);
+        if (active) {
+          const uint32_t rank = cnt_a + mbcnt64(bal);
+          const uint32_t lvb = cnt_v + (excl & 0xFFFFu);
+          const uint32_t lqb = cnt_q + (excl >> 16);
+          a.slots[(size_t)seg * CAP + rank] = make_record(x - x0, cfg, lut2, lvb, lqb);
+        }
+        cnt_a += (uint32_t)__popcll(bal);
+        cnt_v += tot & 0xFFFFu;
+        cnt_q += tot >> 16;
+      }
+      if (lane == 0) a.seg_counts[seg] = make_uint2(cnt_a | (cnt_q << 16), cnt_v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Decoupled look-back scan over segments.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanIpt = 8;
+constexpr int kScanTile = kScanThreads * kScanIpt;
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+struct ScanArgs {
+  const uint2 *seg_counts;
+  uint32_t nsegs;
+  uint32_t ntiles;
+  uint2 *seg_vq;       // per segment: {first vertex, first quad}
+  uint32_t *ne_seg;    // non-empty segments in order
+  uint32_t *ne_ap;     // their first active index (sentinel = A at [NE])
+  uint32_t *chunk_ne;  // per 64-record chunk: index into ne_* of its first record
+  unsigned long long *state;  // 4 words per tile: (status << 32) | value
+  uint32_t *ticket;
+  Totals *totals;
+  uint32_t *counts_out;  // optional user mdmc_counts
+  uint64_t max_chunks;   // capacity of chunk_ne
+};
+
+__device__ __forceinline__ unsigned long long ld_state(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_state(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint4 s_wave[kScanThreads / kWave];
+  __shared__ uint4 s_prefix;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
+  __syncthreads();
+  const uint32_t t = s_tile;
+  const uint32_t base = t * kScanTile + tid * kScanIpt;
+
+  uint32_t ia[kScanIpt], iv[kScanIpt], iq[kScanIpt];
+  uint4 sum = make_uint4(0, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < kScanIpt; ++i) {
+    const uint32_t s = base + i;
+    uint2 c = (s < a.nsegs) ? a.seg_counts[s] : make_uint2(0, 0);
+    ia[i] = c.x & 0xFFFFu;
+    iq[i] = c.x >> 16;
+    iv[i] = c.y;
+    sum.x += ia[i];
+    sum.y += iv[i];
+    sum.z += iq[i];
+    sum.w += ia[i] ? 1u : 0u;
+  }
+  uint4 inc;
+  inc.x = wave_incl_scan(sum.x);
+  inc.y = wave_incl_scan(sum.y);
+  inc.z = wave_incl_scan(sum.z);
+  inc.w = wave_incl_scan(sum.w);
+  if (lane == kWave - 1) s_wave[wave] = inc;
+  __syncthreads();
+  uint4 woff = make_uint4(0, 0, 0, 0), agg = make_uint4(0, 0, 0, 0);
+#pragma unroll
+  for (int w = 0; w < kScanThreads / kWave; ++w) {
+    const uint4 s = s_wave[w];
+    if ((uint32_t)w < wave) { woff.x += s.x; woff.y += s.y; woff.z += s.z; woff.w += s.w; }
+    agg.x += s.x; agg.y += s.y; agg.z += s.z; agg.w += s.w;
+  }
+
+  if (wave == 0) {
+    unsigned long long *my = a.state + 4ull * t;
+    uint4 excl = make_uint4(0, 0, 0, 0);
+    const unsigned long long AGG = 1ull << 32, INC = 2ull << 32;
+    if (t == 0) {
+      if (lane == 0) {
+        st_state(my + 0, INC | agg.x); st_state(my + 1, INC | agg.y);
+        st_state(my + 2, INC | agg.z); st_state(my + 3, INC | agg.w);
+      }
+    } else {
+      if (lane == 0) {
+        st_state(my + 0, AGG | agg.x); st_state(my + 1, AGG | agg.y);
+        st_state(my + 2, AGG | agg.z); st_state(my + 3, AGG | agg.w);
+      }
+      // Each channel word is self-describing (status, value) and a tile flips
+      // its four words from aggregate to inclusive one at a time, so every
+      // channel is resolved on its own: sum aggregates back to the nearest
+      // tile whose word in *that* channel is inclusive.
+      int64_t look = (int64_t)t - 1;
+      bool timed_out = false;
+      uint32_t done = 0;  // bit c: channel c resolved
+      while (done != 0xFu) {
+        const int64_t idx = look - (int64_t)lane;
+        unsigned long long w[4] = {INC, INC, INC, INC};  // before tile 0: inclusive zeros
+        if (idx >= 0) {
+          const unsigned long long *p = a.state + 4ull * (uint64_t)idx;
+          uint32_t spins = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            if ((done >> c) & 1u) continue;
+            do { w[c] = ld_state(p + c); } while ((w[c] >> 32) == 0 && ++spins < kSpinLimit);
+          }
+          if (spins >= kSpinLimit) timed_out = true;
+        }
+        uint32_t add[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          add[c] = 0;
+          if ((done >> c) & 1u) continue;
+          const uint64_t incmask = __ballot((w[c] >> 32) == 2);
+          const int stop = incmask ? (__ffsll((long long)incmask) - 1) : kWave;
+          const uint32_t val = ((int)lane <= stop && idx >= 0) ? (uint32_t)w[c] : 0u;
+          add[c] = __shfl(wave_incl_scan(val), kWave - 1, kWave);
+          if (incmask) done |= 1u << c;
+        }
+        excl.x += add[0]; excl.y += add[1]; excl.z += add[2]; excl.w += add[3];
+        look -= kWave;
+      }
+      if (__ballot(timed_out) && lane == 0 && a.counts_out)
+        atomicOr(a.counts_out + 3, kStatusScanTimeout);
+      if (lane == 0) {
+        st_state(my + 0, INC | (uint32_t)(excl.x + agg.x));
+        st_state(my + 1, INC | (uint32_t)(excl.y + agg.y));
+        st_state(my + 2, INC | (uint32_t)(excl.z + agg.z));
+        st_state(my + 3, INC | (uint32_t)(excl.w + agg.w));
+      }
+    }
+    if (lane == 0) s_prefix = excl;
+  }
+  __syncthreads();
+  const uint4 tp = s_prefix;
+  // exclusive prefix of this thread's first item
+  uint32_t pa = tp.x + woff.x + inc.x - sum.x;
+  uint32_t pv = tp.y + woff.y + inc.y - sum.y;
+  uint32_t pq = tp.z + woff.z + inc.z - sum.z;
+  uint32_t pe = tp.w + woff.w + inc.w - sum.w;
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < kScanIpt; ++i) {
+    const uint32_t s = base + i;
+    if (s < a.nsegs) {
+      a.seg_vq[s] = make_uint2(pv, pq);
+      if (ia[i]) {
+        if (pe < a.nsegs) {
+          a.ne_seg[pe] = s;
+          a.ne_ap[pe] = pa;
+        } else {
+          bad = true;
+        }
+        for (uint32_t gch = (pa + 63u) >> 6; (gch << 6) < pa + ia[i]; ++gch) {
+          if (gch < a.max_chunks) a.chunk_ne[gch] = pe; else bad = true;
+        }
+        ++pe;
+      }
+    }
+    pa += ia[i];
+    pv += iv[i];
+    pq += iq[i];
+  }
+  if (t == a.ntiles - 1 && tid == kScanThreads - 1) {
+    // pa/pv/pq/pe now hold the grand totals
+    Totals tot;
+    tot.num_active = pa;
+    tot.num_vertices = pv;
+    tot.num_quads = pq;
+    tot.num_nonempty = pe;
+    tot.num_chunks = (pa + 63u) >> 6;
+    tot.pad0 = tot.pad1 = tot.pad2 = 0;
+    *a.totals = tot;
+    if (pe <= a.nsegs) a.ne_ap[pe] = pa; else bad = true;
+    if (a.counts_out) {
+      a.counts_out[0] = pa;
+      a.counts_out[1] = pv;
+      a.counts_out[2] = pq;
+    }
+  }
+  if (bad && a.counts_out) atomicOr(a.counts_out + 3, kStatusInternal);
+}
+
+// ---------------------------------------------------------------------------
+// Emit: vertices and quads, one lane per active cell.
+
+constexpr int kEmitThreads = 256;
+
+struct EmitArgs {
+  FieldDesc g;
+  float mnx, mny, mnz, rgx, rgy, rgz;  // xyz_min and xyz_max - xyz_min
+  uint32_t XT, cap;
+  uint32_t nsegs;
+  uint64_t max_chunks;  // capacity of chunk_ne
+  const uint2 *seg_counts;
+  const uint2 *seg_vq;
+  const uint64_t *slots;
+  const uint32_t *ne_seg;
+  const uint32_t *ne_ap;
+  const uint32_t *chunk_ne;
+  const Totals *totals;
+  float *verts;
+  uint64_t vcap;
+  int32_t *quads;
+  uint64_t qcap;
+  uint32_t *status;
+};
+
+// utils::ijk_to_xyz (include/utils.hpp:118-122): i / n * range + min, three
+// correctly rounded float operations.
+__device__ __forceinline__ float coordf(uint32_t i, uint32_t n, float range, float mn) {
+  return (float)i / (float)n * range + mn;
+}
+
+// Position of `rank`-th record of segment s holding x offset xl, or -1.
+__device__ __forceinline__ int64_t find_in_segment(const EmitArgs &a, uint32_t s, uint32_t xl) {
+  const uint32_t n = min(a.seg_counts[s].x & 0xFFFFu, a.cap);
+  const uint64_t *p = a.slots + (size_t)s * a.cap;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rec_x(p[mid]) < xl) lo = mid + 1; else hi = mid;
+  }
+  if (lo < n && rec_x(p[lo]) == xl) return (int64_t)s * a.cap + lo;
+  return -1;
+}
+
+// Slot of the cell one step in +x from the cell at slot `pos` (segment s,
+// x offset xl): the next record of the segment, or the first record of the
+// next x-tile's segment.
+__device__ __forceinline__ int64_t next_x(const EmitArgs &a, uint32_t s, int64_t pos, uint32_t xl) {
+  if (pos < 0) return -1;
+  if (xl + 1 < a.cap) {
+    const uint32_t n = min(a.seg_counts[s].x & 0xFFFFu, a.cap);
+    const int64_t q = pos + 1;
+    if ((uint64_t)(q - (int64_t)s * a.cap) < n && rec_x(a.slots[q]) == xl + 1) return q;
+    return -1;
+  }
+  const uint32_t s2 = s + 1;
+  if ((a.seg_counts[s2].x & 0xFFFFu) && rec_x(a.slots[(size_t)s2 * a.cap]) == 0) return (int64_t)s2 * a.cap;
+  return -1;
+}
+
+// Vertex index of edge `e` of the cell at slot `pos`.
+__device__ __forceinline__ int32_t ring_vertex(const EmitArgs &a, const uint64_t *lab, int64_t pos,
+                                               int e, bool &ok) {
+  if (pos < 0) { ok = false; return -1; }
+  const uint64_t r = a.slots[pos];
+  const uint32_t s = (uint32_t)(pos / a.cap);
+  const uint32_t m = (uint32_t)(lab[rec_lut2(r) * 256 + rec_cfg(r)] >> (4 * e)) & 0xFu;
+  if (m == 0xFu) { ok = false; return -1; }
+  return (int32_t)(a.seg_vq[s].x + rec_lvb(r) + m);
+}
+
+__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
+  __shared__ uint64_t s_lab[2 * 256];
+  for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
+  __syncthreads();
+
+  const Totals tot = *a.totals;
+  const uint32_t lane = lane_id();
+  const uint32_t waves_per_block = kEmitThreads / kWave;
+  const uint32_t wave = threadIdx.x / kWave;
+  const FieldDesc g = a.g;
+  const float iso = g.iso;
+  const size_t sy = g.X, sz = (size_t)g.X * g.Y;
+  // Counts are clamped to buffer capacities so that no access can leave the
+  // scratch buffers even if the count pass produced inconsistent data.
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
+
+  for (uint32_t ch = blockIdx.x * waves_per_block + wave; ch < num_chunks;
+       ch += gridDim.x * waves_per_block) {
+    const uint32_t act = ch * 64u + lane;
+    const uint32_t u0 = a.chunk_ne[ch];
+    const uint32_t wl = (u0 < num_ne && u0 + lane < num_ne) ? a.ne_ap[u0 + lane] : 0xFFFFFFFFu;
+    // Segment of this lane's record: the last window entry starting at or
+    // before `act` (all lanes take part in the shuffles).
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+      const uint32_t cand = m + step;
+      const uint32_t wc = __shfl(wl, (int)(cand & 63u), kWave);
+      if (cand < 64u && wc <= act) m = cand;
+    }
+    const uint32_t ap = __shfl(wl, (int)m, kWave);
+    if (act >= tot.num_active) continue;
+    // Inputs come from the count pass; a broken invariant is reported in the
+    // status word instead of being followed out of bounds.
+    if (u0 >= num_ne || u0 + m >= num_ne) { atomicOr(a.status, kStatusInternal); continue; }
+    const uint32_t s = a.ne_seg[u0 + m];
+    const uint32_t rank = act - ap;
+    if (s >= a.nsegs || act < ap || rank >= min(a.seg_counts[s].x & 0xFFFFu, a.cap)) {
+      atomicOr(a.status, kStatusInternal);
+      continue;
+    }
+    const int64_t pos = (int64_t)s * a.cap + rank;
+    const uint64_t rec = a.slots[pos];
+    const uint32_t xl = rec_x(rec), cfg = rec_cfg(rec), lut2 = rec_lut2(rec);
+    const uint32_t xt = s % a.XT, row = s / a.XT;
+    const uint32_t j = row % g.cy, k = row / g.cy;
+    const uint32_t x = xt * a.cap + xl;
+    if (x >= g.cx || xl >= a.cap) { atomicOr(a.status, kStatusInternal); continue; }
+    const uint2 vq = a.seg_vq[s];
+    const uint32_t vb = vq.x + rec_lvb(rec);
+    const uint32_t qb = vq.y + rec_lqb(rec);
+
+    const float *p = g.f + (size_t)k * sz + (size_t)j * sy + x;
+    float v[8];
+    v[0] = p[0];
+    v[1] = p[1];
+    v[2] = p[sy + 1];
+    v[3] = p[sy];
+    v[4] = p[sz];
+    v[5] = p[sz + 1];
+    v[6] = p[sz + sy + 1];
+    v[7] = p[sz + sy];
+
+    const float X0 = coordf(x, g.cx, a.rgx, a.mnx), X1 = coordf(x + 1, g.cx, a.rgx, a.mnx);
+    const float Y0 = coordf(j, g.cy, a.rgy, a.mny), Y1 = coordf(j + 1, g.cy, a.rgy, a.mny);
+    const float Z0 = coordf(k, g.cz, a.rgz, a.mnz), Z1 = coordf(k + 1, g.cz, a.rgz, a.mnz);
+
+    // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
+    // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
+    // point is bitwise the one the owner writes (dmc.cu:1468-1514).
+    constexpr uint8_t EF[12] = {1, 1, 2, 0, 0, 1, 2, 3, 5, 5, 6, 4};
+    constexpr uint8_t ET[12] = {0, 2, 3, 3, 4, 5, 6, 7, 4, 6, 7, 7};
+    // Owner of each edge lies at a non-positive offset; an edge is skipped
+    // when its owner would be outside the grid (dmc.cu:1539-1545).
+    // bit 0: needs x>0, bit 1: needs y>0, bit 2: needs z>0
+    constexpr uint8_t NEED[12] = {6, 4, 4, 5, 3, 2, 0, 1, 2, 0, 0, 1};
+    const uint32_t have = (x > 0 ? 1u : 0u) | (j > 0 ? 2u : 0u) | (k > 0 ? 4u : 0u);
+
+    const uint64_t labels = s_lab[lut2 * 256 + cfg];
+    float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    float ex[3] = {0, 0, 0}, ey[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+      const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
+      if (lab == 0xFu) continue;
+      const int f = EF[e], to = ET[e];
+      const float vf = v[f], vt = v[to];
+      const float tt = (iso - vf) / (vt - vf);
+      const float omt = 1.0f - tt;
+      const float fx = (f == 1 || f == 2 || f == 5 || f == 6) ? X1 : X0;
+      const float fy = (f == 2 || f == 3 || f == 6 || f == 7) ? Y1 : Y0;
+      const float fz = (f >= 4) ? Z1 : Z0;
+      const float tx = (to == 1 || to == 2 || to == 5 || to == 6) ? X1 : X0;
+      const float ty = (to == 2 || to == 3 || to == 6 || to == 7) ? Y1 : Y0;
+      const float tz = (to >= 4) ? Z1 : Z0;
+      const float px = fx * omt + tx * tt;
+      const float py = fy * omt + ty * tt;
+      const float pz = fz * omt + tz * tt;
+      if (e == 6) { ex[0] = px; ey[0] = py; ez[0] = pz; }
+      if (e == 9) { ex[1] = px; ey[1] = py; ez[1] = pz; }
+      if (e == 10) { ex[2] = px; ey[2] = py; ez[2] = pz; }
+      if ((NEED[e] & have) != NEED[e]) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (lab == (uint32_t)q) {
+          if (cnt[q] == 0) { ax[q] = px; ay[q] = py; az[q] = pz; }
+          else { ax[q] = ax[q] + px; ay[q] = ay[q] + py; az[q] = az[q] + pz; }
+          ++cnt[q];
+        }
+      }
+    }
+    const uint16_t info = c_cfg_info[cfg];
+    const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
+    const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
+    const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
+    const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
+    const uint32_t nv = niso + b6 + b9 + b10;
+    if ((uint64_t)vb + nv <= a.vcap) {
+      float *out = a.verts + 3ull * vb;
+      const float qnan = __int_as_float(0x7FC00000);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if ((uint32_t)q < niso) {
+          const float d = (float)cnt[q];
+          out[3 * q + 0] = cnt[q] ? ax[q] / d : qnan;
+          out[3 * q + 1] = cnt[q] ? ay[q] / d : qnan;
+          out[3 * q + 2] = cnt[q] ? az[q] / d : qnan;
+        }
+      }
+      uint32_t w = niso;
+      if (b6) { out[3 * w] = ex[0]; out[3 * w + 1] = ey[0]; out[3 * w + 2] = ez[0]; ++w; }
+      if (b9) { out[3 * w] = ex[1]; out[3 * w + 1] = ey[1]; out[3 * w + 2] = ez[1]; ++w; }
+      if (b10) { out[3 * w] = ex[2]; out[3 * w + 1] = ey[2]; out[3 * w + 2] = ez[2]; ++w; }
+    } else {
+      atomicOr(a.status, kStatusVertexOverflow);
+    }
+
+    // Quads (dmc.cu:1983-2047): rings in ccw order, reversed (0,3,2,1) when
+    // the owned edge's first corner is not above iso.
+    const bool xin = x + 1 < g.cx, yin = j + 1 < g.cy, zin = k + 1 < g.cz;
+    const bool e6 = b6 && xin && yin, e9 = b9 && xin && zin, e10 = b10 && yin && zin;
+    if (!(e6 || e9 || e10)) continue;
+    const uint32_t nq = (uint32_t)e6 + (uint32_t)e9 + (uint32_t)e10;
+    if ((uint64_t)qb + nq > a.qcap) { atomicOr(a.status, kStatusQuadOverflow); continue; }
+    const int32_t self = (int32_t)vb;
+    const int64_t p_x = (e6 || e9) ? next_x(a, s, pos, xl) : -1;
+    const int64_t p_y = (e6 || e10) ? find_in_segment(a, s + a.XT, xl) : -1;
+    const int64_t p_z = (e9 || e10) ? find_in_segment(a, s + a.XT * g.cy, xl) : -1;
+    bool ok = true;
+    int4 *qo = reinterpret_cast<int4 *>(a.quads) + qb;
+    uint32_t qn = 0;
+    if (e6) {
+      const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
+      const int32_t m1 = ring_vertex(a, s_lab, p_x, 7, ok);
+      const int32_t m2 = ring_vertex(a, s_lab, next_x(a, s + a.XT, p_y, xl), 4, ok);
+      const int32_t m3 = ring_vertex(a, s_lab, p_y, 5, ok);
+      const bool ccw = v[2] <= iso;
+      qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+    }
+    if (e9) {
+      const int32_t m0 = self + (int32_t)((labels >> 36) & 0xFu);
+      const int32_t m1 = ring_vertex(a, s_lab, p_z, 1, ok);
+      const int32_t m2 = ring_vertex(a, s_lab, next_x(a, s + a.XT * g.cy, p_z, xl), 3, ok);
+      const int32_t m3 = ring_vertex(a, s_lab, p_x, 11, ok);
+      const bool ccw = v[5] <= iso;
+      qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+    }
+    if (e10) {
+      const int32_t m0 = self + (int32_t)((labels >> 40) & 0xFu);
+      const int32_t m1 = ring_vertex(a, s_lab, p_y, 8, ok);
+      const int32_t m2 = ring_vertex(a, s_lab, find_in_segment(a, s + a.XT + a.XT * g.cy, xl), 0, ok);
+      const int32_t m3 = ring_vertex(a, s_lab, p_z, 2, ok);
+      const bool ccw = v[7] <= iso;
+      qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+    }
+    if (!ok) atomicOr(a.status, kStatusLookup);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Compact per-cell records into ascending order (parity and debugging).
+
+struct CellsArgs {
+  uint32_t XT, cap, cx, cy, nsegs;
+  uint64_t max_chunks;
+  const uint2 *seg_counts;
+  const uint64_t *slots;
+  const uint2 *seg_vq;
+  const uint32_t *ne_seg;
+  const uint32_t *ne_ap;
+  const uint32_t *chunk_ne;
+  const Totals *totals;
+  uint32_t *cell_index;
+  uint8_t *cfg;
+  uint8_t *flags;
+  uint32_t *vbegin;
+  uint64_t capacity;
+};
+
+__global__ __launch_bounds__(256) void k_cells(CellsArgs a) {
+  const Totals tot = *a.totals;
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
+  for (uint32_t ch = blockIdx.x * 4 + wave; ch < num_chunks; ch += gridDim.x * 4) {
+    const uint32_t act = ch * 64u + lane;
+    const uint32_t u0 = a.chunk_ne[ch];
+    const uint32_t wl = (u0 < num_ne && u0 + lane < num_ne) ? a.ne_ap[u0 + lane] : 0xFFFFFFFFu;
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+      const uint32_t cand = m + step;
+      const uint32_t wc = __shfl(wl, (int)(cand & 63u), kWave);
+      if (cand < 64u && wc <= act) m = cand;
+    }
+    const uint32_t ap = __shfl(wl, (int)m, kWave);
+    if (act >= tot.num_active || act >= a.capacity) continue;
+    if (u0 >= num_ne || u0 + m >= num_ne) continue;
+    const uint32_t s = a.ne_seg[u0 + m];
+    if (s >= a.nsegs || act < ap || act - ap >= min(a.seg_counts[s].x & 0xFFFFu, a.cap)) continue;
+    const uint64_t rec = a.slots[(size_t)s * a.cap + (act - ap)];
+    const uint32_t xt = s % a.XT, row = s / a.XT;
+    const uint32_t x = xt * a.cap + rec_x(rec);
+    a.cell_index[act] = x + a.cx * row;
+    a.cfg[act] = (uint8_t)rec_cfg(rec);
+    a.flags[act] = (uint8_t)rec_lut2(rec);
+    a.vbegin[act] = a.seg_vq[s].x + rec_lvb(rec);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic fields (definitions in mdmc_fields.h).
+
+struct BlobArgs {
+  float *out;
+  uint32_t X, Y, Z, nblobs;
+  const double *ex, *ey, *ez, *amp;  // [nblobs][X], [nblobs][Y], [nblobs][Z], [nblobs]
+};
+
+__global__ __launch_bounds__(256) void k_field_blobs(BlobArgs a) {
+  const size_t n = (size_t)a.X * a.Y * a.Z;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t x = (uint32_t)(id % a.X);
+    const size_t r = id / a.X;
+    const uint32_t y = (uint32_t)(r % a.Y), z = (uint32_t)(r / a.Y);
+    double acc = 0.0;
+    for (uint32_t b = 0; b < a.nblobs; ++b)
+      acc = acc + ((a.amp[b] * a.ex[(size_t)b * a.X + x]) * a.ey[(size_t)b * a.Y + y]) *
+                      a.ez[(size_t)b * a.Z + z];
+    a.out[id] = (float)acc;
+  }
+}
+
+struct GyroidArgs {
+  float *out;
+  uint32_t X, Y, Z;
+  const double *sx, *cx, *sy, *cy, *sz, *cz;
+  uint64_t seed;
+  double amp;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_field_gyroid(GyroidArgs a) {
+  const size_t n = (size_t)a.X * a.Y * a.Z;
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
+       id += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t x = (uint32_t)(id % a.X);
+    const size_t r = id / a.X;
+    const uint32_t y = (uint32_t)(r % a.Y), z = (uint32_t)(r / a.Y);
+    const double gval = a.sx[x] * a.cy[y] + a.sy[y] * a.cz[z] + a.sz[z] * a.cx[x];
+    const uint64_t h = mix64(a.seed + ((uint64_t)id + 1) * 0x9E3779B97F4A7C15ull);
+    const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+    a.out[id] = (float)(gval + a.amp * (2.0 * u - 1.0));
+  }
+}
+
+}  // namespace mdmc

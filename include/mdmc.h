@@ -1,0 +1,154 @@
+/*
+ * mdmc — MI355X-native Dual Marching Cubes, C ABI.
+ *
+ * Drop-in for the extraction path of the reference DMC_CUDA library
+ * (DMC_CUDA/DMC_CUDA/include/dmc.hpp:40-57): grid dims, scalar-field pointer
+ * and isovalue in; vertex array and quad index list out.  Plain C types only;
+ * HIP streams are passed as `void*` (a hipStream_t, NULL = default stream).
+ *
+ * Output layout (identical to the reference, dmc.cu:901-921, 1478-1513,
+ * 2011-2047): active cells in ascending cell index; each contributes its iso
+ * (dual) vertices by table label, then the crossing points of its bipolar
+ * owned edges 6, 9, 10 in that order.  Quads are int32 x4 vertex indices; the
+ * reference's triangles are (q0,q1,q2),(q2,q3,q0).  Vertex xyz follows
+ * utils::ijk_to_xyz (include/utils.hpp:118-122): i / cells * (max-min) + min.
+ *
+ * Every entry point returns an MDMC_* status; none calls exit().
+ */
+#ifndef MDMC_H
+#define MDMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MDMC_OK = 0,
+  MDMC_ERR_INVALID = -1,   /* bad argument (null pointer, dims) */
+  MDMC_ERR_HIP = -2,       /* a HIP runtime call failed */
+  MDMC_ERR_NOMEM = -3,     /* device or host allocation failed */
+  MDMC_ERR_CAPACITY = -4,  /* caller-provided output buffer too small */
+  MDMC_ERR_TOO_LARGE = -5, /* grid beyond the 32-bit index range */
+  MDMC_ERR_NOT_SETUP = -6, /* mdmc_setup() not called */
+  MDMC_ERR_INTERNAL = -7,  /* a device-side invariant check failed */
+  MDMC_ERR_REJECTED = -8   /* request larger than the serving memory budget */
+};
+
+/* Device-side status bits reported in mdmc_counts.status. */
+enum {
+  MDMC_STATUS_VERTEX_OVERFLOW = 1, /* vertices beyond vertex_capacity dropped */
+  MDMC_STATUS_QUAD_OVERFLOW = 2,   /* quads beyond quad_capacity dropped */
+  MDMC_STATUS_LOOKUP_FAILED = 4,   /* a ring neighbour was not found */
+  MDMC_STATUS_SCAN_TIMEOUT = 8,    /* look-back spin limit hit */
+  MDMC_STATUS_INTERNAL = 16        /* inconsistent count-pass data (bug) */
+};
+
+/* Sizes of one extraction.  Written by the device; read after a stream sync. */
+typedef struct mdmc_counts {
+  uint32_t num_active;   /* cells with 0 < config < 255 */
+  uint32_t num_vertices; /* iso vertices + owned edge points */
+  uint32_t num_quads;
+  uint32_t status;       /* MDMC_STATUS_* bits */
+} mdmc_counts;
+
+/* Host-side mesh returned by mdmc_run (free with mdmc_mesh_free). */
+typedef struct mdmc_mesh {
+  float *vertices;   /* 3 * num_vertices */
+  int32_t *quads;    /* 4 * num_quads */
+  uint32_t num_vertices;
+  uint32_t num_quads;
+  uint32_t num_active;
+} mdmc_mesh;
+
+/* ---- reference-compatible entry points ---------------------------------- */
+
+/* Replaces dmc::setup_device_luts() (include/dmc.hpp:40, dmc.cu:713-779):
+ * builds the case tables and uploads them to device constant memory on the
+ * current device.  Idempotent. */
+int mdmc_setup(void);
+
+/* Replaces dmc::cleanup_device_luts() (include/dmc.hpp:48, dmc.cu:781-811):
+ * releases the library's cached device state.  Idempotent. */
+int mdmc_cleanup(void);
+
+/* Replaces dmc::run_dmc(vertices, triangles, grid, xyz_min, xyz_max, iso,
+ * num_smooth) (include/dmc.hpp:56-57, dmc.cu:2069-2208).  `field` is host
+ * memory, dims[3] are point counts (x fastest).  num_smooth must be 0
+ * (smoothing is not provided; non-zero returns MDMC_ERR_INVALID).  The mesh
+ * is allocated by the library. */
+int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
+             const float xyz_max[3], float iso, uint32_t num_smooth, mdmc_mesh *out);
+
+void mdmc_mesh_free(mdmc_mesh *mesh);
+
+const char *mdmc_status_string(int status);
+
+/* Message of the last failing call on this thread. */
+const char *mdmc_last_error(void);
+
+/* ---- device-resident path (no counterpart in the reference, whose only
+ * entry copies host memory every call; this is the same computation with the
+ * field and outputs resident in HBM and stream-ordered launches) ---------- */
+
+typedef struct mdmc_ctx mdmc_ctx;
+
+/* A context owns scratch buffers on the current device; one context per
+ * concurrent stream/thread. */
+int mdmc_ctx_create(mdmc_ctx **ctx);
+void mdmc_ctx_destroy(mdmc_ctx *ctx);
+
+/* Classify + scan.  Counts are written to device memory `d_counts`; nothing
+ * is synchronised. */
+int mdmc_count_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3], float iso,
+                      mdmc_counts *d_counts, void *stream);
+
+/* Emit vertices and quads for the extraction last counted on `ctx` (same
+ * field, dims, iso, stream).  Writes beyond a capacity are dropped and flagged
+ * in d_counts->status. */
+int mdmc_emit_device(mdmc_ctx *ctx, const float *d_field, const float xyz_min[3],
+                     const float xyz_max[3], float *d_vertices, uint64_t vertex_capacity,
+                     int32_t *d_quads, uint64_t quad_capacity, mdmc_counts *d_counts,
+                     void *stream);
+
+/* mdmc_count_device followed by mdmc_emit_device. */
+int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3],
+                        const float xyz_min[3], const float xyz_max[3], float iso,
+                        float *d_vertices, uint64_t vertex_capacity, int32_t *d_quads,
+                        uint64_t quad_capacity, mdmc_counts *d_counts, void *stream);
+
+/* Per-active-cell records of the last count on `ctx`, in ascending cell
+ * order: cell index1D, config, flags (bit 0 = uses the variant-1 table) and
+ * vertex_begin.  `capacity` entries at most.  For parity checks against the
+ * reference's _VoxelInfo records (dmc.cu:814-1006). */
+int mdmc_cells_device(mdmc_ctx *ctx, uint32_t *d_cell_index, uint8_t *d_cfg, uint8_t *d_flags,
+                      uint32_t *d_vertex_begin, uint64_t capacity, void *stream);
+
+/* ---- per-kernel timing (no counterpart in the reference, which has no
+ * timers).  While enabled, every kernel the context launches is bracketed by
+ * HIP events on its stream.  Kernel ids: 0 classify, 1 scan, 2 emit, 3 other. */
+int mdmc_ctx_profile(mdmc_ctx *ctx, int enable);
+/* Synchronises the recorded events, returns the summed milliseconds and the
+ * launch count per kernel id, and clears the record. */
+int mdmc_ctx_profile_read(mdmc_ctx *ctx, double ms_total[4], uint32_t launches[4]);
+
+/* ---- synthetic fields on the device ------------------------------------- */
+
+/* Sum of `nblobs` Gaussian blobs (centres uniform in the grid, sigma uniform
+ * in [sigma_lo, sigma_hi], amplitude uniform in [amp_lo, amp_hi]) from a
+ * SplitMix64 stream `seed`. */
+int mdmc_field_blobs_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], uint64_t seed,
+                            uint32_t nblobs, double sigma_lo, double sigma_hi, double amp_lo,
+                            double amp_hi, void *stream);
+
+/* sin(wx)cos(wy)+sin(wy)cos(wz)+sin(wz)cos(wx), w = 2 pi / period, plus
+ * uniform hash noise in [-noise_amp, noise_amp). */
+int mdmc_field_gyroid_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], double period,
+                             uint64_t seed, double noise_amp, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDMC_H */

@@ -1,0 +1,188 @@
+"""HIP path vs the CPU oracle on identical float bits (GPU).
+
+Bar: case indices, per-cell vertex counts/offsets and quads bit-exact; vertex
+positions bitwise equal to the oracle's FP32 sequence (which implies the
+1e-5 voxel-unit tolerance of the brief).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import assert_same_mesh, check_against_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    torch.cuda.set_device(0)
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_cuda):
+    import framework_amd as fa
+
+    fa.setup()
+    c = fa.Context()
+    yield c
+    c.close()
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def test_sphere_64(ctx, torch_cuda):
+    """Config 0: 64^3 signed distance to a sphere r=20 at (32,32,32), iso 0."""
+    f = oracle.field_sphere((64, 64, 64), (32.0, 32.0, 32.0), 20.0)
+    ref = check_against_oracle(ctx, _dev(torch_cuda, f), 0.0)
+    assert (ref["num_active"], ref["num_vertices"], ref["num_quads"]) == (7472, 14942, 7470)
+    assert ref["num_tie_edges"] == 72
+
+
+@pytest.mark.parametrize("dims,seed,nblobs,iso", [
+    ((96, 80, 72), 1, 12, 0.1),
+    ((130, 70, 33), 2, 8, 0.05),
+    ((64, 64, 64), 3, 20, -0.1),
+    ((200, 17, 9), 4, 6, 0.0),
+])
+def test_blobs_small(ctx, torch_cuda, dims, seed, nblobs, iso):
+    f = ctx.field_blobs(dims, seed=seed, nblobs=nblobs, sigma=(3.0, 12.0))
+    torch_cuda.cuda.synchronize()
+    check_against_oracle(ctx, f, iso)
+
+
+@pytest.mark.parametrize("dims,iso", [((96, 96, 96), 0.0), ((77, 50, 41), 0.15), ((33, 129, 20), -0.2)])
+def test_gyroid_noise(ctx, torch_cuda, dims, iso):
+    """Noisy gyroid: many 2B/3B configurations and dense output."""
+    f = ctx.field_gyroid(dims, period=12.0, seed=5, noise_amp=0.3)
+    torch_cuda.cuda.synchronize()
+    ref = check_against_oracle(ctx, f, iso)
+    assert ((ref["info"] >> 7) & 1).sum() > 0, "expected some variant-1 cells"
+
+
+def test_ragged_x_tiles(ctx, torch_cuda):
+    """Rows longer than one 1024-cell x-tile (two segments per row)."""
+    f = ctx.field_gyroid((1100, 9, 7), period=10.0, seed=9, noise_amp=0.2)
+    torch_cuda.cuda.synchronize()
+    check_against_oracle(ctx, f, 0.0)
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 2, 2), (2, 5, 3), (65, 2, 2), (64, 3, 2), (129, 4, 5)])
+def test_tiny_grids(ctx, torch_cuda, dims):
+    rng = np.random.default_rng(sum(dims))
+    X, Y, Z = dims
+    f = rng.standard_normal((Z, Y, X)).astype(np.float32)
+    check_against_oracle(ctx, _dev(torch_cuda, f), 0.0)
+
+
+def test_random_many_shapes(ctx, torch_cuda):
+    rng = np.random.default_rng(1234)
+    for _ in range(12):
+        X, Y, Z = (int(v) for v in rng.integers(2, 90, size=3))
+        f = rng.standard_normal((Z, Y, X)).astype(np.float32)
+        check_against_oracle(ctx, _dev(torch_cuda, f), float(rng.uniform(-0.5, 0.5)))
+
+
+def test_empty_surface(ctx, torch_cuda):
+    """No active cell -> empty mesh (the reference exits here, dmc.cu:2113)."""
+    f = np.full((20, 30, 40), 3.0, dtype=np.float32)
+    verts, quads, cnt = ctx.extract(_dev(torch_cuda, f), 0.0)
+    assert cnt["num_active"] == 0 and verts.shape == (0, 3) and quads.shape == (0, 4)
+
+
+def test_degenerate_dims(ctx, torch_cuda):
+    for dims in [(1, 5, 5), (5, 1, 5), (5, 5, 1)]:
+        X, Y, Z = dims
+        f = np.zeros((Z, Y, X), dtype=np.float32)
+        verts, quads, cnt = ctx.extract(_dev(torch_cuda, f), 0.5)
+        assert cnt["num_active"] == 0 and quads.shape == (0, 4)
+
+
+def test_exact_ties(ctx, torch_cuda):
+    """Integer field with integer iso: many corners exactly at iso."""
+    rng = np.random.default_rng(7)
+    f = rng.integers(-2, 3, size=(40, 37, 53)).astype(np.float32)
+    ref = check_against_oracle(ctx, _dev(torch_cuda, f), 0.0)
+    assert ref["num_tie_edges"] > 0
+
+
+def test_nan_corners(ctx, torch_cuda):
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal((30, 31, 45)).astype(np.float32)
+    f.reshape(-1)[rng.choice(f.size, 200, replace=False)] = np.nan
+    check_against_oracle(ctx, _dev(torch_cuda, f), 0.1)
+
+
+def test_denormal_values(ctx, torch_cuda):
+    rng = np.random.default_rng(9)
+    f = (rng.standard_normal((20, 24, 28)) * 1e-40).astype(np.float32)
+    check_against_oracle(ctx, _dev(torch_cuda, f), 0.0)
+
+
+def test_world_box(ctx, torch_cuda):
+    """Non-unit coordinate box (dmc.cu:2149 xyz_range)."""
+    f = oracle.field_sphere((41, 45, 50), (20.5, 22.0, 24.9), 15.3)
+    check_against_oracle(ctx, _dev(torch_cuda, f), 0.0, xyz_min=(-5.0, -2.5, 1.0), xyz_max=(5.0, 7.5, 3.0))
+
+
+def test_field_generators_match_cpu(ctx, torch_cuda):
+    dims = (70, 51, 33)
+    gb = ctx.field_blobs(dims, seed=11, nblobs=9)
+    gg = ctx.field_gyroid(dims, period=24.0, seed=12, noise_amp=0.05)
+    torch_cuda.cuda.synchronize()
+    cb = oracle.field_blobs(dims, 11, 9)
+    cg = oracle.field_gyroid(dims, 24.0, 12, 0.05)
+    assert np.array_equal(gb.cpu().numpy().view(np.uint32), cb.view(np.uint32))
+    assert np.array_equal(gg.cpu().numpy().view(np.uint32), cg.view(np.uint32))
+
+
+def test_host_entry_matches_device(ctx, torch_cuda):
+    """mdmc_run (host field in, host mesh out) == device path == oracle."""
+    import framework_amd as fa
+
+    f = oracle.field_blobs((57, 61, 49), 21, 7, sigma=(3.0, 9.0))
+    v, q = fa.run_host(f, 0.1)
+    ref = oracle.run(f, 0.1)
+    assert_same_mesh(v, q, ref)
+    verts, tris = fa.run_dmc(f, (0, 0, 0), (56, 60, 48), 0.1)
+    assert tris.shape == (2 * ref["num_quads"], 3)
+
+
+def test_deterministic(ctx, torch_cuda):
+    f = ctx.field_gyroid((90, 90, 90), period=15.0, seed=3, noise_amp=0.1)
+    torch_cuda.cuda.synchronize()
+    v1, q1, _ = ctx.extract(f, 0.0)
+    v2, q2, _ = ctx.extract(f, 0.0)
+    assert torch_cuda.equal(q1, q2) and torch_cuda.equal(v1.view(torch_cuda.int32), v2.view(torch_cuda.int32))
+
+
+def test_capacity_overflow_is_flagged(ctx, torch_cuda):
+    torch = torch_cuda
+    f = ctx.field_gyroid((60, 60, 60), period=12.0, seed=4, noise_amp=0.05)
+    ctx.count(f, 0.0)
+    torch.cuda.synchronize()
+    full = ctx.read_counts()
+    V, Q = full["num_vertices"], full["num_quads"]
+    verts = torch.full((V // 2, 3), -7.0, device="cuda")
+    quads = torch.full((Q // 2, 4), -7, dtype=torch.int32, device="cuda")
+    guard_v = torch.full((1024,), 123.0, device="cuda")
+    ctx.emit(f, verts, quads)
+    torch.cuda.synchronize()
+    st = ctx.read_counts()["status"]
+    assert st & 1 and st & 2
+    assert bool((guard_v == 123.0).all())
+
+
+def test_blobs_512_full(ctx, torch_cuda):
+    """Config 1 at full size: 512^3, 32 blobs, sigma U[8,32], amp U[-1,1], iso 0.1."""
+    f = ctx.field_blobs((512, 512, 512), seed=0, nblobs=32)
+    torch_cuda.cuda.synchronize()
+    ref = check_against_oracle(ctx, f, 0.1, check_cells=True)
+    assert ref["num_active"] > 100000
