@@ -76,7 +76,7 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   // x-tile: smallest 64*W covering the row, at most 1024 cells.
   uint32_t W = 1;
   while (W < 16 && 64u * W < p->cx) W <<= 1;
-  uint32_t R = (W >= 16) ? 4 : 8;
+  uint32_t R = (W >= 8) ? 4 : 8;
   if (const char *t = std::getenv("MDMC_TILE")) {  // "W,R,Kc" override for tuning
     int w = 0, r = 0, kc = 0;
     if (std::sscanf(t, "%d,%d,%d", &w, &r, &kc) == 3) {
@@ -93,7 +93,7 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   p->NRB = (p->cy + R - 1) / R;
   if (p->Kc == 0) {
     // Enough waves to cover the chip several times over: ~8 per CU.
-    const uint64_t target = (uint64_t)env_int("MDMC_TARGET_WAVES", 2048);
+    const uint64_t target = (uint64_t)env_int("MDMC_TARGET_WAVES", 4096);
     const uint64_t cols = (uint64_t)p->XT * p->NRB;
     uint64_t nkc = (target + cols - 1) / cols;
     if (nkc < 1) nkc = 1;
@@ -108,6 +108,10 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   const uint64_t cells = (uint64_t)p->cx * p->cy * p->cz;
   p->max_chunks = (cells + 63) / 64 + 1;
   p->classify_blocks = (uint64_t)p->XT * p->NRB * p->NKC;
+  p->vec4 = (p->X % 4 == 0) && W >= 4 && env_int("MDMC_VEC4", 1) != 0;
+  const uint32_t rows = pt_rows(W / 4);
+  p->NRBp = ((p->cy + rows - 1) / rows + 7) / 8 * 8;
+  p->pt_blocks = (uint64_t)p->XT * p->NRBp * p->cz;
   return MDMC_OK;
 }
 
@@ -245,7 +249,12 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   int rc;
   {
     ProfScope ps(ctx, 0, st);
-    rc = launch_classify(p.W, p.R, ca, (uint32_t)p.classify_blocks, st);
+    if (p.vec4 && ((uintptr_t)d_field % 16 == 0)) {
+      PtArgs pa{ca, p.NRBp};
+      rc = launch_classify_pt(p.W, pa, (uint32_t)p.pt_blocks, st);
+    } else {
+      rc = launch_classify(p.W, p.R, ca, (uint32_t)p.classify_blocks, st);
+    }
   }
   if (rc != MDMC_OK) return fail(MDMC_ERR_INVALID, "no classify kernel for this tile shape");
   HIP_TRY(hipGetLastError());
@@ -355,6 +364,17 @@ int mdmc_setup(void) {
   mdmc_pack_luts(&L, packed, info);
   HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_label), packed, sizeof(packed)));
   HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_cfg_info), info, sizeof(info)));
+  static uint16_t count[256];
+  for (int c = 0; c < 256; ++c) {
+    const int b6 = ((c >> 2) ^ (c >> 6)) & 1, b9 = ((c >> 5) ^ (c >> 6)) & 1, b10 = ((c >> 7) ^ (c >> 6)) & 1;
+    const int nbip = b6 + b9 + b10;
+    uint16_t w = (uint16_t)((L.num_vertex[0][c] + nbip) | ((L.num_vertex[1][c] + nbip) << 4) |
+                            ((b6 | (b9 << 1) | (b10 << 2)) << 8));
+    if (L.amb_face[c] != MDMC_NOT_AMBIGUOUS)
+      w = (uint16_t)(w | (1u << 11) | ((unsigned)MDMC_FACE_DIR[L.amb_face[c]] << 12));
+    count[c] = w;
+  }
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_cfg_count), count, sizeof(count)));
   g_setup_mask |= 1ull << dev;
   return MDMC_OK;
 }

@@ -17,6 +17,9 @@ struct Plan {
   uint32_t W, R, Kc, cap, XT, NRB, NKC, ntiles;
   uint64_t nsegs, nslots, max_chunks, classify_blocks;
   int empty;
+  int vec4;  // rows are a multiple of 4 points and span >= 256 cells
+  uint32_t NRBp;             // plane-tile row blocks, padded to a multiple of 8
+  uint64_t pt_blocks;        // plane-tile grid
 };
 
 struct Scratch {
@@ -58,6 +61,22 @@ inline int launch_classify_w(uint32_t R, const ClassifyArgs &a, uint32_t blocks,
   }
 }
 
+template <int W4>
+inline void launch_classify_pt_t(const PtArgs &a, uint32_t blocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_classify_pt<W4, (int)pt_rows(W4)>), dim3(blocks), dim3(kPtThreads), 0, st, a);
+}
+
+// Plane-tile kernel for rows of a multiple of 4 points (W = 4, 8 or 16).
+inline int launch_classify_pt(uint32_t W, const PtArgs &a, uint32_t blocks, hipStream_t st) {
+  switch (W) {
+    case 4: launch_classify_pt_t<1>(a, blocks, st); return 0;
+    case 8: launch_classify_pt_t<2>(a, blocks, st); return 0;
+    case 16: launch_classify_pt_t<4>(a, blocks, st); return 0;
+    default: return -1;
+  }
+}
+
+// z-marching kernel for any row length.
 inline int launch_classify(uint32_t W, uint32_t R, const ClassifyArgs &a, uint32_t blocks,
                            hipStream_t st) {
   switch (W) {

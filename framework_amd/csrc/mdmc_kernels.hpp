@@ -42,6 +42,9 @@ constexpr uint32_t kStatusVertexOverflow = 1, kStatusQuadOverflow = 2, kStatusLo
 //                     7-9 check direction (POS_X=0 .. NEG_Z=5)
 __constant__ uint64_t c_label[2][256];
 __constant__ uint16_t c_cfg_info[256];
+// c_cfg_count[cfg]: bits 0-3 NV variant 0 + bipolar owned edges, 4-7 the same
+// for variant 1, 8-10 bipolar owned edges 6/9/10, 11 = 2B/3B, 12-14 check dir.
+__constant__ uint16_t c_cfg_count[256];
 
 struct Totals {
   uint32_t num_active, num_vertices, num_quads, num_nonempty;
@@ -73,6 +76,12 @@ __device__ __forceinline__ uint32_t rec_lqb(uint64_t r) { return (uint32_t)(r >>
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Index of the calling wave in its workgroup, as a wave-uniform (scalar)
+// value so that per-wave bookkeeping compiles to scalar instructions.
+__device__ __forceinline__ uint32_t wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
 // Remap so that each XCD works on a contiguous range of tiles: neighbouring
 // tiles (which share halo rows) then hit the same L2.
@@ -89,6 +98,28 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     if (lane >= d) v += t;
   }
   return v;
+}
+
+// DPP controls (GFX9 encoding).
+constexpr int kDppRowShr = 0x110;  // + n: lane i reads lane i-n of its 16-lane row
+constexpr int kDppWaveShl1 = 0x130;  // lane i reads lane i+1 across the wave
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+// Value of lane+1 (0 in lane 63).
+__device__ __forceinline__ uint32_t lane_next(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kDppWaveShl1, 0xF, 0xF, true);
+}
+
+// Inclusive prefix sum over the 64 lanes with DPP (no LDS traffic).
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppRowShr + 1, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppRowShr + 2, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppRowShr + 4, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppRowShr + 8, 0xF, 0xF, true);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppRowBcast15, 0xA, 0xF, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kDppRowBcast31, 0xC, 0xF, false);
+  return x;
 }
 
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t mask) {
@@ -294,6 +325,213 @@ __global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
   }
 }
 
+// Plane-tile classify: the main kernel for rows that are a multiple of 4
+// points.  A 256-thread workgroup owns one tile = kPtRows (16 for 256-cell
+// x-tiles, 8 for wider ones) cell rows of one x-tile in one plane k; tiles are numbered in memory order, so concurrently running
+// workgroups read one contiguous region of the field (like a plain stream)
+// and plane k+1, which the next plane's tile reads again, is still in L2.
+// Row blocks are padded to a multiple of 8 so that the tile of the same rows
+// one plane up runs on the same XCD (block b runs on XCD b % 8).
+//
+// Phase 1: the 2*(kPtRows+1) point rows (rows j0..j0+kPtRows of planes k and
+// k+1) are loaded
+// with 16-byte loads (lane owns points 4l..4l+3 of a 256-point chunk); each
+// lane packs its sign bits plus the first bit of lane l+1 into a 5-bit value
+// and stores it into a 32-bit word per (cell row, chunk, lane):
+//   byte 0: point row j, plane k     byte 1: point row j+1, plane k
+//   byte 2: point row j, plane k+1   byte 3: point row j+1, plane k+1
+// Phase 2: each wave takes 4 cell rows; one LDS read gives all 20 corner bits
+// of the lane's 4 cells.  For cell q, t = (w >> q) & 0x03030303 holds the
+// corner pairs (p0,p1), (p3,p2), (p4,p5), (p7,p6) in its 4 bytes and
+// (t * 0x41041) >> 18 gathers them into an 8-bit key.
+constexpr int kPtThreads = 256;
+
+__device__ __forceinline__ uint32_t key_to_cfg(uint32_t key) {
+  // key bits: p0 p1 p3 p2 p4 p5 p7 p6 -> cfg bits p0..p7
+  return (key & 0x33u) | ((key & 0x44u) << 1) | ((key & 0x88u) >> 1);
+}
+
+struct PtArgs {
+  ClassifyArgs c;
+  uint32_t NRBp;  // row blocks per plane, padded to a multiple of 8
+};
+
+// Cell rows per plane tile: 16 for 256-cell x-tiles, 8 for wider ones
+// (register budget and occupancy).
+constexpr uint32_t pt_rows(uint32_t W4) { return W4 >= 2 ? 8u : 16u; }
+
+template <int W4, int kPtRows>
+__global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
+  static_assert(W4 >= 1 && W4 <= 4, "x-tile is at most 1024 cells");
+  static_assert(kPtRows % 4 == 0 && kPtRows <= 32, "rows are split evenly over 4 waves");
+  const ClassifyArgs &a = pa.c;
+  __shared__ uint32_t s_w[kPtRows][W4][kWave];
+  __shared__ uint16_t s_cnt[256];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
+  const uint16_t tab = c_cfg_count[tid & 255u];
+  if (blockIdx.x == 0) {
+    for (uint32_t i = tid; i < a.scan_state_words; i += kPtThreads) a.scan_state[i] = 0ull;
+    if (tid < 8) reinterpret_cast<uint32_t *>(a.totals)[tid] = 0u;
+    if (tid == 0) *a.scan_ticket = 0u;
+    if (a.counts_out != nullptr && tid < 4) a.counts_out[tid] = 0u;
+  }
+
+  const FieldDesc g = a.g;
+  const float iso = g.iso;
+  constexpr uint32_t CAP = 256u * W4;
+  const uint32_t t = blockIdx.x;
+  const uint32_t xt = t % a.XT;
+  const uint32_t rest = t / a.XT;
+  const uint32_t rb = rest % pa.NRBp;
+  const uint32_t k = rest / pa.NRBp;
+  const uint32_t j0 = rb * kPtRows;
+  const bool tile_ok = (k < g.cz) && (j0 < g.cy);  // padded row blocks do nothing
+  const uint32_t x0 = xt * CAP;
+  const size_t plane = (size_t)g.X * g.Y;
+  const bool has_extra = x0 + CAP < g.X;
+
+  // ---- phase 1: the 2*(kPtRows+1) point rows -> packed sign bits in LDS.
+  // Wave w takes point rows w, w+4, w+8, ...  All loads are issued first from
+  // clamped, always in-bounds addresses, then converted.
+  constexpr int kRowsPerWave = (2 * (kPtRows + 1) + 3) / 4;
+  if (tile_ok) {
+    float4 v[kRowsPerWave][W4];
+    float ve[kRowsPerWave];
+    uint32_t xo[W4];
+    bool xok[W4];
+#pragma unroll
+    for (int c = 0; c < W4; ++c) {
+      const uint32_t x = x0 + 256u * c + 4u * lane;
+      xok[c] = x < g.X;
+      xo[c] = xok[c] ? x : 0u;
+    }
+    const uint32_t xe = has_extra ? x0 + CAP : 0u;
+#pragma unroll
+    for (int i = 0; i < kRowsPerWave; ++i) {
+      const uint32_t pr = min(wave + 4u * i, 2u * (kPtRows + 1) - 1u);
+      const uint32_t p = pr / (kPtRows + 1);
+      const uint32_t r = pr - p * (kPtRows + 1);
+      const float *rowp = g.f + (size_t)min(k + p, g.Z - 1u) * plane + (size_t)min(j0 + r, g.Y - 1u) * g.X;
+#pragma unroll
+      for (int c = 0; c < W4; ++c) v[i][c] = *reinterpret_cast<const float4 *>(rowp + xo[c]);
+      ve[i] = rowp[xe];
+    }
+    s_cnt[tid & 255u] = tab;
+#pragma unroll
+    for (int i = 0; i < kRowsPerWave; ++i) {
+      const uint32_t pr = wave + 4u * i;
+      if (pr >= 2u * (kPtRows + 1)) break;
+      const uint32_t p = pr / (kPtRows + 1);        // 0: plane k, 1: plane k+1
+      const uint32_t r = pr - p * (kPtRows + 1);    // point row j0 + r
+      const bool row_ok = (j0 + r) < g.Y && (k + p) < g.Z;
+      uint32_t nib[W4];
+#pragma unroll
+      for (int c = 0; c < W4; ++c) {
+        const uint32_t bits = (uint32_t)(v[i][c].x < iso) | ((uint32_t)(v[i][c].y < iso) << 1) |
+                              ((uint32_t)(v[i][c].z < iso) << 2) | ((uint32_t)(v[i][c].w < iso) << 3);
+        nib[c] = (row_ok && xok[c]) ? bits : 0u;
+      }
+      const uint32_t ext = (row_ok && has_extra && ve[i] < iso) ? 1u : 0u;
+#pragma unroll
+      for (int c = 0; c < W4; ++c) {
+        // first point of lane+1, or of the next chunk / next x-tile for lane 63
+        uint32_t nb = lane_next(nib[c]);
+        const uint32_t nxt = (c + 1 < W4) ? (uint32_t)__builtin_amdgcn_readfirstlane(nib[(c + 1 < W4) ? c + 1 : c]) : ext;
+        if (lane == kWave - 1) nb = nxt;
+        const uint32_t v5 = nib[c] | ((nb & 1u) << 4);
+        // this point row is row j of cell row r (byte 2p) and row j+1 of cell
+        // row r-1 (byte 2p+1)
+        uint8_t *wb = reinterpret_cast<uint8_t *>(&s_w[0][c][lane]);
+        if (r < kPtRows) wb[(size_t)r * W4 * kWave * 4 + 2 * p] = (uint8_t)v5;
+        if (r > 0) wb[(size_t)(r - 1) * W4 * kWave * 4 + 2 * p + 1] = (uint8_t)v5;
+      }
+    }
+  }
+  __syncthreads();
+  if (!tile_ok) return;
+
+  // ---- phase 2: each wave classifies kPtRows/4 cell rows
+  const bool zin = k + 1 < g.cz;
+  for (uint32_t r = wave; r < kPtRows; r += kPtThreads / kWave) {
+    const uint32_t y = j0 + r;
+    if (y >= g.cy) break;
+    const bool yin = y + 1 < g.cy;
+    uint32_t cnt_a = 0, cnt_v = 0, cnt_q = 0;
+    const uint32_t seg = xt + a.XT * (y + g.cy * k);
+#pragma unroll
+    for (int c = 0; c < W4; ++c) {
+      const uint32_t w = s_w[r][c][lane];
+      // all 20 corner bits equal -> none of the lane's 4 cells is active
+      if (__ballot(w != 0u && w != 0x1F1F1F1Fu) == 0ull) continue;
+      const uint32_t xb = x0 + 256u * c + 4u * lane;
+      uint32_t keys = 0, lact = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t tq = (w >> q) & 0x03030303u;
+        const uint32_t key = ((tq * 0x41041u) >> 18) & 0xFFu;
+        keys |= key << (8 * q);
+        lact |= (uint32_t)((xb + q < g.cx) && key != 0u && key != 0xFFu) << q;
+      }
+      if (__ballot(lact != 0u) == 0ull) continue;
+      // per cell: variant, vertex count, quad count (0 for inactive cells)
+      uint32_t pk = 0, info4 = 0;
+      if (lact) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!((lact >> q) & 1u)) continue;
+          const uint32_t cfg = key_to_cfg((keys >> (8 * q)) & 0xFFu);
+          const uint32_t e = s_cnt[cfg];
+          uint32_t l2 = 0;
+          if (e & 0x800u) {  // 2B/3B: check the neighbour across the listed face
+            const uint32_t x = xb + q;
+            int nx = (int)x, ny = (int)y, nz = (int)k;
+            switch ((e >> 12) & 7u) {
+              case 0: nx += 1; break;
+              case 1: nx -= 1; break;
+              case 2: ny += 1; break;
+              case 3: ny -= 1; break;
+              case 4: nz += 1; break;
+              default: nz -= 1; break;
+            }
+            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.cx && ny < (int)g.cy && nz < (int)g.cz)
+              l2 = (s_cnt[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)] >> 11) & 1u;
+          }
+          const uint32_t nv = (e >> (4 * l2)) & 0xFu;
+          const bool xin = xb + q + 1 < g.cx;
+          const uint32_t qmask = (uint32_t)(xin && yin) | ((uint32_t)(xin && zin) << 1) |
+                                 ((uint32_t)(yin && zin) << 2);
+          const uint32_t nq = (uint32_t)__popc((e >> 8) & 7u & qmask);
+          // per-cell byte: nv (3 bits) | nq (2 bits) << 3 | variant << 5
+          info4 |= (nv | (nq << 3) | (l2 << 5)) << (8 * q);
+          pk += 1u | (nv << 9) | (nq << 20);
+        }
+      }
+      const uint32_t incl = wave_incl_scan_dpp(pk);
+      const uint32_t excl = incl - pk;
+      const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+      if (lact) {
+        uint32_t rank = cnt_a + (excl & 0x1FFu);
+        uint32_t lvb = cnt_v + ((excl >> 9) & 0x7FFu);
+        uint32_t lqb = cnt_q + (excl >> 20);
+        uint64_t *dst = a.slots + (size_t)seg * CAP;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!((lact >> q) & 1u)) continue;
+          const uint32_t b = (info4 >> (8 * q)) & 0xFFu;
+          dst[rank] = make_record(xb + q - x0, key_to_cfg((keys >> (8 * q)) & 0xFFu), b >> 5, lvb, lqb);
+          rank += 1;
+          lvb += b & 7u;
+          lqb += (b >> 3) & 3u;
+        }
+      }
+      cnt_a += tot & 0x1FFu;
+      cnt_v += (tot >> 9) & 0x7FFu;
+      cnt_q += tot >> 20;
+    }
+    if (lane == 0) a.seg_counts[seg] = make_uint2(cnt_a | (cnt_q << 16), cnt_v);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Decoupled look-back scan over segments.
 
@@ -328,7 +566,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
   __shared__ uint32_t s_tile;
   __shared__ uint4 s_wave[kScanThreads / kWave];
   __shared__ uint4 s_prefix;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
   if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
   __syncthreads();
   const uint32_t t = s_tile;
@@ -551,7 +789,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
   const Totals tot = *a.totals;
   const uint32_t lane = lane_id();
   const uint32_t waves_per_block = kEmitThreads / kWave;
-  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t wave = wave_id();
   const FieldDesc g = a.g;
   const float iso = g.iso;
   const size_t sy = g.X, sz = (size_t)g.X * g.Y;
@@ -746,7 +984,7 @@ struct CellsArgs {
 
 __global__ __launch_bounds__(256) void k_cells(CellsArgs a) {
   const Totals tot = *a.totals;
-  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
   for (uint32_t ch = blockIdx.x * 4 + wave; ch < num_chunks; ch += gridDim.x * 4) {

@@ -60,11 +60,31 @@ def test_blobs_small(ctx, torch_cuda, dims, seed, nblobs, iso):
 
 @pytest.mark.parametrize("dims,iso", [((96, 96, 96), 0.0), ((77, 50, 41), 0.15), ((33, 129, 20), -0.2)])
 def test_gyroid_noise(ctx, torch_cuda, dims, iso):
-    """Noisy gyroid: many 2B/3B configurations and dense output."""
-    f = ctx.field_gyroid(dims, period=12.0, seed=5, noise_amp=0.3)
+    """Strongly perturbed gyroid: dense output and adjacent 2B/3B cells."""
+    f = ctx.field_gyroid(dims, period=8.0, seed=5, noise_amp=0.8)
     torch_cuda.cuda.synchronize()
     ref = check_against_oracle(ctx, f, iso)
     assert ((ref["info"] >> 7) & 1).sum() > 0, "expected some variant-1 cells"
+
+
+@pytest.mark.parametrize("dims", [(260, 9, 11), (1028, 5, 6), (516, 13, 7), (256, 8, 8), (772, 3, 4)])
+def test_wide_rows_vectorised(ctx, torch_cuda, dims):
+    """Rows that are a multiple of 4 points and >= 256 cells take the 16-byte
+    load kernel: partial chunks, a second x-tile, rows ending mid-chunk."""
+    f = ctx.field_gyroid(dims, period=7.0, seed=13, noise_amp=0.6)
+    torch_cuda.cuda.synchronize()
+    check_against_oracle(ctx, f, 0.05)
+
+
+def test_misaligned_field_pointer(ctx, torch_cuda):
+    """A field pointer that is not 16-byte aligned uses the scalar kernel."""
+    torch = torch_cuda
+    dims = (520, 6, 5)
+    X, Y, Z = dims
+    buf = torch.empty(X * Y * Z + 1, dtype=torch.float32, device="cuda")
+    f = ctx.field_gyroid(dims, period=9.0, seed=14, noise_amp=0.4, out=buf[1:].view(Z, Y, X))
+    torch.cuda.synchronize()
+    check_against_oracle(ctx, f, 0.0)
 
 
 def test_ragged_x_tiles(ctx, torch_cuda):
