@@ -111,7 +111,8 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   p->vec4 = (p->X % 4 == 0) && W >= 4 && env_int("MDMC_VEC4", 1) != 0;
   const uint32_t rows = pt_rows(W / 4);
   p->NRBp = ((p->cy + rows - 1) / rows + 7) / 8 * 8;
-  p->pt_blocks = (uint64_t)p->XT * p->NRBp * p->cz;
+  const uint32_t kp = pt_planes(W / 4);
+  p->pt_blocks = (uint64_t)p->XT * p->NRBp * ((p->cz + kp - 1) / kp);
   return MDMC_OK;
 }
 

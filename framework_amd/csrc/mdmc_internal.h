@@ -63,7 +63,8 @@ inline int launch_classify_w(uint32_t R, const ClassifyArgs &a, uint32_t blocks,
 
 template <int W4>
 inline void launch_classify_pt_t(const PtArgs &a, uint32_t blocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_classify_pt<W4, (int)pt_rows(W4)>), dim3(blocks), dim3(kPtThreads), 0, st, a);
+  hipLaunchKernelGGL((k_classify_pt<W4, (int)pt_rows(W4), (int)pt_planes(W4)>), dim3(blocks),
+                     dim3(kPtThreads), 0, st, a);
 }
 
 // Plane-tile kernel for rows of a multiple of 4 points (W = 4, 8 or 16).

@@ -357,15 +357,18 @@ struct PtArgs {
 };
 
 // Cell rows per plane tile: 16 for 256-cell x-tiles, 8 for wider ones
-// (register budget and occupancy).
+// (register budget and occupancy).  Cell planes per tile: 2, or 1 for the
+// widest x-tiles.
 constexpr uint32_t pt_rows(uint32_t W4) { return W4 >= 2 ? 8u : 16u; }
+constexpr uint32_t pt_planes(uint32_t W4) { return W4 >= 4 ? 1u : 2u; }
 
-template <int W4, int kPtRows>
+template <int W4, int kPtRows, int KP>
 __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   static_assert(W4 >= 1 && W4 <= 4, "x-tile is at most 1024 cells");
   static_assert(kPtRows % 4 == 0 && kPtRows <= 32, "rows are split evenly over 4 waves");
+  static_assert(KP == 1 || KP == 2, "one or two cell planes per tile");
   const ClassifyArgs &a = pa.c;
-  __shared__ uint32_t s_w[kPtRows][W4][kWave];
+  __shared__ uint32_t s_w[KP][kPtRows][W4][kWave];
   __shared__ uint16_t s_cnt[256];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
   const uint16_t tab = c_cfg_count[tid & 255u];
@@ -383,17 +386,18 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   const uint32_t xt = t % a.XT;
   const uint32_t rest = t / a.XT;
   const uint32_t rb = rest % pa.NRBp;
-  const uint32_t k = rest / pa.NRBp;
+  const uint32_t k = (rest / pa.NRBp) * KP;  // first cell plane of the tile
   const uint32_t j0 = rb * kPtRows;
   const bool tile_ok = (k < g.cz) && (j0 < g.cy);  // padded row blocks do nothing
   const uint32_t x0 = xt * CAP;
   const size_t plane = (size_t)g.X * g.Y;
   const bool has_extra = x0 + CAP < g.X;
 
-  // ---- phase 1: the 2*(kPtRows+1) point rows -> packed sign bits in LDS.
-  // Wave w takes point rows w, w+4, w+8, ...  All loads are issued first from
-  // clamped, always in-bounds addresses, then converted.
-  constexpr int kRowsPerWave = (2 * (kPtRows + 1) + 3) / 4;
+  // ---- phase 1: the (KP+1)*(kPtRows+1) point rows -> packed sign bits in
+  // LDS.  Wave w takes point rows w, w+4, w+8, ...  All loads are issued
+  // first from clamped, always in-bounds addresses, then converted.
+  constexpr uint32_t kPointRows = (KP + 1) * (kPtRows + 1);
+  constexpr int kRowsPerWave = (kPointRows + 3) / 4;
   if (tile_ok) {
     float4 v[kRowsPerWave][W4];
     float ve[kRowsPerWave];
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
     const uint32_t xe = has_extra ? x0 + CAP : 0u;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; ++i) {
-      const uint32_t pr = min(wave + 4u * i, 2u * (kPtRows + 1) - 1u);
+      const uint32_t pr = min(wave + 4u * i, kPointRows - 1u);
       const uint32_t p = pr / (kPtRows + 1);
       const uint32_t r = pr - p * (kPtRows + 1);
       const float *rowp = g.f + (size_t)min(k + p, g.Z - 1u) * plane + (size_t)min(j0 + r, g.Y - 1u) * g.X;
@@ -420,8 +424,8 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; ++i) {
       const uint32_t pr = wave + 4u * i;
-      if (pr >= 2u * (kPtRows + 1)) break;
-      const uint32_t p = pr / (kPtRows + 1);        // 0: plane k, 1: plane k+1
+      if (pr >= kPointRows) break;
+      const uint32_t p = pr / (kPtRows + 1);        // point plane k + p
       const uint32_t r = pr - p * (kPtRows + 1);    // point row j0 + r
       const bool row_ok = (j0 + r) < g.Y && (k + p) < g.Z;
       uint32_t nib[W4];
@@ -439,28 +443,37 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
         const uint32_t nxt = (c + 1 < W4) ? (uint32_t)__builtin_amdgcn_readfirstlane(nib[(c + 1 < W4) ? c + 1 : c]) : ext;
         if (lane == kWave - 1) nb = nxt;
         const uint32_t v5 = nib[c] | ((nb & 1u) << 4);
-        // this point row is row j of cell row r (byte 2p) and row j+1 of cell
-        // row r-1 (byte 2p+1)
-        uint8_t *wb = reinterpret_cast<uint8_t *>(&s_w[0][c][lane]);
-        if (r < kPtRows) wb[(size_t)r * W4 * kWave * 4 + 2 * p] = (uint8_t)v5;
-        if (r > 0) wb[(size_t)(r - 1) * W4 * kWave * 4 + 2 * p + 1] = (uint8_t)v5;
+        // Point row r of point plane k+p is, for cell plane k+p (if in the
+        // tile), row j of cell row r (byte 0) and row j+1 of cell row r-1
+        // (byte 1); for cell plane k+p-1, the same rows one plane up (bytes
+        // 2 and 3).
+        uint8_t *wb = reinterpret_cast<uint8_t *>(&s_w[0][0][c][lane]);
+        constexpr size_t kRowStride = (size_t)W4 * kWave * 4, kPlaneStride = kRowStride * kPtRows;
+        if (p < (uint32_t)KP) {
+          if (r < kPtRows) wb[p * kPlaneStride + r * kRowStride + 0] = (uint8_t)v5;
+          if (r > 0) wb[p * kPlaneStride + (r - 1) * kRowStride + 1] = (uint8_t)v5;
+        }
+        if (p > 0) {
+          if (r < kPtRows) wb[(p - 1) * kPlaneStride + r * kRowStride + 2] = (uint8_t)v5;
+          if (r > 0) wb[(p - 1) * kPlaneStride + (r - 1) * kRowStride + 3] = (uint8_t)v5;
+        }
       }
     }
   }
   __syncthreads();
   if (!tile_ok) return;
 
-  // ---- phase 2: each wave classifies kPtRows/4 cell rows
-  const bool zin = k + 1 < g.cz;
-  for (uint32_t r = wave; r < kPtRows; r += kPtThreads / kWave) {
-    const uint32_t y = j0 + r;
-    if (y >= g.cy) break;
-    const bool yin = y + 1 < g.cy;
+  // ---- phase 2: each wave classifies KP*kPtRows/4 cell rows
+  for (uint32_t pr2 = wave; pr2 < KP * kPtRows; pr2 += kPtThreads / kWave) {
+    const uint32_t cp = pr2 / kPtRows, r = pr2 - cp * kPtRows;
+    const uint32_t y = j0 + r, z = k + cp;
+    if (y >= g.cy || z >= g.cz) continue;
+    const bool yin = y + 1 < g.cy, zin = z + 1 < g.cz;
     uint32_t cnt_a = 0, cnt_v = 0, cnt_q = 0;
-    const uint32_t seg = xt + a.XT * (y + g.cy * k);
+    const uint32_t seg = xt + a.XT * (y + g.cy * z);
 #pragma unroll
     for (int c = 0; c < W4; ++c) {
-      const uint32_t w = s_w[r][c][lane];
+      const uint32_t w = s_w[cp][r][c][lane];
       // all 20 corner bits equal -> none of the lane's 4 cells is active
       if (__ballot(w != 0u && w != 0x1F1F1F1Fu) == 0ull) continue;
       const uint32_t xb = x0 + 256u * c + 4u * lane;
@@ -484,7 +497,7 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
           uint32_t l2 = 0;
           if (e & 0x800u) {  // 2B/3B: check the neighbour across the listed face
             const uint32_t x = xb + q;
-            int nx = (int)x, ny = (int)y, nz = (int)k;
+            int nx = (int)x, ny = (int)y, nz = (int)z;
             switch ((e >> 12) & 7u) {
               case 0: nx += 1; break;
               case 1: nx -= 1; break;

@@ -1,0 +1,56 @@
+"""The C ABI library loads and exports every declared symbol (CPU, no GPU calls)."""
+import ctypes
+import os
+import subprocess
+
+import pytest
+
+import framework_amd as fa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_function():
+    L = fa.lib()
+    names = fa.declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_exported_symbols_have_c_linkage():
+    out = subprocess.run(["nm", "-D", "--defined-only", fa.mdmc.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    for n in fa.declared_functions():
+        assert n in exported, n
+
+
+def test_status_strings():
+    L = fa.lib()
+    assert L.mdmc_status_string(0) == b"ok"
+    assert L.mdmc_status_string(-4) == b"output capacity too small"
+    assert L.mdmc_status_string(12345) == b"unknown status"
+
+
+def test_smoothing_is_rejected_before_touching_the_device():
+    import numpy as np
+
+    with pytest.raises(fa.MDMCError) as e:
+        fa.run_host(np.zeros((4, 4, 4), np.float32), 0.0, num_smooth=3)
+    assert e.value.code == -1
+
+
+def test_null_arguments_are_rejected():
+    raw = ctypes.CDLL(fa.mdmc.LIB_PATH)  # untyped handle: plain null pointers
+    null = ctypes.c_void_p(None)
+    assert raw.mdmc_run(null, null, null, null, ctypes.c_float(0.0), 0, null) == -1
+    assert raw.mdmc_ctx_create(null) == -1
+    assert raw.mdmc_count_device(null, null, null, ctypes.c_float(0.0), null, null) == -1
+
+
+def test_cpp_shim_header_compiles():
+    """include/dmc.hpp (the dmc::run_dmc drop-in) builds with the host compiler."""
+    src = os.path.join(ROOT, "tests", "golden", "shim_compile_check.cpp")
+    subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                    src], check=True)
