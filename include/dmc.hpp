@@ -8,7 +8,9 @@
 //     dmc.cu:2113-2114);
 //   * num_smooth > 0 throws (the smoothing pass is not provided);
 //   * float3/uint3 are this header's own 12-byte PODs (the reference uses the
-//     CUDA vector types of the same layout).
+//     CUDA vector types of the same layout); they are also declared in the
+//     global namespace, as the reference's callers use them unqualified
+//     (main.cpp:8-18), unless MDMC_DMC_NO_GLOBAL_TYPES is defined.
 // Link with framework_amd/lib/libmdmc.so.
 #ifndef MDMC_DMC_HPP
 #define MDMC_DMC_HPP
@@ -44,6 +46,13 @@ class error : public std::runtime_error {
 };
 
 }  // namespace dmc
+
+#ifndef MDMC_DMC_NO_GLOBAL_TYPES
+using dmc::float3;
+using dmc::make_float3;
+using dmc::make_uint3;
+using dmc::uint3;
+#endif
 
 namespace utils {
 

@@ -49,8 +49,13 @@ def test_null_arguments_are_rejected():
     assert raw.mdmc_count_device(null, null, null, ctypes.c_float(0.0), null, null) == -1
 
 
-def test_cpp_shim_header_compiles():
-    """include/dmc.hpp (the dmc::run_dmc drop-in) builds with the host compiler."""
+def test_cpp_shim_header_compiles_and_links(tmp_path):
+    """include/dmc.hpp (the dmc::run_dmc drop-in) builds with the host compiler
+    and links against the library (the program is not run: no GPU here)."""
     src = os.path.join(ROOT, "tests", "golden", "shim_compile_check.cpp")
-    subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
-                    src], check=True)
+    main = tmp_path / "main.cpp"
+    main.write_text("int shim_compile_check();\nint main() { return 0; }\n")
+    libdir = os.path.dirname(fa.mdmc.LIB_PATH)
+    subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(main), src,
+                    "-L", libdir, "-lmdmc", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(tmp_path / "shim")],
+                   check=True)
