@@ -39,6 +39,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy --steps times (a kernel of known "
+                         "HBM traffic, used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -51,6 +54,14 @@ def main() -> int:
     import framework_amd as fa
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        src = torch.ones(DIMS, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {src.numel() * 4} bytes x {args.steps}")
+        return 0
     fa.setup()
     ctx = fa.Context()
     N = DIMS[0] * DIMS[1] * DIMS[2]
@@ -96,6 +107,15 @@ def main() -> int:
     dom_ms = per_kernel[dom]
     achieved = algo[dom] / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     B = 4.0 * N + 12.0 * V + 16.0 * Q
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "dmc_blobs512_hbm_traffic.json")
+    if os.path.exists(tfile):
+        # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+        # (scaled by a copy of known size) on this workload
+        with open(tfile) as fh:
+            tk = json.load(fh)["kernels"].get(dom)
+        if tk:
+            traffic = tk["read_bytes"] + tk["write_bytes"]
     pipe_gbs = B / (ms_step * 1e-3) / 1e9
 
     cpu = None
@@ -135,7 +155,7 @@ def main() -> int:
                                "amp U[-1,1]), iso 0.1, field resident in HBM",
                    "dims": list(DIMS), "iso": ISO, "active_cells": A, "vertices": V, "quads": Q},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": dom_ms, "algorithmic_bytes": algo[dom]},
         "pipeline": {"algorithmic_bytes": B, "achieved_gbs": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
                      "kernel_avg_ms": per_kernel},
