@@ -103,16 +103,28 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   }
   p->NKC = (p->cz + p->Kc - 1) / p->Kc;
   p->nsegs = (uint64_t)p->XT * p->cy * p->cz;
-  p->ntiles = (uint32_t)((p->nsegs + kScanTile - 1) / kScanTile);
   p->nslots = p->nsegs * p->cap;
   const uint64_t cells = (uint64_t)p->cx * p->cy * p->cz;
   p->max_chunks = (cells + 63) / 64 + 1;
+  p->ntiles = (uint32_t)((p->nsegs + kScanTile - 1) / kScanTile);
   p->classify_blocks = (uint64_t)p->XT * p->NRB * p->NKC;
   p->vec4 = (p->X % 4 == 0) && W >= 4 && env_int("MDMC_VEC4", 1) != 0;
-  const uint32_t rows = pt_rows(W / 4);
-  p->NRBp = ((p->cy + rows - 1) / rows + 7) / 8 * 8;
-  const uint32_t kp = pt_planes(W / 4);
-  p->pt_blocks = (uint64_t)p->XT * p->NRBp * ((p->cz + kp - 1) / kp);
+  p->ptR = pt_rows(W / 4);
+  p->ptKP = pt_planes(W / 4);
+  if (const char *t = std::getenv("MDMC_PT_SHAPE")) {  // "R,KP" override for tuning
+    int r = 0, kp = 0;
+    if (std::sscanf(t, "%d,%d", &r, &kp) == 2 && pt_shape_supported(W / 4, r, kp)) {
+      p->ptR = (uint32_t)r;
+      p->ptKP = (uint32_t)kp;
+    }
+  }
+  const uint32_t rows = p->ptR;
+  const uint32_t nrb = (p->cy + rows - 1) / rows;
+  p->band = (uint32_t)env_int("MDMC_PT_BAND", 1);
+  if (p->band < 1) p->band = 1;
+  p->NS = (nrb + 8 * p->band - 1) / (8 * p->band);
+  const uint32_t kp = p->ptKP;
+  p->pt_blocks = 8ull * p->band * p->NS * p->XT * ((p->cz + kp - 1) / kp);
   return MDMC_OK;
 }
 
@@ -177,15 +189,16 @@ int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
     off += align256(bytes);
     return o;
   };
-  const size_t o_counts = take(p.nsegs * sizeof(uint2));
-  const size_t o_vq = take(p.nsegs * sizeof(uint2));
-  const size_t o_ne_seg = take((p.nsegs + 1) * sizeof(uint32_t));
-  const size_t o_ne_ap = take((p.nsegs + 1) * sizeof(uint32_t));
+  const size_t seg_padded = (size_t)p.ntiles * kScanTile;  // the scan reads whole tiles
+  const size_t o_counts = take(seg_padded * sizeof(uint2));
+  const size_t o_vq = take(seg_padded * sizeof(uint2));
+  const size_t o_ne = take((p.nsegs + 1) * sizeof(uint2));
   const size_t o_chunk = take(p.max_chunks * sizeof(uint32_t));
   const size_t o_state = take((size_t)p.ntiles * 4 * sizeof(unsigned long long));
   const size_t o_ticket = take(sizeof(uint32_t));
   const size_t o_totals = take(sizeof(Totals));
   const size_t o_slots = take(p.nslots * sizeof(uint64_t));
+  const size_t o_gmap = take(p.nsegs * (p.cap / 64) * sizeof(uint4));
   if (off > ctx->scratch_bytes) {
     if (ctx->scratch) {
       hipError_t e = hipFree(ctx->scratch);
@@ -209,13 +222,13 @@ int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
   char *b = static_cast<char *>(ctx->scratch);
   s.seg_counts = reinterpret_cast<uint2 *>(b + o_counts);
   s.seg_vq = reinterpret_cast<uint2 *>(b + o_vq);
-  s.ne_seg = reinterpret_cast<uint32_t *>(b + o_ne_seg);
-  s.ne_ap = reinterpret_cast<uint32_t *>(b + o_ne_ap);
+  s.ne = reinterpret_cast<uint2 *>(b + o_ne);
   s.chunk_ne = reinterpret_cast<uint32_t *>(b + o_chunk);
   s.scan_state = reinterpret_cast<unsigned long long *>(b + o_state);
   s.ticket = reinterpret_cast<uint32_t *>(b + o_ticket);
   s.totals = reinterpret_cast<Totals *>(b + o_totals);
   s.slots = reinterpret_cast<uint64_t *>(b + o_slots);
+  s.gmap = reinterpret_cast<uint4 *>(b + o_gmap);
   ctx->s = s;
   return MDMC_OK;
 }
@@ -242,8 +255,11 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ca.Kc = p.Kc;
   ca.seg_counts = s.seg_counts;
   ca.slots = s.slots;
+  ca.gmap = s.gmap;
   ca.scan_state = s.scan_state;
   ca.scan_state_words = p.ntiles * 4;
+  ca.seg_pad = s.seg_counts + p.nsegs;
+  ca.seg_pad_n = (uint32_t)((uint64_t)p.ntiles * kScanTile - p.nsegs);
   ca.scan_ticket = s.ticket;
   ca.totals = s.totals;
   ca.counts_out = reinterpret_cast<uint32_t *>(d_counts);
@@ -251,8 +267,8 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   {
     ProfScope ps(ctx, 0, st);
     if (p.vec4 && ((uintptr_t)d_field % 16 == 0)) {
-      PtArgs pa{ca, p.NRBp};
-      rc = launch_classify_pt(p.W, pa, (uint32_t)p.pt_blocks, st);
+      PtArgs pa{ca, p.band, p.NS};
+      rc = launch_classify_pt(p.W, p.ptR, p.ptKP, pa, (uint32_t)p.pt_blocks, st);
     } else {
       rc = launch_classify(p.W, p.R, ca, (uint32_t)p.classify_blocks, st);
     }
@@ -264,15 +280,14 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   sa.seg_counts = s.seg_counts;
   sa.nsegs = (uint32_t)p.nsegs;
   sa.ntiles = p.ntiles;
+  sa.max_chunks = p.max_chunks;
   sa.seg_vq = s.seg_vq;
-  sa.ne_seg = s.ne_seg;
-  sa.ne_ap = s.ne_ap;
+  sa.ne = s.ne;
   sa.chunk_ne = s.chunk_ne;
   sa.state = s.scan_state;
   sa.ticket = s.ticket;
   sa.totals = s.totals;
   sa.counts_out = reinterpret_cast<uint32_t *>(d_counts);
-  sa.max_chunks = p.max_chunks;
   {
     ProfScope ps(ctx, 1, st);
     hipLaunchKernelGGL(k_scan, dim3(p.ntiles), dim3(kScanThreads), 0, st, sa);
@@ -281,11 +296,6 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   return MDMC_OK;
 }
 
-uint32_t emit_blocks(const mdmc_ctx *ctx, const Plan &p) {
-  const uint64_t want = (p.max_chunks + 3) / 4;
-  const uint64_t cap = (uint64_t)ctx->num_cus * 8;
-  return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
-}
 
 int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
                 const float mn[3], const float mx[3], float *d_vertices, uint64_t vcap,
@@ -308,8 +318,8 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ea.seg_counts = s.seg_counts;
   ea.seg_vq = s.seg_vq;
   ea.slots = s.slots;
-  ea.ne_seg = s.ne_seg;
-  ea.ne_ap = s.ne_ap;
+  ea.gmap = s.gmap;
+  ea.ne = s.ne;
   ea.chunk_ne = s.chunk_ne;
   ea.totals = s.totals;
   ea.verts = d_vertices;
@@ -319,7 +329,10 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ea.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
   {
     ProfScope ps(ctx, 2, st);
-    hipLaunchKernelGGL(k_emit, dim3(emit_blocks(ctx, p)), dim3(kEmitThreads), 0, st, ea);
+    // one wave per chunk of 64 active cells, at most 8 workgroups per CU
+    const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * 8));
+    hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
   }
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
@@ -489,22 +502,18 @@ int mdmc_cells_device(mdmc_ctx *ctx, uint32_t *d_cell_index, uint8_t *d_cfg, uin
   ca.XT = p.XT;
   ca.cap = p.cap;
   ca.cx = p.cx;
-  ca.cy = p.cy;
   ca.nsegs = (uint32_t)p.nsegs;
-  ca.max_chunks = p.max_chunks;
   ca.seg_counts = ctx->s.seg_counts;
   ca.slots = ctx->s.slots;
   ca.seg_vq = ctx->s.seg_vq;
-  ca.ne_seg = ctx->s.ne_seg;
-  ca.ne_ap = ctx->s.ne_ap;
-  ca.chunk_ne = ctx->s.chunk_ne;
+  ca.ne = ctx->s.ne;
   ca.totals = ctx->s.totals;
   ca.cell_index = d_cell_index;
   ca.cfg = d_cfg;
   ca.flags = d_flags;
   ca.vbegin = d_vertex_begin;
   ca.capacity = capacity;
-  hipLaunchKernelGGL(k_cells, dim3(emit_blocks(ctx, p)), dim3(256), 0, static_cast<hipStream_t>(stream), ca);
+  hipLaunchKernelGGL(k_cells, dim3(std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)p.nsegs, (uint32_t)ctx->num_cus * 8))), dim3(256), 0, static_cast<hipStream_t>(stream), ca);
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
 }

@@ -173,8 +173,12 @@ struct ClassifyArgs {
   uint32_t XT, NRB, NKC, Kc;
   uint2 *seg_counts;      // per segment: {nact | nquad << 16, nvert}
   uint64_t *slots;        // per segment: 64*W record slots
+  uint4 *gmap;            // per segment and 64-cell group: {active mask lo, hi, rank of
+                          // the group's first active cell, 0}; written for non-empty groups
   unsigned long long *scan_state;  // zeroed here for k_scan
   uint32_t scan_state_words;
+  uint2 *seg_pad;         // segment totals past the last segment, up to a whole scan tile
+  uint32_t seg_pad_n;     // (zeroed here, read by the scan)
   uint32_t *scan_ticket;
   Totals *totals;
   uint32_t *counts_out;   // user mdmc_counts (4 words), zeroed here
@@ -190,6 +194,7 @@ __global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
   for (uint32_t i = lane; i < 256; i += kWave) s_info[i] = c_cfg_info[i];
   if (blockIdx.x == 0) {
     for (uint32_t i = lane; i < a.scan_state_words; i += kWave) a.scan_state[i] = 0ull;
+    for (uint32_t i = lane; i < a.seg_pad_n; i += kWave) a.seg_pad[i] = make_uint2(0u, 0u);
     if (lane < 8) reinterpret_cast<uint32_t *>(a.totals)[lane] = 0u;
     if (lane == 0) *a.scan_ticket = 0u;
     if (a.counts_out != nullptr && lane < 4) a.counts_out[lane] = 0u;
@@ -294,6 +299,8 @@ __global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
         const bool active = (x < g.cx) && (cfg != 0u) && (cfg != 0xFFu);
         const uint64_t bal = __ballot(active);
         if (bal == 0ull) continue;
+        if (lane == 0)
+          a.gmap[(size_t)seg * W + c] = make_uint4((uint32_t)bal, (uint32_t)(bal >> 32), cnt_a, 0u);
         uint32_t nv = 0, nq = 0, lut2 = 0;
         if (active) {
           const uint32_t info = s_info[cfg];
@@ -326,54 +333,71 @@ __global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
 }
 
 // Plane-tile classify: the main kernel for rows that are a multiple of 4
-// points.  A 256-thread workgroup owns one tile = kPtRows (16 for 256-cell
-// x-tiles, 8 for wider ones) cell rows of one x-tile in one plane k; tiles are numbered in memory order, so concurrently running
-// workgroups read one contiguous region of the field (like a plain stream)
-// and plane k+1, which the next plane's tile reads again, is still in L2.
-// Row blocks are padded to a multiple of 8 so that the tile of the same rows
-// one plane up runs on the same XCD (block b runs on XCD b % 8).
+// points.  A 256-thread workgroup owns one tile = R cell rows x one x-tile
+// (256*W4 cells) x KP cell planes.  Workgroups are dispatched round-robin over
+// the 8 XCDs (block b runs on XCD b % 8), each with its own L2; XCD x is given
+// the x-th band of row blocks of every plane block, walked in memory order
+// (x-tile, then row block, then plane block).  The point row shared by two
+// neighbouring row blocks and the point plane shared by two plane blocks are
+// then read twice by the same XCD a few tiles apart and come from its L2, so
+// HBM sees (almost) every field value once.
 //
-// Phase 1: the 2*(kPtRows+1) point rows (rows j0..j0+kPtRows of planes k and
-// k+1) are loaded
-// with 16-byte loads (lane owns points 4l..4l+3 of a 256-point chunk); each
-// lane packs its sign bits plus the first bit of lane l+1 into a 5-bit value
-// and stores it into a 32-bit word per (cell row, chunk, lane):
-//   byte 0: point row j, plane k     byte 1: point row j+1, plane k
-//   byte 2: point row j, plane k+1   byte 3: point row j+1, plane k+1
-// Phase 2: each wave takes 4 cell rows; one LDS read gives all 20 corner bits
-// of the lane's 4 cells.  For cell q, t = (w >> q) & 0x03030303 holds the
-// corner pairs (p0,p1), (p3,p2), (p4,p5), (p7,p6) in its 4 bytes and
-// (t * 0x41041) >> 18 gathers them into an 8-bit key.
+// Phase 1: wave w converts point rows r = w, w+4, ... (R+1 of them, so every
+// wave has the same share) of all KP+1 point planes.  A lane owns 4
+// consecutive points of a 256-point chunk (one 16-byte load from a uniform
+// row address plus a fixed lane offset); it packs their sign bits and the
+// first bit of lane l+1 into a 5-bit value v5, pairs the values of planes
+// p and p+1 into 16 bits and stores that pair into the LDS word of cell row
+// r (low half) and of cell row r-1 (high half) of cell plane p:
+//   byte 0: point row j,   plane k     byte 1: point row j,   plane k+1
+//   byte 2: point row j+1, plane k     byte 3: point row j+1, plane k+1
+// (the LDS has one padding row above and below, so every store is
+// unconditional and every address is the wave's base plus a constant).
+// Phase 2: each wave takes cell rows w, w+4, ...; one LDS read gives all 20
+// corner bits of the lane's 4 cells.  For cell q, t = (w >> q) & 0x03030303
+// holds the corner pairs (p0,p1), (p4,p5), (p3,p2), (p7,p6) in its bytes and
+// (t * 0x41041) >> 18 gathers them into an 8-bit key, which indexes a table of
+// configuration and counts.
 constexpr int kPtThreads = 256;
 
-__device__ __forceinline__ uint32_t key_to_cfg(uint32_t key) {
-  // key bits: p0 p1 p3 p2 p4 p5 p7 p6 -> cfg bits p0..p7
-  return (key & 0x33u) | ((key & 0x44u) << 1) | ((key & 0x88u) >> 1);
+// key bits p0 p1 p4 p5 p3 p2 p7 p6 -> configuration bits p0..p7
+__host__ __device__ constexpr uint32_t key_to_cfg(uint32_t key) {
+  return (key & 0x03u) | ((key >> 3) & 0x04u) | ((key >> 1) & 0x08u) | ((key << 2) & 0x30u) |
+         ((key & 0x40u) << 1) | ((key & 0x80u) >> 1);
 }
 
 struct PtArgs {
   ClassifyArgs c;
-  uint32_t NRBp;  // row blocks per plane, padded to a multiple of 8
+  uint32_t B;   // row blocks per XCD band
+  uint32_t NS;  // bands of 8*B row blocks per plane block
 };
 
-// Cell rows per plane tile: 16 for 256-cell x-tiles, 8 for wider ones
-// (register budget and occupancy).  Cell planes per tile: 2, or 1 for the
-// widest x-tiles.
-constexpr uint32_t pt_rows(uint32_t W4) { return W4 >= 2 ? 8u : 16u; }
-constexpr uint32_t pt_planes(uint32_t W4) { return W4 >= 4 ? 1u : 2u; }
+// Cell rows per plane tile (R + 1 point rows, 2 per wave) and cell planes per
+// tile (KP + 1 point planes), chosen so that all loads of a wave fit in
+// registers: (R+1)/4 * (KP+1) * W4 float4 per lane.
+constexpr uint32_t pt_rows(uint32_t) { return 7u; }
+constexpr uint32_t pt_planes(uint32_t W4) { return W4 >= 4 ? 1u : (W4 == 2 ? 2u : 4u); }
 
-template <int W4, int kPtRows, int KP>
+template <int W4, int R, int KP>
 __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   static_assert(W4 >= 1 && W4 <= 4, "x-tile is at most 1024 cells");
-  static_assert(kPtRows % 4 == 0 && kPtRows <= 32, "rows are split evenly over 4 waves");
-  static_assert(KP == 1 || KP == 2, "one or two cell planes per tile");
+  static_assert((R + 1) % 4 == 0 && R <= 31, "point rows are split evenly over 4 waves");
+  static_assert(KP >= 1 && KP <= 4, "one to four cell planes per tile");
+  constexpr int kRowsPerWave = (R + 1) / 4;
+  constexpr uint32_t CAP = 256u * W4;
+  constexpr uint32_t kRowWords = W4 * kWave;           // LDS words per cell row
+  constexpr uint32_t kPlaneWords = kRowWords * (R + 2);  // incl. two padding rows
+  __shared__ uint32_t s_w[KP * kPlaneWords];
+  __shared__ uint32_t s_tab[256];
   const ClassifyArgs &a = pa.c;
-  __shared__ uint32_t s_w[KP][kPtRows][W4][kWave];
-  __shared__ uint16_t s_cnt[256];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-  const uint16_t tab = c_cfg_count[tid & 255u];
+  {
+    const uint32_t cfg = key_to_cfg(tid);
+    s_tab[tid] = (uint32_t)c_cfg_count[cfg] | (cfg << 16);
+  }
   if (blockIdx.x == 0) {
     for (uint32_t i = tid; i < a.scan_state_words; i += kPtThreads) a.scan_state[i] = 0ull;
+    for (uint32_t i = tid; i < a.seg_pad_n; i += kPtThreads) a.seg_pad[i] = make_uint2(0u, 0u);
     if (tid < 8) reinterpret_cast<uint32_t *>(a.totals)[tid] = 0u;
     if (tid == 0) *a.scan_ticket = 0u;
     if (a.counts_out != nullptr && tid < 4) a.counts_out[tid] = 0u;
@@ -381,81 +405,72 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
 
   const FieldDesc g = a.g;
   const float iso = g.iso;
-  constexpr uint32_t CAP = 256u * W4;
-  const uint32_t t = blockIdx.x;
-  const uint32_t xt = t % a.XT;
-  const uint32_t rest = t / a.XT;
-  const uint32_t rb = rest % pa.NRBp;
-  const uint32_t k = (rest / pa.NRBp) * KP;  // first cell plane of the tile
-  const uint32_t j0 = rb * kPtRows;
-  const bool tile_ok = (k < g.cz) && (j0 < g.cy);  // padded row blocks do nothing
+  const uint32_t s_idx = blockIdx.x >> 3;  // position in this XCD's sequence
+  const uint32_t xt = s_idx % a.XT;
+  const uint32_t u = s_idx / a.XT;
+  const uint32_t u2 = u / pa.B;
+  const uint32_t rb = ((u2 % pa.NS) * 8u + (blockIdx.x & 7u)) * pa.B + u % pa.B;
+  const uint32_t k = (u2 / pa.NS) * KP;  // first cell plane of the tile
+  const uint32_t j0 = rb * R;
+  const bool tile_ok = (k < g.cz) && (j0 < g.cy);  // band padding does nothing
   const uint32_t x0 = xt * CAP;
   const size_t plane = (size_t)g.X * g.Y;
   const bool has_extra = x0 + CAP < g.X;
 
-  // ---- phase 1: the (KP+1)*(kPtRows+1) point rows -> packed sign bits in
-  // LDS.  Wave w takes point rows w, w+4, w+8, ...  All loads are issued
-  // first from clamped, always in-bounds addresses, then converted.
-  constexpr uint32_t kPointRows = (KP + 1) * (kPtRows + 1);
-  constexpr int kRowsPerWave = (kPointRows + 3) / 4;
+  // ---- phase 1.  All loads are issued first, from in-bounds addresses:
+  // rows and planes past the grid are clamped to its last row / plane and
+  // lanes past the row end to its last 4 points; the cells that would use
+  // such values lie outside the grid and are never classified.
   if (tile_ok) {
-    float4 v[kRowsPerWave][W4];
-    float ve[kRowsPerWave];
-    uint32_t xo[W4];
-    bool xok[W4];
+    float4 v[kRowsPerWave][KP + 1][W4];
+    float ve[kRowsPerWave][KP + 1];
+    uint32_t xob[W4];
 #pragma unroll
-    for (int c = 0; c < W4; ++c) {
-      const uint32_t x = x0 + 256u * c + 4u * lane;
-      xok[c] = x < g.X;
-      xo[c] = xok[c] ? x : 0u;
-    }
+    for (int c = 0; c < W4; ++c) xob[c] = min(x0 + 256u * c + 4u * lane, g.X - 4u) * 4u;
     const uint32_t xe = has_extra ? x0 + CAP : 0u;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; ++i) {
-      const uint32_t pr = min(wave + 4u * i, kPointRows - 1u);
-      const uint32_t p = pr / (kPtRows + 1);
-      const uint32_t r = pr - p * (kPtRows + 1);
-      const float *rowp = g.f + (size_t)min(k + p, g.Z - 1u) * plane + (size_t)min(j0 + r, g.Y - 1u) * g.X;
+      const size_t yoff = (size_t)min(j0 + wave + 4u * i, g.Y - 1u) * g.X;
 #pragma unroll
-      for (int c = 0; c < W4; ++c) v[i][c] = *reinterpret_cast<const float4 *>(rowp + xo[c]);
-      ve[i] = rowp[xe];
+      for (int p = 0; p <= KP; ++p) {
+        const float *rowp = g.f + (size_t)min(k + p, g.Z - 1u) * plane + yoff;
+#pragma unroll
+        for (int c = 0; c < W4; ++c)
+          v[i][p][c] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(rowp) + xob[c]);
+        ve[i][p] = rowp[xe];
+      }
     }
-    s_cnt[tid & 255u] = tab;
+    uint16_t *s_h = reinterpret_cast<uint16_t *>(s_w);
+    const uint32_t base = wave * kRowWords + lane;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; ++i) {
-      const uint32_t pr = wave + 4u * i;
-      if (pr >= kPointRows) break;
-      const uint32_t p = pr / (kPtRows + 1);        // point plane k + p
-      const uint32_t r = pr - p * (kPtRows + 1);    // point row j0 + r
-      const bool row_ok = (j0 + r) < g.Y && (k + p) < g.Z;
-      uint32_t nib[W4];
+      uint32_t v5[KP + 1][W4];
 #pragma unroll
-      for (int c = 0; c < W4; ++c) {
-        const uint32_t bits = (uint32_t)(v[i][c].x < iso) | ((uint32_t)(v[i][c].y < iso) << 1) |
-                              ((uint32_t)(v[i][c].z < iso) << 2) | ((uint32_t)(v[i][c].w < iso) << 3);
-        nib[c] = (row_ok && xok[c]) ? bits : 0u;
-      }
-      const uint32_t ext = (row_ok && has_extra && ve[i] < iso) ? 1u : 0u;
+      for (int p = 0; p <= KP; ++p) {
+        uint32_t nib[W4];
 #pragma unroll
-      for (int c = 0; c < W4; ++c) {
-        // first point of lane+1, or of the next chunk / next x-tile for lane 63
-        uint32_t nb = lane_next(nib[c]);
-        const uint32_t nxt = (c + 1 < W4) ? (uint32_t)__builtin_amdgcn_readfirstlane(nib[(c + 1 < W4) ? c + 1 : c]) : ext;
-        if (lane == kWave - 1) nb = nxt;
-        const uint32_t v5 = nib[c] | ((nb & 1u) << 4);
-        // Point row r of point plane k+p is, for cell plane k+p (if in the
-        // tile), row j of cell row r (byte 0) and row j+1 of cell row r-1
-        // (byte 1); for cell plane k+p-1, the same rows one plane up (bytes
-        // 2 and 3).
-        uint8_t *wb = reinterpret_cast<uint8_t *>(&s_w[0][0][c][lane]);
-        constexpr size_t kRowStride = (size_t)W4 * kWave * 4, kPlaneStride = kRowStride * kPtRows;
-        if (p < (uint32_t)KP) {
-          if (r < kPtRows) wb[p * kPlaneStride + r * kRowStride + 0] = (uint8_t)v5;
-          if (r > 0) wb[p * kPlaneStride + (r - 1) * kRowStride + 1] = (uint8_t)v5;
+        for (int c = 0; c < W4; ++c)
+          nib[c] = (uint32_t)(v[i][p][c].x < iso) | ((uint32_t)(v[i][p][c].y < iso) << 1) |
+                   ((uint32_t)(v[i][p][c].z < iso) << 2) | ((uint32_t)(v[i][p][c].w < iso) << 3);
+        const uint32_t ext = (uint32_t)(ve[i][p] < iso);
+#pragma unroll
+        for (int c = 0; c < W4; ++c) {
+          // first point of lane+1, or of the next chunk / next x-tile for lane 63
+          uint32_t nb = lane_next(nib[c]);
+          const uint32_t nxt =
+              (c + 1 < W4) ? (uint32_t)__builtin_amdgcn_readfirstlane(nib[(c + 1 < W4) ? c + 1 : c]) : ext;
+          if (lane == kWave - 1) nb = nxt;
+          v5[p][c] = nib[c] | ((nb & 1u) << 4);
         }
-        if (p > 0) {
-          if (r < kPtRows) wb[(p - 1) * kPlaneStride + r * kRowStride + 2] = (uint8_t)v5;
-          if (r > 0) wb[(p - 1) * kPlaneStride + (r - 1) * kRowStride + 3] = (uint8_t)v5;
+      }
+#pragma unroll
+      for (int cp = 0; cp < KP; ++cp) {
+#pragma unroll
+        for (int c = 0; c < W4; ++c) {
+          const uint16_t h = (uint16_t)(v5[cp][c] | (v5[cp + 1][c] << 8));
+          // cell row r = wave + 4i sits at padded row r + 1; cell row r - 1 at r
+          s_h[2u * (base + cp * kPlaneWords + (4u * i + 1u) * kRowWords + c * kWave)] = h;
+          s_h[2u * (base + cp * kPlaneWords + (4u * i) * kRowWords + c * kWave) + 1u] = h;
         }
       }
     }
@@ -463,9 +478,16 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   __syncthreads();
   if (!tile_ok) return;
 
-  // ---- phase 2: each wave classifies KP*kPtRows/4 cell rows
-  for (uint32_t pr2 = wave; pr2 < KP * kPtRows; pr2 += kPtThreads / kWave) {
-    const uint32_t cp = pr2 / kPtRows, r = pr2 - cp * kPtRows;
+  // ---- phase 2: each wave classifies cell rows wave, wave+4, ... of the
+  // KP*R rows of the tile
+  uint32_t xvalid[W4];  // bit q: cell x0 + 256c + 4 lane + q lies in the grid
+#pragma unroll
+  for (int c = 0; c < W4; ++c) {
+    const uint32_t xb = x0 + 256u * c + 4u * lane;
+    xvalid[c] = xb >= g.cx ? 0u : (g.cx - xb >= 4u ? 0xFu : (1u << (g.cx - xb)) - 1u);
+  }
+  for (uint32_t m = wave; m < (uint32_t)(KP * R); m += kPtThreads / kWave) {
+    const uint32_t cp = m / R, r = m - cp * R;
     const uint32_t y = j0 + r, z = k + cp;
     if (y >= g.cy || z >= g.cz) continue;
     const bool yin = y + 1 < g.cy, zin = z + 1 < g.cz;
@@ -473,7 +495,7 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
     const uint32_t seg = xt + a.XT * (y + g.cy * z);
 #pragma unroll
     for (int c = 0; c < W4; ++c) {
-      const uint32_t w = s_w[cp][r][c][lane];
+      const uint32_t w = s_w[cp * kPlaneWords + (r + 1u) * kRowWords + c * kWave + lane];
       // all 20 corner bits equal -> none of the lane's 4 cells is active
       if (__ballot(w != 0u && w != 0x1F1F1F1Fu) == 0ull) continue;
       const uint32_t xb = x0 + 256u * c + 4u * lane;
@@ -483,17 +505,19 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
         const uint32_t tq = (w >> q) & 0x03030303u;
         const uint32_t key = ((tq * 0x41041u) >> 18) & 0xFFu;
         keys |= key << (8 * q);
-        lact |= (uint32_t)((xb + q < g.cx) && key != 0u && key != 0xFFu) << q;
+        lact |= (uint32_t)(key != 0u && key != 0xFFu) << q;
       }
+      lact &= xvalid[c];
       if (__ballot(lact != 0u) == 0ull) continue;
       // per cell: variant, vertex count, quad count (0 for inactive cells)
-      uint32_t pk = 0, info4 = 0;
+      uint32_t pk = 0, info4 = 0, cfgs = 0;
       if (lact) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           if (!((lact >> q) & 1u)) continue;
-          const uint32_t cfg = key_to_cfg((keys >> (8 * q)) & 0xFFu);
-          const uint32_t e = s_cnt[cfg];
+          const uint32_t e32 = s_tab[(keys >> (8 * q)) & 0xFFu];
+          const uint32_t e = e32 & 0xFFFFu;
+          cfgs |= (e32 >> 16) << (8 * q);
           uint32_t l2 = 0;
           if (e & 0x800u) {  // 2B/3B: check the neighbour across the listed face
             const uint32_t x = xb + q;
@@ -507,7 +531,7 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
               default: nz -= 1; break;
             }
             if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.cx && ny < (int)g.cy && nz < (int)g.cz)
-              l2 = (s_cnt[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)] >> 11) & 1u;
+              l2 = (c_cfg_count[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)] >> 11) & 1u;
           }
           const uint32_t nv = (e >> (4 * l2)) & 0xFu;
           const bool xin = xb + q + 1 < g.cx;
@@ -522,6 +546,24 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
       const uint32_t incl = wave_incl_scan_dpp(pk);
       const uint32_t excl = incl - pk;
       const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+      {
+        // Group map: the 16 lanes of a DPP row hold the 64 cells of one group
+        // (4 per lane); OR their nibbles into lane 15 of the row.
+        uint32_t mlo = (lane & 8u) ? 0u : lact << (4u * (lane & 7u));
+        uint32_t mhi = (lane & 8u) ? lact << (4u * (lane & 7u)) : 0u;
+        mlo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mlo, kDppRowShr + 1, 0xF, 0xF, true);
+        mhi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mhi, kDppRowShr + 1, 0xF, 0xF, true);
+        mlo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mlo, kDppRowShr + 2, 0xF, 0xF, true);
+        mhi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mhi, kDppRowShr + 2, 0xF, 0xF, true);
+        mlo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mlo, kDppRowShr + 4, 0xF, 0xF, true);
+        mhi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mhi, kDppRowShr + 4, 0xF, 0xF, true);
+        mlo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mlo, kDppRowShr + 8, 0xF, 0xF, true);
+        mhi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)mhi, kDppRowShr + 8, 0xF, 0xF, true);
+        if ((lane & 15u) == 15u && (mlo | mhi) != 0u) {
+          const uint32_t first = cnt_a + (incl & 0x1FFu) - (uint32_t)(__popc(mlo) + __popc(mhi));
+          a.gmap[(size_t)seg * (CAP / 64u) + 4u * c + (lane >> 4)] = make_uint4(mlo, mhi, first, 0u);
+        }
+      }
       if (lact) {
         uint32_t rank = cnt_a + (excl & 0x1FFu);
         uint32_t lvb = cnt_v + ((excl >> 9) & 0x7FFu);
@@ -531,7 +573,7 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
         for (int q = 0; q < 4; ++q) {
           if (!((lact >> q) & 1u)) continue;
           const uint32_t b = (info4 >> (8 * q)) & 0xFFu;
-          dst[rank] = make_record(xb + q - x0, key_to_cfg((keys >> (8 * q)) & 0xFFu), b >> 5, lvb, lqb);
+          dst[rank] = make_record(xb + q - x0, (cfgs >> (8 * q)) & 0xFFu, b >> 5, lvb, lqb);
           rank += 1;
           lvb += b & 7u;
           lqb += (b >> 3) & 3u;
@@ -546,26 +588,31 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
 }
 
 // ---------------------------------------------------------------------------
-// Decoupled look-back scan over segments.
+// Decoupled look-back scan over segment totals.  A tile is 4096 segments, 16
+// per thread read as eight 16-byte loads (the totals array is padded to a
+// whole number of tiles with zeros).  Four channels are scanned: active
+// cells, vertices, quads and non-empty segments.  Outputs: the first vertex
+// and quad of every segment; the list of non-empty segments with the index
+// of their first active cell (ne[i] = {segment, first active}, ne[NE] =
+// {nsegs, A}); for every chunk of 64 active cells the list position of the
+// segment holding its first cell; the grand totals.
 
 constexpr int kScanThreads = 256;
-constexpr int kScanIpt = 8;
+constexpr int kScanIpt = 16;  // segments per thread
 constexpr int kScanTile = kScanThreads * kScanIpt;
 constexpr uint32_t kSpinLimit = 1u << 24;
 
 struct ScanArgs {
-  const uint2 *seg_counts;
-  uint32_t nsegs;
-  uint32_t ntiles;
-  uint2 *seg_vq;       // per segment: {first vertex, first quad}
-  uint32_t *ne_seg;    // non-empty segments in order
-  uint32_t *ne_ap;     // their first active index (sentinel = A at [NE])
-  uint32_t *chunk_ne;  // per 64-record chunk: index into ne_* of its first record
+  const uint2 *seg_counts;  // padded to ntiles * kScanTile entries
+  uint32_t nsegs, ntiles;
+  uint64_t max_chunks;      // capacity of chunk_ne
+  uint2 *seg_vq;            // per segment: first vertex, first quad (same padding)
+  uint2 *ne;                // non-empty segments: {segment, first active cell}; nsegs + 1
+  uint32_t *chunk_ne;       // per 64 active cells: index into ne of its first cell's segment
   unsigned long long *state;  // 4 words per tile: (status << 32) | value
   uint32_t *ticket;
   Totals *totals;
   uint32_t *counts_out;  // optional user mdmc_counts
-  uint64_t max_chunks;   // capacity of chunk_ne
 };
 
 __device__ __forceinline__ unsigned long long ld_state(const unsigned long long *p) {
@@ -585,33 +632,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
   const uint32_t t = s_tile;
   const uint32_t base = t * kScanTile + tid * kScanIpt;
 
-  uint32_t ia[kScanIpt], iv[kScanIpt], iq[kScanIpt];
+  uint4 c[kScanIpt / 2];
+  const uint4 *src = reinterpret_cast<const uint4 *>(a.seg_counts + base);
+#pragma unroll
+  for (int i = 0; i < kScanIpt / 2; ++i) c[i] = src[i];
+  // per segment: .x = nact | nquad << 16, .y = nvert; two segments per uint4
   uint4 sum = make_uint4(0, 0, 0, 0);
 #pragma unroll
-  for (int i = 0; i < kScanIpt; ++i) {
-    const uint32_t s = base + i;
-    uint2 c = (s < a.nsegs) ? a.seg_counts[s] : make_uint2(0, 0);
-    ia[i] = c.x & 0xFFFFu;
-    iq[i] = c.x >> 16;
-    iv[i] = c.y;
-    sum.x += ia[i];
-    sum.y += iv[i];
-    sum.z += iq[i];
-    sum.w += ia[i] ? 1u : 0u;
+  for (int i = 0; i < kScanIpt / 2; ++i) {
+    sum.x += (c[i].x & 0xFFFFu) + (c[i].z & 0xFFFFu);
+    sum.y += c[i].y + c[i].w;
+    sum.z += (c[i].x >> 16) + (c[i].z >> 16);
+    sum.w += ((c[i].x & 0xFFFFu) ? 1u : 0u) + ((c[i].z & 0xFFFFu) ? 1u : 0u);
   }
   uint4 inc;
-  inc.x = wave_incl_scan(sum.x);
-  inc.y = wave_incl_scan(sum.y);
-  inc.z = wave_incl_scan(sum.z);
-  inc.w = wave_incl_scan(sum.w);
+  inc.x = wave_incl_scan_dpp(sum.x);
+  inc.y = wave_incl_scan_dpp(sum.y);
+  inc.z = wave_incl_scan_dpp(sum.z);
+  inc.w = wave_incl_scan_dpp(sum.w);
   if (lane == kWave - 1) s_wave[wave] = inc;
   __syncthreads();
   uint4 woff = make_uint4(0, 0, 0, 0), agg = make_uint4(0, 0, 0, 0);
 #pragma unroll
   for (int w = 0; w < kScanThreads / kWave; ++w) {
-    const uint4 s = s_wave[w];
-    if ((uint32_t)w < wave) { woff.x += s.x; woff.y += s.y; woff.z += s.z; woff.w += s.w; }
-    agg.x += s.x; agg.y += s.y; agg.z += s.z; agg.w += s.w;
+    const uint4 v = s_wave[w];
+    if ((uint32_t)w < wave) { woff.x += v.x; woff.y += v.y; woff.z += v.z; woff.w += v.w; }
+    agg.x += v.x; agg.y += v.y; agg.z += v.z; agg.w += v.w;
   }
 
   if (wave == 0) {
@@ -642,22 +688,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
           const unsigned long long *p = a.state + 4ull * (uint64_t)idx;
           uint32_t spins = 0;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            if ((done >> c) & 1u) continue;
-            do { w[c] = ld_state(p + c); } while ((w[c] >> 32) == 0 && ++spins < kSpinLimit);
+          for (int ch = 0; ch < 4; ++ch) {
+            if ((done >> ch) & 1u) continue;
+            do { w[ch] = ld_state(p + ch); } while ((w[ch] >> 32) == 0 && ++spins < kSpinLimit);
           }
           if (spins >= kSpinLimit) timed_out = true;
         }
         uint32_t add[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          add[c] = 0;
-          if ((done >> c) & 1u) continue;
-          const uint64_t incmask = __ballot((w[c] >> 32) == 2);
+        for (int ch = 0; ch < 4; ++ch) {
+          add[ch] = 0;
+          if ((done >> ch) & 1u) continue;
+          const uint64_t incmask = __ballot((w[ch] >> 32) == 2);
           const int stop = incmask ? (__ffsll((long long)incmask) - 1) : kWave;
-          const uint32_t val = ((int)lane <= stop && idx >= 0) ? (uint32_t)w[c] : 0u;
-          add[c] = __shfl(wave_incl_scan(val), kWave - 1, kWave);
-          if (incmask) done |= 1u << c;
+          const uint32_t val = ((int)lane <= stop && idx >= 0) ? (uint32_t)w[ch] : 0u;
+          add[ch] = __shfl(wave_incl_scan(val), kWave - 1, kWave);
+          if (incmask) done |= 1u << ch;
         }
         excl.x += add[0]; excl.y += add[1]; excl.z += add[2]; excl.w += add[3];
         look -= kWave;
@@ -675,33 +721,34 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
   }
   __syncthreads();
   const uint4 tp = s_prefix;
-  // exclusive prefix of this thread's first item
+  // exclusive prefix of this thread's first segment
   uint32_t pa = tp.x + woff.x + inc.x - sum.x;
   uint32_t pv = tp.y + woff.y + inc.y - sum.y;
   uint32_t pq = tp.z + woff.z + inc.z - sum.z;
   uint32_t pe = tp.w + woff.w + inc.w - sum.w;
   bool bad = false;
+  uint4 *dst = reinterpret_cast<uint4 *>(a.seg_vq + base);
 #pragma unroll
-  for (int i = 0; i < kScanIpt; ++i) {
-    const uint32_t s = base + i;
-    if (s < a.nsegs) {
-      a.seg_vq[s] = make_uint2(pv, pq);
-      if (ia[i]) {
-        if (pe < a.nsegs) {
-          a.ne_seg[pe] = s;
-          a.ne_ap[pe] = pa;
-        } else {
-          bad = true;
-        }
-        for (uint32_t gch = (pa + 63u) >> 6; (gch << 6) < pa + ia[i]; ++gch) {
+  for (int i = 0; i < kScanIpt / 2; ++i) {
+    uint4 o;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t cx_ = h ? c[i].z : c[i].x, cy_ = h ? c[i].w : c[i].y;
+      const uint32_t n = cx_ & 0xFFFFu;
+      if (h) { o.z = pv; o.w = pq; } else { o.x = pv; o.y = pq; }
+      if (n) {
+        const uint32_t sg = base + 2 * i + h;
+        if (pe < a.nsegs) a.ne[pe] = make_uint2(sg, pa); else bad = true;
+        for (uint32_t gch = (pa + 63u) >> 6; (gch << 6) < pa + n; ++gch) {
           if (gch < a.max_chunks) a.chunk_ne[gch] = pe; else bad = true;
         }
         ++pe;
       }
+      pa += n;
+      pv += cy_;
+      pq += cx_ >> 16;
     }
-    pa += ia[i];
-    pv += iv[i];
-    pq += iq[i];
+    dst[i] = o;
   }
   if (t == a.ntiles - 1 && tid == kScanThreads - 1) {
     // pa/pv/pq/pe now hold the grand totals
@@ -713,7 +760,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
     tot.num_chunks = (pa + 63u) >> 6;
     tot.pad0 = tot.pad1 = tot.pad2 = 0;
     *a.totals = tot;
-    if (pe <= a.nsegs) a.ne_ap[pe] = pa; else bad = true;
+    if (pe <= a.nsegs) a.ne[pe] = make_uint2(a.nsegs, pa); else bad = true;
     if (a.counts_out) {
       a.counts_out[0] = pa;
       a.counts_out[1] = pv;
@@ -724,21 +771,30 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Emit: vertices and quads, one lane per active cell.
+// Emit: vertices and quads.  Each wave takes chunks of 64 consecutive active
+// cells (in cell order), one lane per cell, and finds each cell's segment in
+// the non-empty segment list (the chunk's first list entry comes from the
+// scan; a window of 64 entries is searched with shuffles).  Per cell: edge
+// crossing points, dual vertices (FP32, reference operation order) and
+// quads.  A ring
+// neighbour of an owned edge is active by construction; its record is found
+// in O(1): the +x neighbour is the next record of the same segment (or the
+// first of the next x-tile's segment), and in a neighbour row the rank of
+// cell x is the group's first rank plus the population count of the group's
+// active mask below x.
 
 constexpr int kEmitThreads = 256;
 
 struct EmitArgs {
   FieldDesc g;
   float mnx, mny, mnz, rgx, rgy, rgz;  // xyz_min and xyz_max - xyz_min
-  uint32_t XT, cap;
-  uint32_t nsegs;
+  uint32_t XT, cap, nsegs;
   uint64_t max_chunks;  // capacity of chunk_ne
   const uint2 *seg_counts;
   const uint2 *seg_vq;
   const uint64_t *slots;
-  const uint32_t *ne_seg;
-  const uint32_t *ne_ap;
+  const uint4 *gmap;
+  const uint2 *ne;
   const uint32_t *chunk_ne;
   const Totals *totals;
   float *verts;
@@ -754,44 +810,14 @@ __device__ __forceinline__ float coordf(uint32_t i, uint32_t n, float range, flo
   return (float)i / (float)n * range + mn;
 }
 
-// Position of `rank`-th record of segment s holding x offset xl, or -1.
-__device__ __forceinline__ int64_t find_in_segment(const EmitArgs &a, uint32_t s, uint32_t xl) {
-  const uint32_t n = min(a.seg_counts[s].x & 0xFFFFu, a.cap);
-  const uint64_t *p = a.slots + (size_t)s * a.cap;
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (rec_x(p[mid]) < xl) lo = mid + 1; else hi = mid;
-  }
-  if (lo < n && rec_x(p[lo]) == xl) return (int64_t)s * a.cap + lo;
-  return -1;
-}
-
-// Slot of the cell one step in +x from the cell at slot `pos` (segment s,
-// x offset xl): the next record of the segment, or the first record of the
-// next x-tile's segment.
-__device__ __forceinline__ int64_t next_x(const EmitArgs &a, uint32_t s, int64_t pos, uint32_t xl) {
-  if (pos < 0) return -1;
-  if (xl + 1 < a.cap) {
-    const uint32_t n = min(a.seg_counts[s].x & 0xFFFFu, a.cap);
-    const int64_t q = pos + 1;
-    if ((uint64_t)(q - (int64_t)s * a.cap) < n && rec_x(a.slots[q]) == xl + 1) return q;
-    return -1;
-  }
-  const uint32_t s2 = s + 1;
-  if ((a.seg_counts[s2].x & 0xFFFFu) && rec_x(a.slots[(size_t)s2 * a.cap]) == 0) return (int64_t)s2 * a.cap;
-  return -1;
-}
-
-// Vertex index of edge `e` of the cell at slot `pos`.
-__device__ __forceinline__ int32_t ring_vertex(const EmitArgs &a, const uint64_t *lab, int64_t pos,
-                                               int e, bool &ok) {
-  if (pos < 0) { ok = false; return -1; }
-  const uint64_t r = a.slots[pos];
-  const uint32_t s = (uint32_t)(pos / a.cap);
-  const uint32_t m = (uint32_t)(lab[rec_lut2(r) * 256 + rec_cfg(r)] >> (4 * e)) & 0xFu;
-  if (m == 0xFu) { ok = false; return -1; }
-  return (int32_t)(a.seg_vq[s].x + rec_lvb(r) + m);
+// Rank of the active cell at x offset xl in segment s, or -1 if the group
+// map says it is not active.
+__device__ __forceinline__ int32_t rank_in_segment(const EmitArgs &a, uint32_t s, uint32_t xl) {
+  const uint4 e = a.gmap[(size_t)s * (a.cap >> 6) + (xl >> 6)];
+  const uint64_t mask = (uint64_t)e.x | ((uint64_t)e.y << 32);
+  const uint32_t b = xl & 63u;
+  if (!((mask >> b) & 1ull)) return -1;
+  return (int32_t)(e.z + (uint32_t)__popcll(mask & ((1ull << b) - 1ull)));
 }
 
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
@@ -799,194 +825,222 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   __syncthreads();
 
-  const Totals tot = *a.totals;
-  const uint32_t lane = lane_id();
-  const uint32_t waves_per_block = kEmitThreads / kWave;
-  const uint32_t wave = wave_id();
+  const uint32_t lane = lane_id(), wave = wave_id();
   const FieldDesc g = a.g;
   const float iso = g.iso;
   const size_t sy = g.X, sz = (size_t)g.X * g.Y;
+  const uint32_t cap = a.cap;
+  const Totals tot = *a.totals;
   // Counts are clamped to buffer capacities so that no access can leave the
   // scratch buffers even if the count pass produced inconsistent data.
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
 
-  for (uint32_t ch = blockIdx.x * waves_per_block + wave; ch < num_chunks;
-       ch += gridDim.x * waves_per_block) {
-    const uint32_t act = ch * 64u + lane;
-    const uint32_t u0 = a.chunk_ne[ch];
-    const uint32_t wl = (u0 < num_ne && u0 + lane < num_ne) ? a.ne_ap[u0 + lane] : 0xFFFFFFFFu;
-    // Segment of this lane's record: the last window entry starting at or
-    // before `act` (all lanes take part in the shuffles).
-    uint32_t m = 0;
+  for (uint32_t ch = blockIdx.x * (kEmitThreads / kWave) + wave; ch < num_chunks;
+       ch += gridDim.x * (kEmitThreads / kWave)) {
+    {
+      const uint32_t act = ch * 64u + lane;
+      const uint32_t u0 = a.chunk_ne[ch];
+      const uint2 wl = (u0 < num_ne && u0 + lane <= num_ne) ? a.ne[u0 + lane] : make_uint2(0u, 0xFFFFFFFFu);
+      // segment of this lane's cell: the last window entry starting at or
+      // before `act` (all lanes take part in the shuffles)
+      uint32_t m = 0;
 #pragma unroll
-    for (uint32_t step = 32; step >= 1; step >>= 1) {
-      const uint32_t cand = m + step;
-      const uint32_t wc = __shfl(wl, (int)(cand & 63u), kWave);
-      if (cand < 64u && wc <= act) m = cand;
-    }
-    const uint32_t ap = __shfl(wl, (int)m, kWave);
-    if (act >= tot.num_active) continue;
-    // Inputs come from the count pass; a broken invariant is reported in the
-    // status word instead of being followed out of bounds.
-    if (u0 >= num_ne || u0 + m >= num_ne) { atomicOr(a.status, kStatusInternal); continue; }
-    const uint32_t s = a.ne_seg[u0 + m];
-    const uint32_t rank = act - ap;
-    if (s >= a.nsegs || act < ap || rank >= min(a.seg_counts[s].x & 0xFFFFu, a.cap)) {
-      atomicOr(a.status, kStatusInternal);
-      continue;
-    }
-    const int64_t pos = (int64_t)s * a.cap + rank;
-    const uint64_t rec = a.slots[pos];
-    const uint32_t xl = rec_x(rec), cfg = rec_cfg(rec), lut2 = rec_lut2(rec);
-    const uint32_t xt = s % a.XT, row = s / a.XT;
-    const uint32_t j = row % g.cy, k = row / g.cy;
-    const uint32_t x = xt * a.cap + xl;
-    if (x >= g.cx || xl >= a.cap) { atomicOr(a.status, kStatusInternal); continue; }
-    const uint2 vq = a.seg_vq[s];
-    const uint32_t vb = vq.x + rec_lvb(rec);
-    const uint32_t qb = vq.y + rec_lqb(rec);
+      for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t cand = m + step;
+        const uint32_t wc = (uint32_t)__shfl((int)wl.y, (int)(cand & 63u), kWave);
+        if (cand < 64u && wc <= act) m = cand;
+      }
+      const uint32_t ap = (uint32_t)__shfl((int)wl.y, (int)m, kWave);
+      const uint32_t s = (uint32_t)__shfl((int)wl.x, (int)m, kWave);
+      if (act >= tot.num_active) continue;
+      // Inputs come from the count pass; a broken invariant is reported in the
+      // status word instead of being followed out of bounds.
+      const uint32_t rank = act - ap;
+      if (u0 >= num_ne || u0 + m >= num_ne || s >= a.nsegs || act < ap ||
+          rank >= min(a.seg_counts[s].x & 0xFFFFu, cap)) {
+        atomicOr(a.status, kStatusInternal);
+        continue;
+      }
+      const uint64_t rec = a.slots[(size_t)s * cap + rank];
+      const uint32_t xl = rec_x(rec), cfg = rec_cfg(rec), lut2 = rec_lut2(rec);
+      const uint32_t xt = s % a.XT, row = s / a.XT;
+      const uint32_t j = row % g.cy, k = row / g.cy;
+      const uint32_t x = xt * cap + xl;
+      if (x >= g.cx || xl >= cap) { atomicOr(a.status, kStatusInternal); continue; }
+      const uint2 vq = a.seg_vq[s];
+      const uint32_t vb = vq.x + rec_lvb(rec);
+      const uint32_t qb = vq.y + rec_lqb(rec);
 
-    const float *p = g.f + (size_t)k * sz + (size_t)j * sy + x;
-    float v[8];
-    v[0] = p[0];
-    v[1] = p[1];
-    v[2] = p[sy + 1];
-    v[3] = p[sy];
-    v[4] = p[sz];
-    v[5] = p[sz + 1];
-    v[6] = p[sz + sy + 1];
-    v[7] = p[sz + sy];
+      const float *p = g.f + (size_t)k * sz + (size_t)j * sy + x;
+      float v[8];
+      v[0] = p[0];
+      v[1] = p[1];
+      v[2] = p[sy + 1];
+      v[3] = p[sy];
+      v[4] = p[sz];
+      v[5] = p[sz + 1];
+      v[6] = p[sz + sy + 1];
+      v[7] = p[sz + sy];
 
-    const float X0 = coordf(x, g.cx, a.rgx, a.mnx), X1 = coordf(x + 1, g.cx, a.rgx, a.mnx);
-    const float Y0 = coordf(j, g.cy, a.rgy, a.mny), Y1 = coordf(j + 1, g.cy, a.rgy, a.mny);
-    const float Z0 = coordf(k, g.cz, a.rgz, a.mnz), Z1 = coordf(k + 1, g.cz, a.rgz, a.mnz);
+      const float X0 = coordf(x, g.cx, a.rgx, a.mnx), X1 = coordf(x + 1, g.cx, a.rgx, a.mnx);
+      const float Y0 = coordf(j, g.cy, a.rgy, a.mny), Y1 = coordf(j + 1, g.cy, a.rgy, a.mny);
+      const float Z0 = coordf(k, g.cz, a.rgz, a.mnz), Z1 = coordf(k + 1, g.cz, a.rgz, a.mnz);
 
-    // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
-    // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
-    // point is bitwise the one the owner writes (dmc.cu:1468-1514).
-    constexpr uint8_t EF[12] = {1, 1, 2, 0, 0, 1, 2, 3, 5, 5, 6, 4};
-    constexpr uint8_t ET[12] = {0, 2, 3, 3, 4, 5, 6, 7, 4, 6, 7, 7};
-    // Owner of each edge lies at a non-positive offset; an edge is skipped
-    // when its owner would be outside the grid (dmc.cu:1539-1545).
-    // bit 0: needs x>0, bit 1: needs y>0, bit 2: needs z>0
-    constexpr uint8_t NEED[12] = {6, 4, 4, 5, 3, 2, 0, 1, 2, 0, 0, 1};
-    const uint32_t have = (x > 0 ? 1u : 0u) | (j > 0 ? 2u : 0u) | (k > 0 ? 4u : 0u);
+      // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
+      // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
+      // point is bitwise the one the owner writes (dmc.cu:1468-1514).
+      constexpr uint8_t EF[12] = {1, 1, 2, 0, 0, 1, 2, 3, 5, 5, 6, 4};
+      constexpr uint8_t ET[12] = {0, 2, 3, 3, 4, 5, 6, 7, 4, 6, 7, 7};
+      // Owner of each edge lies at a non-positive offset; an edge is skipped
+      // when its owner would be outside the grid (dmc.cu:1539-1545).
+      // bit 0: needs x>0, bit 1: needs y>0, bit 2: needs z>0
+      constexpr uint8_t NEED[12] = {6, 4, 4, 5, 3, 2, 0, 1, 2, 0, 0, 1};
+      const uint32_t have = (x > 0 ? 1u : 0u) | (j > 0 ? 2u : 0u) | (k > 0 ? 4u : 0u);
 
-    const uint64_t labels = s_lab[lut2 * 256 + cfg];
-    float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
-    uint32_t cnt[4] = {0, 0, 0, 0};
-    float ex[3] = {0, 0, 0}, ey[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
+      const uint64_t labels = s_lab[lut2 * 256 + cfg];
+      float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+      uint32_t cnt[4] = {0, 0, 0, 0};
+      float ex[3] = {0, 0, 0}, ey[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
 #pragma unroll
-    for (int e = 0; e < 12; ++e) {
-      const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
-      if (lab == 0xFu) continue;
-      const int f = EF[e], to = ET[e];
-      const float vf = v[f], vt = v[to];
-      const float tt = (iso - vf) / (vt - vf);
-      const float omt = 1.0f - tt;
-      const float fx = (f == 1 || f == 2 || f == 5 || f == 6) ? X1 : X0;
-      const float fy = (f == 2 || f == 3 || f == 6 || f == 7) ? Y1 : Y0;
-      const float fz = (f >= 4) ? Z1 : Z0;
-      const float tx = (to == 1 || to == 2 || to == 5 || to == 6) ? X1 : X0;
-      const float ty = (to == 2 || to == 3 || to == 6 || to == 7) ? Y1 : Y0;
-      const float tz = (to >= 4) ? Z1 : Z0;
-      const float px = fx * omt + tx * tt;
-      const float py = fy * omt + ty * tt;
-      const float pz = fz * omt + tz * tt;
-      if (e == 6) { ex[0] = px; ey[0] = py; ez[0] = pz; }
-      if (e == 9) { ex[1] = px; ey[1] = py; ez[1] = pz; }
-      if (e == 10) { ex[2] = px; ey[2] = py; ez[2] = pz; }
-      if ((NEED[e] & have) != NEED[e]) continue;
+      for (int e = 0; e < 12; ++e) {
+        const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
+        if (lab == 0xFu) continue;
+        const int f = EF[e], to = ET[e];
+        const float vf = v[f], vt = v[to];
+        const float tt = (iso - vf) / (vt - vf);
+        const float omt = 1.0f - tt;
+        const float fx = (f == 1 || f == 2 || f == 5 || f == 6) ? X1 : X0;
+        const float fy = (f == 2 || f == 3 || f == 6 || f == 7) ? Y1 : Y0;
+        const float fz = (f >= 4) ? Z1 : Z0;
+        const float tx = (to == 1 || to == 2 || to == 5 || to == 6) ? X1 : X0;
+        const float ty = (to == 2 || to == 3 || to == 6 || to == 7) ? Y1 : Y0;
+        const float tz = (to >= 4) ? Z1 : Z0;
+        const float px = fx * omt + tx * tt;
+        const float py = fy * omt + ty * tt;
+        const float pz = fz * omt + tz * tt;
+        if (e == 6) { ex[0] = px; ey[0] = py; ez[0] = pz; }
+        if (e == 9) { ex[1] = px; ey[1] = py; ez[1] = pz; }
+        if (e == 10) { ex[2] = px; ey[2] = py; ez[2] = pz; }
+        if ((NEED[e] & have) != NEED[e]) continue;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (lab == (uint32_t)q) {
-          if (cnt[q] == 0) { ax[q] = px; ay[q] = py; az[q] = pz; }
-          else { ax[q] = ax[q] + px; ay[q] = ay[q] + py; az[q] = az[q] + pz; }
-          ++cnt[q];
+        for (int q = 0; q < 4; ++q) {
+          if (lab == (uint32_t)q) {
+            if (cnt[q] == 0) { ax[q] = px; ay[q] = py; az[q] = pz; }
+            else { ax[q] = ax[q] + px; ay[q] = ay[q] + py; az[q] = az[q] + pz; }
+            ++cnt[q];
+          }
         }
       }
-    }
-    const uint16_t info = c_cfg_info[cfg];
-    const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
-    const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
-    const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
-    const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
-    const uint32_t nv = niso + b6 + b9 + b10;
-    if ((uint64_t)vb + nv <= a.vcap) {
-      float *out = a.verts + 3ull * vb;
-      const float qnan = __int_as_float(0x7FC00000);
+      const uint16_t info = c_cfg_info[cfg];
+      const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
+      const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
+      const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
+      const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
+      const uint32_t nv = niso + b6 + b9 + b10;
+      if ((uint64_t)vb + nv <= a.vcap) {
+        float *out = a.verts + 3ull * vb;
+        const float qnan = __int_as_float(0x7FC00000);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if ((uint32_t)q < niso) {
-          const float d = (float)cnt[q];
-          out[3 * q + 0] = cnt[q] ? ax[q] / d : qnan;
-          out[3 * q + 1] = cnt[q] ? ay[q] / d : qnan;
-          out[3 * q + 2] = cnt[q] ? az[q] / d : qnan;
+        for (int q = 0; q < 4; ++q) {
+          if ((uint32_t)q < niso) {
+            const float d = (float)cnt[q];
+            out[3 * q + 0] = cnt[q] ? ax[q] / d : qnan;
+            out[3 * q + 1] = cnt[q] ? ay[q] / d : qnan;
+            out[3 * q + 2] = cnt[q] ? az[q] / d : qnan;
+          }
         }
+        uint32_t wv = niso;
+        if (b6) { out[3 * wv] = ex[0]; out[3 * wv + 1] = ey[0]; out[3 * wv + 2] = ez[0]; ++wv; }
+        if (b9) { out[3 * wv] = ex[1]; out[3 * wv + 1] = ey[1]; out[3 * wv + 2] = ez[1]; ++wv; }
+        if (b10) { out[3 * wv] = ex[2]; out[3 * wv + 1] = ey[2]; out[3 * wv + 2] = ez[2]; ++wv; }
+      } else {
+        atomicOr(a.status, kStatusVertexOverflow);
       }
-      uint32_t w = niso;
-      if (b6) { out[3 * w] = ex[0]; out[3 * w + 1] = ey[0]; out[3 * w + 2] = ez[0]; ++w; }
-      if (b9) { out[3 * w] = ex[1]; out[3 * w + 1] = ey[1]; out[3 * w + 2] = ez[1]; ++w; }
-      if (b10) { out[3 * w] = ex[2]; out[3 * w + 1] = ey[2]; out[3 * w + 2] = ez[2]; ++w; }
-    } else {
-      atomicOr(a.status, kStatusVertexOverflow);
-    }
 
-    // Quads (dmc.cu:1983-2047): rings in ccw order, reversed (0,3,2,1) when
-    // the owned edge's first corner is not above iso.
-    const bool xin = x + 1 < g.cx, yin = j + 1 < g.cy, zin = k + 1 < g.cz;
-    const bool e6 = b6 && xin && yin, e9 = b9 && xin && zin, e10 = b10 && yin && zin;
-    if (!(e6 || e9 || e10)) continue;
-    const uint32_t nq = (uint32_t)e6 + (uint32_t)e9 + (uint32_t)e10;
-    if ((uint64_t)qb + nq > a.qcap) { atomicOr(a.status, kStatusQuadOverflow); continue; }
-    const int32_t self = (int32_t)vb;
-    const int64_t p_x = (e6 || e9) ? next_x(a, s, pos, xl) : -1;
-    const int64_t p_y = (e6 || e10) ? find_in_segment(a, s + a.XT, xl) : -1;
-    const int64_t p_z = (e9 || e10) ? find_in_segment(a, s + a.XT * g.cy, xl) : -1;
-    bool ok = true;
-    int4 *qo = reinterpret_cast<int4 *>(a.quads) + qb;
-    uint32_t qn = 0;
-    if (e6) {
-      const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
-      const int32_t m1 = ring_vertex(a, s_lab, p_x, 7, ok);
-      const int32_t m2 = ring_vertex(a, s_lab, next_x(a, s + a.XT, p_y, xl), 4, ok);
-      const int32_t m3 = ring_vertex(a, s_lab, p_y, 5, ok);
-      const bool ccw = v[2] <= iso;
-      qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+      // Quads (dmc.cu:1983-2047): rings in ccw order, reversed (0,3,2,1) when
+      // the owned edge's first corner is not above iso.
+      const bool xin = x + 1 < g.cx, yin = j + 1 < g.cy, zin = k + 1 < g.cz;
+      const bool e6 = b6 && xin && yin, e9 = b9 && xin && zin, e10 = b10 && yin && zin;
+      if (!(e6 || e9 || e10)) continue;
+      const uint32_t nq = (uint32_t)e6 + (uint32_t)e9 + (uint32_t)e10;
+      if ((uint64_t)qb + nq > a.qcap) { atomicOr(a.status, kStatusQuadOverflow); continue; }
+      bool ok = true;
+      // Cell (x, row of segment s2) has rank r2 in s2; the cell one step in +x
+      // is rank r2 + 1 of s2 or, past the x-tile, rank 0 of the next x-tile's
+      // segment.  Returns its vertex for local edge e.
+      auto vertex_of = [&](uint32_t s2, int32_t r2, int e) -> int32_t {
+        if (r2 < 0 || (uint32_t)r2 >= cap) { ok = false; return -1; }
+        const uint32_t m2 = (uint32_t)a.slots[(size_t)s2 * cap + (uint32_t)r2];
+        const uint32_t lab = (uint32_t)(s_lab[rec_lut2(m2) * 256 + rec_cfg(m2)] >> (4 * e)) & 0xFu;
+        if (lab == 0xFu) { ok = false; return -1; }
+        return (int32_t)(a.seg_vq[s2].x + rec_lvb(m2) + lab);
+      };
+      // the +x neighbour of (segment s2, rank r2, x offset xo)
+      auto next_x = [&](uint32_t s2, int32_t r2, uint32_t xo, uint32_t &s_out) -> int32_t {
+        if (r2 < 0) return -1;
+        if (xo + 1u < cap) {
+          s_out = s2;
+          const int32_t r = r2 + 1;
+          if ((uint32_t)r >= cap || rec_x(a.slots[(size_t)s2 * cap + (uint32_t)r]) != xo + 1u) return -1;
+          return r;
+        }
+        s_out = s2 + 1u;  // same row, next x-tile (in the grid since x + 1 < cx)
+        if (rec_x(a.slots[(size_t)s_out * cap]) != 0u || (a.seg_counts[s_out].x & 0xFFFFu) == 0u) return -1;
+        return 0;
+      };
+      const uint32_t s_y = s + a.XT, s_z = s + a.XT * g.cy, s_yz = s_z + a.XT;
+      const int32_t r_y = (e6 || e10) ? rank_in_segment(a, s_y, xl) : -1;
+      const int32_t r_z = (e9 || e10) ? rank_in_segment(a, s_z, xl) : -1;
+      int4 *qo = reinterpret_cast<int4 *>(a.quads) + qb;
+      uint32_t qn = 0;
+      const int32_t self = (int32_t)vb;
+      if (e6) {
+        uint32_t sx, sxy;
+        const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
+        const int32_t rxy = next_x(s_y, r_y, xl, sxy);
+        const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
+        const int32_t m1 = vertex_of(sx, rx, 7);
+        const int32_t m2 = vertex_of(sxy, rxy, 4);
+        const int32_t m3 = vertex_of(s_y, r_y, 5);
+        const bool ccw = v[2] <= iso;
+        qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+      }
+      if (e9) {
+        uint32_t sx, sxz;
+        const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
+        const int32_t rxz = next_x(s_z, r_z, xl, sxz);
+        const int32_t m0 = self + (int32_t)((labels >> 36) & 0xFu);
+        const int32_t m1 = vertex_of(s_z, r_z, 1);
+        const int32_t m2 = vertex_of(sxz, rxz, 3);
+        const int32_t m3 = vertex_of(sx, rx, 11);
+        const bool ccw = v[5] <= iso;
+        qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+      }
+      if (e10) {
+        const int32_t r_yz = rank_in_segment(a, s_yz, xl);
+        const int32_t m0 = self + (int32_t)((labels >> 40) & 0xFu);
+        const int32_t m1 = vertex_of(s_y, r_y, 8);
+        const int32_t m2 = vertex_of(s_yz, r_yz, 0);
+        const int32_t m3 = vertex_of(s_z, r_z, 2);
+        const bool ccw = v[7] <= iso;
+        qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+      }
+      if (!ok) atomicOr(a.status, kStatusLookup);
     }
-    if (e9) {
-      const int32_t m0 = self + (int32_t)((labels >> 36) & 0xFu);
-      const int32_t m1 = ring_vertex(a, s_lab, p_z, 1, ok);
-      const int32_t m2 = ring_vertex(a, s_lab, next_x(a, s + a.XT * g.cy, p_z, xl), 3, ok);
-      const int32_t m3 = ring_vertex(a, s_lab, p_x, 11, ok);
-      const bool ccw = v[5] <= iso;
-      qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
-    }
-    if (e10) {
-      const int32_t m0 = self + (int32_t)((labels >> 40) & 0xFu);
-      const int32_t m1 = ring_vertex(a, s_lab, p_y, 8, ok);
-      const int32_t m2 = ring_vertex(a, s_lab, find_in_segment(a, s + a.XT + a.XT * g.cy, xl), 0, ok);
-      const int32_t m3 = ring_vertex(a, s_lab, p_z, 2, ok);
-      const bool ccw = v[7] <= iso;
-      qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
-    }
-    if (!ok) atomicOr(a.status, kStatusLookup);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Compact per-cell records into ascending order (parity and debugging).
+// Per-cell records in ascending cell order (parity and debugging): one
+// workgroup per non-empty segment (grid-stride over the list).
 
 struct CellsArgs {
-  uint32_t XT, cap, cx, cy, nsegs;
-  uint64_t max_chunks;
+  uint32_t XT, cap, cx, nsegs;
   const uint2 *seg_counts;
   const uint64_t *slots;
   const uint2 *seg_vq;
-  const uint32_t *ne_seg;
-  const uint32_t *ne_ap;
-  const uint32_t *chunk_ne;
+  const uint2 *ne;
   const Totals *totals;
   uint32_t *cell_index;
   uint8_t *cfg;
@@ -996,33 +1050,23 @@ struct CellsArgs {
 };
 
 __global__ __launch_bounds__(256) void k_cells(CellsArgs a) {
-  const Totals tot = *a.totals;
-  const uint32_t lane = lane_id(), wave = wave_id();
-  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
-  const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
-  for (uint32_t ch = blockIdx.x * 4 + wave; ch < num_chunks; ch += gridDim.x * 4) {
-    const uint32_t act = ch * 64u + lane;
-    const uint32_t u0 = a.chunk_ne[ch];
-    const uint32_t wl = (u0 < num_ne && u0 + lane < num_ne) ? a.ne_ap[u0 + lane] : 0xFFFFFFFFu;
-    uint32_t m = 0;
-#pragma unroll
-    for (uint32_t step = 32; step >= 1; step >>= 1) {
-      const uint32_t cand = m + step;
-      const uint32_t wc = __shfl(wl, (int)(cand & 63u), kWave);
-      if (cand < 64u && wc <= act) m = cand;
+  const uint32_t num_ne = min(a.totals->num_nonempty, a.nsegs);
+  for (uint32_t i = blockIdx.x; i < num_ne; i += gridDim.x) {
+    const uint2 e = a.ne[i];
+    const uint32_t sg = e.x;
+    if (sg >= a.nsegs) continue;
+    const uint32_t n = min(a.seg_counts[sg].x & 0xFFFFu, a.cap);
+    const uint32_t vb = a.seg_vq[sg].x;
+    const uint32_t xt = sg % a.XT, row = sg / a.XT;
+    for (uint32_t r = threadIdx.x; r < n; r += blockDim.x) {
+      const uint64_t pos = (uint64_t)e.y + r;
+      if (pos >= a.capacity) break;
+      const uint64_t rec = a.slots[(size_t)sg * a.cap + r];
+      a.cell_index[pos] = xt * a.cap + rec_x(rec) + a.cx * row;
+      a.cfg[pos] = (uint8_t)rec_cfg(rec);
+      a.flags[pos] = (uint8_t)rec_lut2(rec);
+      a.vbegin[pos] = vb + rec_lvb(rec);
     }
-    const uint32_t ap = __shfl(wl, (int)m, kWave);
-    if (act >= tot.num_active || act >= a.capacity) continue;
-    if (u0 >= num_ne || u0 + m >= num_ne) continue;
-    const uint32_t s = a.ne_seg[u0 + m];
-    if (s >= a.nsegs || act < ap || act - ap >= min(a.seg_counts[s].x & 0xFFFFu, a.cap)) continue;
-    const uint64_t rec = a.slots[(size_t)s * a.cap + (act - ap)];
-    const uint32_t xt = s % a.XT, row = s / a.XT;
-    const uint32_t x = xt * a.cap + rec_x(rec);
-    a.cell_index[act] = x + a.cx * row;
-    a.cfg[act] = (uint8_t)rec_cfg(rec);
-    a.flags[act] = (uint8_t)rec_lut2(rec);
-    a.vbegin[act] = a.seg_vq[s].x + rec_lvb(rec);
   }
 }
 
