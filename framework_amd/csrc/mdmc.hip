@@ -73,8 +73,13 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   p->cx = p->X - 1;
   p->cy = p->Y - 1;
   p->cz = p->Z - 1;
+  // Classify kernel: plane tiles (x-tiles of 256, 512 or 1024 cells), or the
+  // z-marching kernel when MDMC_CLASSIFY=zmarch (x-tiles from 64 cells; the
+  // two produce identical records, which the tests use as a cross-check).
+  const char *ck = std::getenv("MDMC_CLASSIFY");
+  p->zmarch = (ck != nullptr && std::strcmp(ck, "zmarch") == 0) ? 1 : 0;
   // x-tile: smallest 64*W covering the row, at most 1024 cells.
-  uint32_t W = 1;
+  uint32_t W = p->zmarch ? 1 : 4;
   while (W < 16 && 64u * W < p->cx) W <<= 1;
   uint32_t R = (W >= 8) ? 4 : 8;
   if (const char *t = std::getenv("MDMC_TILE")) {  // "W,R,Kc" override for tuning
@@ -106,9 +111,11 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   p->nslots = p->nsegs * p->cap;
   const uint64_t cells = (uint64_t)p->cx * p->cy * p->cz;
   p->max_chunks = (cells + 63) / 64 + 1;
-  p->ntiles = (uint32_t)((p->nsegs + kScanTile - 1) / kScanTile);
+  // whole scan tiles with at least one padding segment (its vertex / quad
+  // prefix is the grand total)
+  p->ntiles = (uint32_t)(p->nsegs / kScanTile + 1);
   p->classify_blocks = (uint64_t)p->XT * p->NRB * p->NKC;
-  p->vec4 = (p->X % 4 == 0) && W >= 4 && env_int("MDMC_VEC4", 1) != 0;
+  p->vec4 = (p->X % 4 == 0) && env_int("MDMC_VEC4", 1) != 0;
   p->ptR = pt_rows(W / 4);
   p->ptKP = pt_planes(W / 4);
   if (const char *t = std::getenv("MDMC_PT_SHAPE")) {  // "R,KP" override for tuning
@@ -146,6 +153,20 @@ struct mdmc_ctx {
   // generator tables
   void *gen = nullptr;
   size_t gen_bytes = 0;
+  // batches: device table of BatchVol + bases + the batch's own counts, and
+  // two pinned staging tables (count, emit) uploaded with async copies
+  void *bdev = nullptr;
+  size_t bdev_bytes = 0;
+  mdmc::BatchVol *bhost[2] = {nullptr, nullptr};
+  size_t bhost_cap = 0;  // entries per staging table
+  hipEvent_t bhost_ev[2] = {nullptr, nullptr};
+  bool bhost_ev_used[2] = {false, false};
+  bool batch_counted = false;
+  uint32_t batch_n = 0, batch_W4 = 0, batch_KP = 0, batch_tiles = 0;
+  uint64_t batch_key = 0;
+  mdmc::Plan batch_plan{};
+  // tuning options (0 = default)
+  int opt[4] = {0, 0, 0, 0};
   // per-kernel event timing
   bool profiling = false;
   std::vector<hipEvent_t> events;  // pool, used in (start, stop) pairs
@@ -266,9 +287,10 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   int rc;
   {
     ProfScope ps(ctx, 0, st);
-    if (p.vec4 && ((uintptr_t)d_field % 16 == 0)) {
+    if (!p.zmarch) {
       PtArgs pa{ca, p.band, p.NS};
-      rc = launch_classify_pt(p.W, p.ptR, p.ptKP, pa, (uint32_t)p.pt_blocks, st);
+      const bool vec = p.vec4 && ((uintptr_t)d_field % 16 == 0);
+      rc = launch_classify_pt(p.W, p.ptR, p.ptKP, vec, pa, (uint32_t)p.pt_blocks, st);
     } else {
       rc = launch_classify(p.W, p.R, ca, (uint32_t)p.classify_blocks, st);
     }
@@ -303,15 +325,15 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   if (p.empty) return MDMC_OK;
   const Scratch &s = ctx->s;
   EmitArgs ea{};
-  ea.g = FieldDesc{d_field, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
-  ea.mnx = mn[0];
-  ea.mny = mn[1];
-  ea.mnz = mn[2];
+  ea.v.g = FieldDesc{d_field, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
+  ea.v.mnx = mn[0];
+  ea.v.mny = mn[1];
+  ea.v.mnz = mn[2];
   // xyz_range = xyz_max - xyz_min in float (dmc.cu:2149)
-  ea.rgx = mx[0] - mn[0];
-  ea.rgy = mx[1] - mn[1];
-  ea.rgz = mx[2] - mn[2];
-  ea.XT = p.XT;
+  ea.v.rgx = mx[0] - mn[0];
+  ea.v.rgy = mx[1] - mn[1];
+  ea.v.rgz = mx[2] - mn[2];
+  ea.v.XT = p.XT;
   ea.cap = p.cap;
   ea.nsegs = (uint32_t)p.nsegs;
   ea.max_chunks = p.max_chunks;
@@ -322,17 +344,343 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ea.ne = s.ne;
   ea.chunk_ne = s.chunk_ne;
   ea.totals = s.totals;
-  ea.verts = d_vertices;
-  ea.vcap = d_vertices ? vcap : 0;
-  ea.quads = d_quads;
-  ea.qcap = d_quads ? qcap : 0;
-  ea.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
+  ea.v.verts = d_vertices;
+  ea.v.vcap = d_vertices ? vcap : 0;
+  ea.v.quads = d_quads;
+  ea.v.qcap = d_quads ? qcap : 0;
+  ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
   {
     ProfScope ps(ctx, 2, st);
     // one wave per chunk of 64 active cells, at most 8 workgroups per CU
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * 8));
     hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
+  }
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+
+constexpr uint32_t kBatchRows = 7;  // pt_rows for every x-tile width
+
+uint32_t batch_planes(const mdmc_ctx *ctx, uint32_t W4) {
+  const int kp = ctx->opt[MDMC_OPT_BATCH_PLANES];
+  if (W4 == 1 && (kp == 2 || kp == 4)) return (uint32_t)kp;
+  if (W4 == 2 && (kp == 1 || kp == 2)) return (uint32_t)kp;
+  return pt_planes(W4);
+}
+
+template <int W4, int KP>
+void launch_classify_batch_t(const BatchArgs &b, uint32_t blocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_classify_batch<W4, (int)kBatchRows, KP>), dim3(blocks), dim3(kPtThreads), 0, st, b);
+}
+
+int launch_classify_batch(uint32_t W4, uint32_t KP, const BatchArgs &b, uint32_t blocks, hipStream_t st) {
+  if (W4 == 1 && KP == 4) { launch_classify_batch_t<1, 4>(b, blocks, st); return 0; }
+  if (W4 == 1 && KP == 2) { launch_classify_batch_t<1, 2>(b, blocks, st); return 0; }
+  if (W4 == 2 && KP == 2) { launch_classify_batch_t<2, 2>(b, blocks, st); return 0; }
+  if (W4 == 2 && KP == 1) { launch_classify_batch_t<2, 1>(b, blocks, st); return 0; }
+  if (W4 == 4 && KP == 1) { launch_classify_batch_t<4, 1>(b, blocks, st); return 0; }
+  return -1;
+}
+
+uint64_t batch_key_of(const mdmc_volume *vols, uint32_t n) {
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ n;
+  auto mix = [&](uint64_t v) { h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2); };
+  for (uint32_t v = 0; v < n; ++v) {
+    mix((uint64_t)(uintptr_t)vols[v].field);
+    mix(((uint64_t)vols[v].dims[0] << 32) | vols[v].dims[1]);
+    mix(vols[v].dims[2]);
+    uint32_t iso;
+    std::memcpy(&iso, &vols[v].iso, 4);
+    mix(iso);
+  }
+  return h;
+}
+
+// Device table (n BatchVol), bases (n uint2), batch counts (4 words); two
+// pinned staging tables of n entries.
+int ensure_batch_buffers(mdmc_ctx *ctx, uint32_t n) {
+  const size_t o_base = align256(n * sizeof(BatchVol));
+  const size_t o_cnt = o_base + align256(n * sizeof(uint2));
+  const size_t need = o_cnt + 256;
+  if (need > ctx->bdev_bytes) {
+    if (ctx->bdev) (void)hipFree(ctx->bdev);
+    ctx->bdev = nullptr;
+    ctx->bdev_bytes = 0;
+    if (hipMalloc(&ctx->bdev, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(MDMC_ERR_NOMEM, "batch table allocation failed");
+    }
+    ctx->bdev_bytes = need;
+  }
+  if (n > ctx->bhost_cap) {
+    for (int i = 0; i < 2; ++i) {
+      if (ctx->bhost_ev_used[i]) HIP_TRY(hipEventSynchronize(ctx->bhost_ev[i]));
+      if (ctx->bhost[i]) (void)hipHostFree(ctx->bhost[i]);
+      ctx->bhost[i] = nullptr;
+    }
+    ctx->bhost_cap = 0;
+    for (int i = 0; i < 2; ++i) {
+      if (hipHostMalloc(reinterpret_cast<void **>(&ctx->bhost[i]), kMaxBatch * sizeof(BatchVol)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MDMC_ERR_NOMEM, "pinned staging allocation failed");
+      }
+    }
+    ctx->bhost_cap = kMaxBatch;
+  }
+  for (int i = 0; i < 2; ++i)
+    if (!ctx->bhost_ev[i]) HIP_TRY(hipEventCreateWithFlags(&ctx->bhost_ev[i], hipEventDisableTiming));
+  return MDMC_OK;
+}
+
+BatchVol *batch_table_dev(mdmc_ctx *ctx) { return static_cast<BatchVol *>(ctx->bdev); }
+uint2 *batch_base_dev(mdmc_ctx *ctx, uint32_t n) {
+  return reinterpret_cast<uint2 *>(static_cast<char *>(ctx->bdev) + align256(n * sizeof(BatchVol)));
+}
+uint32_t *batch_counts_dev(mdmc_ctx *ctx, uint32_t n) {
+  return reinterpret_cast<uint32_t *>(static_cast<char *>(ctx->bdev) + align256(n * sizeof(BatchVol)) +
+                                      align256(n * sizeof(uint2)));
+}
+
+// Staging table `slot` may be rewritten once its previous upload has run (not
+// while a graph is being captured: the captured copy reads it at replay).
+int staging_acquire(mdmc_ctx *ctx, int slot, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(st, &cs));
+  if (cs == hipStreamCaptureStatusNone && ctx->bhost_ev_used[slot])
+    HIP_TRY(hipEventSynchronize(ctx->bhost_ev[slot]));
+  return MDMC_OK;
+}
+
+int staging_upload(mdmc_ctx *ctx, int slot, uint32_t n, hipStream_t st) {
+  HIP_TRY(hipMemcpyAsync(ctx->bdev, ctx->bhost[slot], n * sizeof(BatchVol), hipMemcpyHostToDevice, st));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(st, &cs));
+  if (cs == hipStreamCaptureStatusNone) {
+    HIP_TRY(hipEventRecord(ctx->bhost_ev[slot], st));
+    ctx->bhost_ev_used[slot] = true;
+  }
+  return MDMC_OK;
+}
+
+__global__ void k_zero_words(uint32_t *w, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0u;
+}
+
+// Fills staging table `slot` for the batch; with_outputs: copy the output
+// pointers and capacities (the count pass leaves them out when absent).
+int fill_batch_table(mdmc_ctx *ctx, int slot, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                     uint32_t W4, uint32_t KP, const Scratch &S, uint64_t *nsegs_total, uint64_t *cells_total,
+                     uint32_t *tiles_total) {
+  const uint32_t cap = 256u * W4;
+  uint64_t seg_off = 0, tile_off = 0, cells = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const mdmc_volume &mv = vols[v];
+    BatchVol &b = ctx->bhost[slot][v];
+    std::memset(&b, 0, sizeof(b));
+    const uint32_t X = mv.dims[0], Y = mv.dims[1], Z = mv.dims[2];
+    const bool empty = X < 2 || Y < 2 || Z < 2;
+    const uint32_t cx = empty ? 0 : X - 1, cy = empty ? 0 : Y - 1, cz = empty ? 0 : Z - 1;
+    const uint32_t XT = empty ? 1 : (cx + cap - 1) / cap;
+    const uint32_t NRB = empty ? 0 : (cy + kBatchRows - 1) / kBatchRows;
+    const uint32_t NKP = empty ? 0 : (cz + KP - 1) / KP;
+    const uint64_t nsegs = (uint64_t)XT * cy * cz;
+    b.c.g = FieldDesc{mv.field, X, Y, Z, cx, cy, cz, mv.iso};
+    b.c.XT = XT;
+    b.c.seg_counts = S.seg_counts + seg_off;
+    b.c.slots = S.slots + seg_off * cap;
+    b.c.gmap = S.gmap + seg_off * (cap / 64);
+    b.tile_off = (uint32_t)tile_off;
+    b.NRB = NRB;
+    b.NKP = NKP;
+    b.seg_off = (uint32_t)seg_off;
+    b.nsegs = (uint32_t)nsegs;
+    b.mnx = mv.xyz_min[0];
+    b.mny = mv.xyz_min[1];
+    b.mnz = mv.xyz_min[2];
+    // xyz_range = xyz_max - xyz_min in float (dmc.cu:2149)
+    b.rgx = mv.xyz_max[0] - mv.xyz_min[0];
+    b.rgy = mv.xyz_max[1] - mv.xyz_min[1];
+    b.rgz = mv.xyz_max[2] - mv.xyz_min[2];
+    b.verts = mv.vertices;
+    b.vcap = mv.vertices ? mv.vertex_capacity : 0;
+    b.quads = mv.quads;
+    b.qcap = mv.quads ? mv.quad_capacity : 0;
+    b.counts = reinterpret_cast<uint32_t *>(d_counts + v);
+    seg_off += nsegs;
+    tile_off += (uint64_t)XT * NRB * NKP;
+    cells += (uint64_t)cx * cy * cz;
+  }
+  *nsegs_total = seg_off;
+  *cells_total = cells;
+  *tiles_total = (uint32_t)tile_off;
+  return MDMC_OK;
+}
+
+int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts, hipStream_t st) {
+  uint32_t W4 = 1;
+  uint64_t nseg_est = 0, tiles_est = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const mdmc_volume &mv = vols[v];
+    const bool empty = mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2;
+    if (empty) continue;
+    if (!mv.field) return fail(MDMC_ERR_INVALID, "null field in batch");
+    if ((uint64_t)mv.dims[0] * mv.dims[1] * mv.dims[2] > 0xFFFFFFFFull)
+      return fail(MDMC_ERR_TOO_LARGE, "grid has more than 2^32-1 points");
+    const uint32_t cx = mv.dims[0] - 1;
+    while (W4 < 4 && 256u * W4 < cx) W4 <<= 1;
+  }
+  const uint32_t KP = batch_planes(ctx, W4);
+  const uint32_t cap = 256u * W4;
+  for (uint32_t v = 0; v < n; ++v) {
+    const mdmc_volume &mv = vols[v];
+    if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
+    const uint64_t XT = (mv.dims[0] - 1 + cap - 1) / cap;
+    nseg_est += XT * (mv.dims[1] - 1) * (uint64_t)(mv.dims[2] - 1);
+    tiles_est += XT * ((mv.dims[1] - 1 + kBatchRows - 1) / kBatchRows) * ((mv.dims[2] - 1 + KP - 1) / KP);
+  }
+  if (nseg_est >= 0x7FFFFFFFull || tiles_est >= 0x7FFFFFFFull)
+    return fail(MDMC_ERR_TOO_LARGE, "batch has too many cell rows");
+  int rc = ensure_batch_buffers(ctx, n);
+  if (rc != MDMC_OK) return rc;
+  ctx->batch_counted = false;
+  ctx->counted = false;  // the scratch now holds this batch
+  if (nseg_est == 0) {  // nothing to extract: zero every volume's counts
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, reinterpret_cast<uint32_t *>(d_counts), 4u * n);
+    HIP_TRY(hipGetLastError());
+    ctx->batch_n = n;
+    ctx->batch_key = batch_key_of(vols, n);
+    ctx->batch_plan = Plan{};
+    ctx->batch_plan.empty = 1;
+    ctx->batch_counted = true;
+    return MDMC_OK;
+  }
+  // aggregate plan: what the scratch buffers and the scan see
+  Plan agg{};
+  agg.cap = cap;
+  agg.nsegs = nseg_est;
+  agg.nslots = nseg_est * cap;
+  agg.ntiles = (uint32_t)(nseg_est / kScanTile + 1);
+  rc = staging_acquire(ctx, 0, st);
+  if (rc != MDMC_OK) return rc;
+  // cells bound the chunk index; computed from the table below
+  uint64_t cells = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const mdmc_volume &mv = vols[v];
+    if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
+    cells += (uint64_t)(mv.dims[0] - 1) * (mv.dims[1] - 1) * (mv.dims[2] - 1);
+  }
+  agg.max_chunks = (cells + 63) / 64 + 1;
+  rc = ensure_scratch(ctx, agg);
+  if (rc != MDMC_OK) return rc;
+  const Scratch &S = ctx->s;
+  uint64_t nsegs = 0, cells2 = 0;
+  uint32_t tiles = 0;
+  rc = fill_batch_table(ctx, 0, vols, n, d_counts, W4, KP, S, &nsegs, &cells2, &tiles);
+  if (rc != MDMC_OK) return rc;
+  rc = staging_upload(ctx, 0, n, st);
+  if (rc != MDMC_OK) return rc;
+
+  uint32_t *bcounts = batch_counts_dev(ctx, n);
+  BatchArgs ba{};
+  ba.vols = batch_table_dev(ctx);
+  ba.nvol = n;
+  ba.total_tiles = tiles;
+  ba.zero.scan_state = S.scan_state;
+  ba.zero.scan_state_words = agg.ntiles * 4;
+  ba.zero.seg_pad = S.seg_counts + nsegs;
+  ba.zero.seg_pad_n = (uint32_t)((uint64_t)agg.ntiles * kScanTile - nsegs);
+  ba.zero.scan_ticket = S.ticket;
+  ba.zero.totals = S.totals;
+  ba.zero.counts_out = bcounts;
+  {
+    ProfScope ps(ctx, 0, st);
+    const int bpc = ctx->opt[MDMC_OPT_CLASSIFY_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_CLASSIFY_BLOCKS_PER_CU] : 6;
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * bpc));
+    if (launch_classify_batch(W4, KP, ba, blocks, st) != 0) return fail(MDMC_ERR_INVALID, "no batch classify kernel");
+  }
+  HIP_TRY(hipGetLastError());
+  ScanArgs sa{};
+  sa.seg_counts = S.seg_counts;
+  sa.nsegs = (uint32_t)nsegs;
+  sa.ntiles = agg.ntiles;
+  sa.max_chunks = agg.max_chunks;
+  sa.seg_vq = S.seg_vq;
+  sa.ne = S.ne;
+  sa.chunk_ne = S.chunk_ne;
+  sa.state = S.scan_state;
+  sa.ticket = S.ticket;
+  sa.totals = S.totals;
+  sa.counts_out = bcounts;
+  {
+    ProfScope ps(ctx, 1, st);
+    hipLaunchKernelGGL(k_scan, dim3(agg.ntiles), dim3(kScanThreads), 0, st, sa);
+  }
+  HIP_TRY(hipGetLastError());
+  BatchCountArgs bc{};
+  bc.vols = batch_table_dev(ctx);
+  bc.nvol = n;
+  bc.seg_counts = S.seg_counts;
+  bc.seg_vq = S.seg_vq;
+  bc.base = batch_base_dev(ctx, n);
+  bc.batch_counts = bcounts;
+  {
+    ProfScope ps(ctx, 3, st);
+    hipLaunchKernelGGL(k_batch_counts, dim3(std::min<uint32_t>(n, (uint32_t)ctx->num_cus * 4)), dim3(256), 0, st, bc);
+  }
+  HIP_TRY(hipGetLastError());
+  agg.nsegs = nsegs;
+  ctx->batch_plan = agg;
+  ctx->batch_n = n;
+  ctx->batch_W4 = W4;
+  ctx->batch_KP = KP;
+  ctx->batch_tiles = tiles;
+  ctx->batch_key = batch_key_of(vols, n);
+  ctx->batch_counted = true;
+  return MDMC_OK;
+}
+
+int batch_emit(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts, hipStream_t st,
+               bool upload) {
+  if (!ctx->batch_counted || ctx->batch_n != n || ctx->batch_key != batch_key_of(vols, n))
+    return fail(MDMC_ERR_INVALID, "batch emit without a matching batch count on this context");
+  const Plan &agg = ctx->batch_plan;
+  if (agg.empty) return MDMC_OK;
+  const Scratch &S = ctx->s;
+  if (upload) {
+    int rc = staging_acquire(ctx, 1, st);
+    if (rc != MDMC_OK) return rc;
+    uint64_t nsegs = 0, cells = 0;
+    uint32_t tiles = 0;
+    rc = fill_batch_table(ctx, 1, vols, n, d_counts, ctx->batch_W4, ctx->batch_KP, S, &nsegs, &cells, &tiles);
+    if (rc != MDMC_OK) return rc;
+    rc = staging_upload(ctx, 1, n, st);
+    if (rc != MDMC_OK) return rc;
+  }
+  BatchEmitArgs be{};
+  be.a.cap = agg.cap;
+  be.a.nsegs = (uint32_t)agg.nsegs;
+  be.a.max_chunks = agg.max_chunks;
+  be.a.seg_counts = S.seg_counts;
+  be.a.seg_vq = S.seg_vq;
+  be.a.slots = S.slots;
+  be.a.gmap = S.gmap;
+  be.a.ne = S.ne;
+  be.a.chunk_ne = S.chunk_ne;
+  be.a.totals = S.totals;
+  be.a.v.status = batch_counts_dev(ctx, n) + 3;
+  be.vols = batch_table_dev(ctx);
+  be.base = batch_base_dev(ctx, n);
+  be.nvol = n;
+  {
+    ProfScope ps(ctx, 2, st);
+    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+    const uint64_t want = (agg.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
+    hipLaunchKernelGGL(k_emit_batch, dim3(blocks), dim3(kEmitThreads), 0, st, be);
   }
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
@@ -448,6 +796,11 @@ void mdmc_ctx_destroy(mdmc_ctx *ctx) {
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->gen) (void)hipFree(ctx->gen);
+  if (ctx->bdev) (void)hipFree(ctx->bdev);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->bhost[i]) (void)hipHostFree(ctx->bhost[i]);
+    if (ctx->bhost_ev[i]) (void)hipEventDestroy(ctx->bhost_ev[i]);
+  }
   delete ctx;
 }
 
@@ -489,6 +842,44 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
   if (rc != MDMC_OK) return rc;
   return mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_vertices, vertex_capacity, d_quads,
                           quad_capacity, d_counts, stream);
+}
+
+int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value) {
+  if (!ctx) return fail(MDMC_ERR_INVALID, "null ctx");
+  if (option < 0 || option >= 4) return fail(MDMC_ERR_INVALID, "unknown option");
+  ctx->opt[option] = value;
+  return MDMC_OK;
+}
+
+static int check_batch_args(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts) {
+  if (!ctx || !vols || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
+  if (n == 0 || n > kMaxBatch) return fail(MDMC_ERR_INVALID, "batch size must be 1..1024");
+  if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  return MDMC_OK;
+}
+
+int mdmc_batch_count_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                            void *stream) {
+  int rc = check_batch_args(ctx, vols, n, d_counts);
+  if (rc != MDMC_OK) return rc;
+  return batch_count(ctx, vols, n, d_counts, static_cast<hipStream_t>(stream));
+}
+
+int mdmc_batch_emit_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                           void *stream) {
+  int rc = check_batch_args(ctx, vols, n, d_counts);
+  if (rc != MDMC_OK) return rc;
+  return batch_emit(ctx, vols, n, d_counts, static_cast<hipStream_t>(stream), true);
+}
+
+int mdmc_batch_extract_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                              void *stream) {
+  int rc = check_batch_args(ctx, vols, n, d_counts);
+  if (rc != MDMC_OK) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  rc = batch_count(ctx, vols, n, d_counts, st);
+  if (rc != MDMC_OK) return rc;
+  return batch_emit(ctx, vols, n, d_counts, st, false);
 }
 
 int mdmc_cells_device(mdmc_ctx *ctx, uint32_t *d_cell_index, uint8_t *d_cfg, uint8_t *d_flags,

@@ -17,7 +17,8 @@ struct Plan {
   uint32_t W, R, Kc, cap, XT, NRB, NKC, ntiles;
   uint64_t nsegs, nslots, max_chunks, classify_blocks;
   int empty;
-  int vec4;  // rows are a multiple of 4 points and span >= 256 cells
+  int vec4;    // rows are a multiple of 4 points
+  int zmarch;  // use the z-marching classify kernel
   uint32_t ptR, ptKP;        // plane tiles: cell rows, cell planes
   uint32_t band, NS;         // plane tiles: row blocks per XCD band, bands per plane block
   uint64_t pt_blocks;        // plane-tile grid
@@ -63,30 +64,29 @@ inline int launch_classify_w(uint32_t R, const ClassifyArgs &a, uint32_t blocks,
 }
 
 template <int W4, int R, int KP>
-inline void launch_classify_pt_t(const PtArgs &a, uint32_t blocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_classify_pt<W4, R, KP>), dim3(blocks), dim3(kPtThreads), 0, st, a);
+inline void launch_classify_pt_t(bool vec, const PtArgs &a, uint32_t blocks, hipStream_t st) {
+  if (vec)
+    hipLaunchKernelGGL((k_classify_pt<W4, R, KP, true>), dim3(blocks), dim3(kPtThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_classify_pt<W4, R, KP, false>), dim3(blocks), dim3(kPtThreads), 0, st, a);
 }
 
 // Compiled plane-tile shapes besides the defaults (pt_rows, pt_planes).
 inline bool pt_shape_supported(uint32_t W4, int R, int KP) {
-  if (W4 == 2) return (R == 7 && (KP == 1 || KP == 2 || KP == 3)) || (R == 11 && (KP == 1 || KP == 2)) || (R == 3 && KP == 4);
+  if (W4 == 2) return R == 7 && (KP == 2 || KP == 3);
   return (uint32_t)R == pt_rows(W4) && (uint32_t)KP == pt_planes(W4);
 }
 
-// Plane-tile kernel for rows of a multiple of 4 points (W = 4, 8 or 16).
-inline int launch_classify_pt(uint32_t W, uint32_t R, uint32_t KP, const PtArgs &a, uint32_t blocks,
-                              hipStream_t st) {
+// Plane-tile kernel (W = 4, 8 or 16); vec: 16-byte row loads.
+inline int launch_classify_pt(uint32_t W, uint32_t R, uint32_t KP, bool vec, const PtArgs &a,
+                              uint32_t blocks, hipStream_t st) {
   switch (W) {
-    case 4: launch_classify_pt_t<1, (int)pt_rows(1), (int)pt_planes(1)>(a, blocks, st); return 0;
+    case 4: launch_classify_pt_t<1, (int)pt_rows(1), (int)pt_planes(1)>(vec, a, blocks, st); return 0;
     case 8:
-      if (R == 7 && KP == 1) { launch_classify_pt_t<2, 7, 1>(a, blocks, st); return 0; }
-      if (R == 7 && KP == 2) { launch_classify_pt_t<2, 7, 2>(a, blocks, st); return 0; }
-      if (R == 7 && KP == 3) { launch_classify_pt_t<2, 7, 3>(a, blocks, st); return 0; }
-      if (R == 11 && KP == 1) { launch_classify_pt_t<2, 11, 1>(a, blocks, st); return 0; }
-      if (R == 11 && KP == 2) { launch_classify_pt_t<2, 11, 2>(a, blocks, st); return 0; }
-      if (R == 3 && KP == 4) { launch_classify_pt_t<2, 3, 4>(a, blocks, st); return 0; }
+      if (R == 7 && KP == 2) { launch_classify_pt_t<2, 7, 2>(vec, a, blocks, st); return 0; }
+      if (R == 7 && KP == 3) { launch_classify_pt_t<2, 7, 3>(vec, a, blocks, st); return 0; }
       return -1;
-    case 16: launch_classify_pt_t<4, (int)pt_rows(4), (int)pt_planes(4)>(a, blocks, st); return 0;
+    case 16: launch_classify_pt_t<4, (int)pt_rows(4), (int)pt_planes(4)>(vec, a, blocks, st); return 0;
     default: return -1;
   }
 }

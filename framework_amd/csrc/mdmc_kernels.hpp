@@ -379,40 +379,46 @@ constexpr uint32_t pt_rows(uint32_t) { return 7u; }
 constexpr uint32_t pt_planes(uint32_t W4) { return W4 >= 4 ? 1u : (W4 == 2 ? 2u : 4u); }
 
 template <int W4, int R, int KP>
-__global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
+struct PtShape {
   static_assert(W4 >= 1 && W4 <= 4, "x-tile is at most 1024 cells");
   static_assert((R + 1) % 4 == 0 && R <= 31, "point rows are split evenly over 4 waves");
   static_assert(KP >= 1 && KP <= 4, "one to four cell planes per tile");
+  static constexpr uint32_t kRowWords = W4 * kWave;           // LDS words per cell row
+  static constexpr uint32_t kPlaneWords = kRowWords * (R + 2);  // incl. two padding rows
+  static constexpr uint32_t kLdsWords = KP * kPlaneWords;
+};
+
+// Table indexed by key: c_cfg_count of the key's configuration | cfg << 16.
+__device__ __forceinline__ void pt_init_table(uint32_t *s_tab) {
+  const uint32_t cfg = key_to_cfg(threadIdx.x);
+  s_tab[threadIdx.x] = (uint32_t)c_cfg_count[cfg] | (cfg << 16);
+}
+
+// Zeroes the state the scan expects to start from (first workgroup only).
+__device__ __forceinline__ void zero_scan_state(const ClassifyArgs &a, uint32_t tid, uint32_t nthreads) {
+  for (uint32_t i = tid; i < a.scan_state_words; i += nthreads) a.scan_state[i] = 0ull;
+  for (uint32_t i = tid; i < a.seg_pad_n; i += nthreads) a.seg_pad[i] = make_uint2(0u, 0u);
+  if (tid < 8) reinterpret_cast<uint32_t *>(a.totals)[tid] = 0u;
+  if (tid == 0) *a.scan_ticket = 0u;
+}
+
+// One plane tile (x-tile xt, row block rb, plane block kb) of the volume `a`.
+// VEC: rows start on 16-byte boundaries and hold a multiple of 4 points, so a
+// lane's 4 points are one 16-byte load; otherwise 4 separate loads.
+template <int W4, int R, int KP, bool VEC>
+__device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t xt, uint32_t rb,
+                                                 uint32_t kb, uint32_t *s_w, const uint32_t *s_tab) {
+  using S = PtShape<W4, R, KP>;
   constexpr int kRowsPerWave = (R + 1) / 4;
   constexpr uint32_t CAP = 256u * W4;
-  constexpr uint32_t kRowWords = W4 * kWave;           // LDS words per cell row
-  constexpr uint32_t kPlaneWords = kRowWords * (R + 2);  // incl. two padding rows
-  __shared__ uint32_t s_w[KP * kPlaneWords];
-  __shared__ uint32_t s_tab[256];
-  const ClassifyArgs &a = pa.c;
-  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = wave_id();
-  {
-    const uint32_t cfg = key_to_cfg(tid);
-    s_tab[tid] = (uint32_t)c_cfg_count[cfg] | (cfg << 16);
-  }
-  if (blockIdx.x == 0) {
-    for (uint32_t i = tid; i < a.scan_state_words; i += kPtThreads) a.scan_state[i] = 0ull;
-    for (uint32_t i = tid; i < a.seg_pad_n; i += kPtThreads) a.seg_pad[i] = make_uint2(0u, 0u);
-    if (tid < 8) reinterpret_cast<uint32_t *>(a.totals)[tid] = 0u;
-    if (tid == 0) *a.scan_ticket = 0u;
-    if (a.counts_out != nullptr && tid < 4) a.counts_out[tid] = 0u;
-  }
-
+  constexpr uint32_t kRowWords = S::kRowWords;
+  constexpr uint32_t kPlaneWords = S::kPlaneWords;
+  const uint32_t lane = lane_id(), wave = wave_id();
   const FieldDesc g = a.g;
   const float iso = g.iso;
-  const uint32_t s_idx = blockIdx.x >> 3;  // position in this XCD's sequence
-  const uint32_t xt = s_idx % a.XT;
-  const uint32_t u = s_idx / a.XT;
-  const uint32_t u2 = u / pa.B;
-  const uint32_t rb = ((u2 % pa.NS) * 8u + (blockIdx.x & 7u)) * pa.B + u % pa.B;
-  const uint32_t k = (u2 / pa.NS) * KP;  // first cell plane of the tile
+  const uint32_t k = kb * KP;  // first cell plane of the tile
   const uint32_t j0 = rb * R;
-  const bool tile_ok = (k < g.cz) && (j0 < g.cy);  // band padding does nothing
+  const bool tile_ok = (k < g.cz) && (j0 < g.cy);  // padding tiles do nothing
   const uint32_t x0 = xt * CAP;
   const size_t plane = (size_t)g.X * g.Y;
   const bool has_extra = x0 + CAP < g.X;
@@ -424,9 +430,16 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   if (tile_ok) {
     float4 v[kRowsPerWave][KP + 1][W4];
     float ve[kRowsPerWave][KP + 1];
-    uint32_t xob[W4];
+    uint32_t xob[W4][VEC ? 1 : 4];  // byte offsets of the lane's points in a row
 #pragma unroll
-    for (int c = 0; c < W4; ++c) xob[c] = min(x0 + 256u * c + 4u * lane, g.X - 4u) * 4u;
+    for (int c = 0; c < W4; ++c) {
+      if constexpr (VEC) {
+        xob[c][0] = min(x0 + 256u * c + 4u * lane, g.X - 4u) * 4u;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xob[c][e] = min(x0 + 256u * c + 4u * lane + (uint32_t)e, g.X - 1u) * 4u;
+      }
+    }
     const uint32_t xe = has_extra ? x0 + CAP : 0u;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; ++i) {
@@ -434,9 +447,18 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
 #pragma unroll
       for (int p = 0; p <= KP; ++p) {
         const float *rowp = g.f + (size_t)min(k + p, g.Z - 1u) * plane + yoff;
+        const char *rowb = reinterpret_cast<const char *>(rowp);
 #pragma unroll
-        for (int c = 0; c < W4; ++c)
-          v[i][p][c] = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(rowp) + xob[c]);
+        for (int c = 0; c < W4; ++c) {
+          if constexpr (VEC) {
+            v[i][p][c] = *reinterpret_cast<const float4 *>(rowb + xob[c][0]);
+          } else {
+            v[i][p][c].x = *reinterpret_cast<const float *>(rowb + xob[c][0]);
+            v[i][p][c].y = *reinterpret_cast<const float *>(rowb + xob[c][1]);
+            v[i][p][c].z = *reinterpret_cast<const float *>(rowb + xob[c][2]);
+            v[i][p][c].w = *reinterpret_cast<const float *>(rowb + xob[c][3]);
+          }
+        }
         ve[i][p] = rowp[xe];
       }
     }
@@ -476,7 +498,7 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
     }
   }
   __syncthreads();
-  if (!tile_ok) return;
+  if (!tile_ok) return;  // uniform over the workgroup
 
   // ---- phase 2: each wave classifies cell rows wave, wave+4, ... of the
   // KP*R rows of the tile
@@ -584,6 +606,85 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
       cnt_q += tot >> 20;
     }
     if (lane == 0) a.seg_counts[seg] = make_uint2(cnt_a | (cnt_q << 16), cnt_v);
+  }
+}
+
+// Single volume: block b -> tile through the XCD band mapping.
+template <int W4, int R, int KP, bool VEC>
+__global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
+  __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
+  __shared__ uint32_t s_tab[256];
+  const ClassifyArgs &a = pa.c;
+  pt_init_table(s_tab);
+  if (blockIdx.x == 0) {
+    zero_scan_state(a, threadIdx.x, kPtThreads);
+    if (a.counts_out != nullptr && threadIdx.x < 4) a.counts_out[threadIdx.x] = 0u;
+  }
+  const uint32_t s_idx = blockIdx.x >> 3;  // position in this XCD's sequence
+  const uint32_t xt = s_idx % a.XT;
+  const uint32_t u = s_idx / a.XT;
+  const uint32_t u2 = u / pa.B;
+  const uint32_t rb = ((u2 % pa.NS) * 8u + (blockIdx.x & 7u)) * pa.B + u % pa.B;
+  classify_pt_tile<W4, R, KP, VEC>(a, xt, rb, u2 / pa.NS, s_w, s_tab);
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many volumes in one launch sequence.  The segments of all volumes
+// are concatenated (volume v owns segments [seg_off, seg_off + nsegs_v)), so
+// the scan runs once over the whole batch and a volume's local vertex / quad
+// numbers are the global ones minus the prefix at its first segment.
+
+constexpr uint32_t kMaxBatch = 1024;  // volumes per batch call
+
+struct BatchVol {
+  ClassifyArgs c;         // per-volume field and tiling; record pointers start at seg_off
+  uint32_t tile_off;      // first flattened classify tile
+  uint32_t NRB, NKP;      // row blocks and plane blocks of the volume
+  uint32_t seg_off, nsegs;
+  float mnx, mny, mnz, rgx, rgy, rgz;
+  float *verts;
+  uint64_t vcap;
+  int32_t *quads;
+  uint64_t qcap;
+  uint32_t *counts;       // the volume's mdmc_counts
+  uint32_t vbase, qbase;  // written by k_batch_counts: prefix at seg_off
+};
+
+struct BatchArgs {
+  BatchVol *vols;
+  uint32_t nvol, total_tiles;
+  ClassifyArgs zero;      // scan state of the batch (zeroed by block 0)
+};
+
+// Persistent classify over the flattened tiles of all volumes (block b takes
+// tiles b, b + G, ...; tiles of a volume are in memory order).
+template <int W4, int R, int KP>
+__global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
+  __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
+  __shared__ uint32_t s_tab[256];
+  __shared__ uint32_t s_toff[kMaxBatch + 1];
+  pt_init_table(s_tab);
+  for (uint32_t i = threadIdx.x; i <= b.nvol; i += kPtThreads)
+    s_toff[i] = i < b.nvol ? b.vols[i].tile_off : b.total_tiles;
+  if (blockIdx.x == 0) {
+    zero_scan_state(b.zero, threadIdx.x, kPtThreads);
+    if (threadIdx.x < 4) b.zero.counts_out[threadIdx.x] = 0u;  // the batch's scan counts
+  }
+  __syncthreads();
+  for (uint32_t t = blockIdx.x; t < b.total_tiles; t += gridDim.x) {
+    // volume of tile t: the last v with tile_off[v] <= t
+    uint32_t lo = 0, hi = b.nvol;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_toff[mid] <= t) lo = mid; else hi = mid;
+    }
+    const uint32_t v = __builtin_amdgcn_readfirstlane(lo);
+    const BatchVol &bv = b.vols[v];
+    const ClassifyArgs &a = bv.c;
+    const uint32_t local = t - s_toff[v];
+    const uint32_t xt = local % a.XT, u = local / a.XT;
+    __syncthreads();  // the previous tile's LDS words are no longer read
+    classify_pt_tile<W4, R, KP, false>(a, xt, u % bv.NRB, u / bv.NRB, s_w, s_tab);
   }
 }
 
@@ -785,10 +886,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
 
 constexpr int kEmitThreads = 256;
 
-struct EmitArgs {
+// Per-volume part of the emit arguments.
+struct EmitVol {
   FieldDesc g;
   float mnx, mny, mnz, rgx, rgy, rgz;  // xyz_min and xyz_max - xyz_min
-  uint32_t XT, cap, nsegs;
+  uint32_t XT, seg_off;                // x-tiles; first segment of the volume
+  uint32_t vbase, qbase;               // global vertex / quad number of its first
+  float *verts;
+  uint64_t vcap;
+  int32_t *quads;
+  uint64_t qcap;
+  uint32_t *status;
+};
+
+struct EmitArgs {
+  EmitVol v;  // the volume (single-volume launches)
+  uint32_t cap, nsegs;
   uint64_t max_chunks;  // capacity of chunk_ne
   const uint2 *seg_counts;
   const uint2 *seg_vq;
@@ -797,11 +910,6 @@ struct EmitArgs {
   const uint2 *ne;
   const uint32_t *chunk_ne;
   const Totals *totals;
-  float *verts;
-  uint64_t vcap;
-  int32_t *quads;
-  uint64_t qcap;
-  uint32_t *status;
 };
 
 // utils::ijk_to_xyz (include/utils.hpp:118-122): i / n * range + min, three
@@ -820,214 +928,313 @@ __device__ __forceinline__ int32_t rank_in_segment(const EmitArgs &a, uint32_t s
   return (int32_t)(e.z + (uint32_t)__popcll(mask & ((1ull << b) - 1ull)));
 }
 
+// One active cell (segment s, rank within it) of volume vol: its vertices and
+// the quads of its owned edges.
+__device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
+                                          const uint64_t *s_lab) {
+  const FieldDesc &g = vol.g;
+  const float iso = g.iso;
+  const size_t sy = g.X, sz = (size_t)g.X * g.Y;
+  const uint32_t cap = a.cap;
+  const uint64_t rec = a.slots[(size_t)s * cap + rank];
+  const uint32_t xl = rec_x(rec), cfg = rec_cfg(rec), lut2 = rec_lut2(rec);
+  const uint32_t sl = s - vol.seg_off;
+  const uint32_t xt = sl % vol.XT, row = sl / vol.XT;
+  const uint32_t j = row % g.cy, k = row / g.cy;
+  const uint32_t x = xt * cap + xl;
+  if (x >= g.cx || xl >= cap) { atomicOr(vol.status, kStatusInternal); return; }
+  const uint2 vq = a.seg_vq[s];
+  const uint32_t vb = vq.x + rec_lvb(rec) - vol.vbase;  // volume-local numbers
+  const uint32_t qb = vq.y + rec_lqb(rec) - vol.qbase;
+
+  const float *p = g.f + (size_t)k * sz + (size_t)j * sy + x;
+  float v[8];
+  v[0] = p[0];
+  v[1] = p[1];
+  v[2] = p[sy + 1];
+  v[3] = p[sy];
+  v[4] = p[sz];
+  v[5] = p[sz + 1];
+  v[6] = p[sz + sy + 1];
+  v[7] = p[sz + sy];
+
+  const float X0 = coordf(x, g.cx, vol.rgx, vol.mnx), X1 = coordf(x + 1, g.cx, vol.rgx, vol.mnx);
+  const float Y0 = coordf(j, g.cy, vol.rgy, vol.mny), Y1 = coordf(j + 1, g.cy, vol.rgy, vol.mny);
+  const float Z0 = coordf(k, g.cz, vol.rgz, vol.mnz), Z1 = coordf(k + 1, g.cz, vol.rgz, vol.mnz);
+
+  // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
+  // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
+  // point is bitwise the one the owner writes (dmc.cu:1468-1514).
+  constexpr uint8_t EF[12] = {1, 1, 2, 0, 0, 1, 2, 3, 5, 5, 6, 4};
+  constexpr uint8_t ET[12] = {0, 2, 3, 3, 4, 5, 6, 7, 4, 6, 7, 7};
+  // Owner of each edge lies at a non-positive offset; an edge is skipped
+  // when its owner would be outside the grid (dmc.cu:1539-1545).
+  // bit 0: needs x>0, bit 1: needs y>0, bit 2: needs z>0
+  constexpr uint8_t NEED[12] = {6, 4, 4, 5, 3, 2, 0, 1, 2, 0, 0, 1};
+  const uint32_t have = (x > 0 ? 1u : 0u) | (j > 0 ? 2u : 0u) | (k > 0 ? 4u : 0u);
+
+  const uint64_t labels = s_lab[lut2 * 256 + cfg];
+  float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  float ex[3] = {0, 0, 0}, ey[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
+    if (lab == 0xFu) continue;
+    const int f = EF[e], to = ET[e];
+    const float vf = v[f], vt = v[to];
+    const float tt = (iso - vf) / (vt - vf);
+    const float omt = 1.0f - tt;
+    const float fx = (f == 1 || f == 2 || f == 5 || f == 6) ? X1 : X0;
+    const float fy = (f == 2 || f == 3 || f == 6 || f == 7) ? Y1 : Y0;
+    const float fz = (f >= 4) ? Z1 : Z0;
+    const float tx = (to == 1 || to == 2 || to == 5 || to == 6) ? X1 : X0;
+    const float ty = (to == 2 || to == 3 || to == 6 || to == 7) ? Y1 : Y0;
+    const float tz = (to >= 4) ? Z1 : Z0;
+    const float px = fx * omt + tx * tt;
+    const float py = fy * omt + ty * tt;
+    const float pz = fz * omt + tz * tt;
+    if (e == 6) { ex[0] = px; ey[0] = py; ez[0] = pz; }
+    if (e == 9) { ex[1] = px; ey[1] = py; ez[1] = pz; }
+    if (e == 10) { ex[2] = px; ey[2] = py; ez[2] = pz; }
+    if ((NEED[e] & have) != NEED[e]) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (lab == (uint32_t)q) {
+        if (cnt[q] == 0) { ax[q] = px; ay[q] = py; az[q] = pz; }
+        else { ax[q] = ax[q] + px; ay[q] = ay[q] + py; az[q] = az[q] + pz; }
+        ++cnt[q];
+      }
+    }
+  }
+  const uint16_t info = c_cfg_info[cfg];
+  const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
+  const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
+  const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
+  const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
+  const uint32_t nv = niso + b6 + b9 + b10;
+  if ((uint64_t)vb + nv <= vol.vcap) {
+    float *out = vol.verts + 3ull * vb;
+    const float qnan = __int_as_float(0x7FC00000);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if ((uint32_t)q < niso) {
+        const float d = (float)cnt[q];
+        out[3 * q + 0] = cnt[q] ? ax[q] / d : qnan;
+        out[3 * q + 1] = cnt[q] ? ay[q] / d : qnan;
+        out[3 * q + 2] = cnt[q] ? az[q] / d : qnan;
+      }
+    }
+    uint32_t wv = niso;
+    if (b6) { out[3 * wv] = ex[0]; out[3 * wv + 1] = ey[0]; out[3 * wv + 2] = ez[0]; ++wv; }
+    if (b9) { out[3 * wv] = ex[1]; out[3 * wv + 1] = ey[1]; out[3 * wv + 2] = ez[1]; ++wv; }
+    if (b10) { out[3 * wv] = ex[2]; out[3 * wv + 1] = ey[2]; out[3 * wv + 2] = ez[2]; ++wv; }
+  } else {
+    atomicOr(vol.status, kStatusVertexOverflow);
+  }
+
+  // Quads (dmc.cu:1983-2047): rings in ccw order, reversed (0,3,2,1) when
+  // the owned edge's first corner is not above iso.
+  const bool xin = x + 1 < g.cx, yin = j + 1 < g.cy, zin = k + 1 < g.cz;
+  const bool e6 = b6 && xin && yin, e9 = b9 && xin && zin, e10 = b10 && yin && zin;
+  if (!(e6 || e9 || e10)) return;
+  const uint32_t nq = (uint32_t)e6 + (uint32_t)e9 + (uint32_t)e10;
+  if ((uint64_t)qb + nq > vol.qcap) { atomicOr(vol.status, kStatusQuadOverflow); return; }
+  bool ok = true;
+  // Cell (x, row of segment s2) has rank r2 in s2; the cell one step in +x
+  // is rank r2 + 1 of s2 or, past the x-tile, rank 0 of the next x-tile's
+  // segment.  Returns its vertex for local edge e.
+  auto vertex_of = [&](uint32_t s2, int32_t r2, int e) -> int32_t {
+    if (r2 < 0 || (uint32_t)r2 >= cap) { ok = false; return -1; }
+    const uint32_t m2 = (uint32_t)a.slots[(size_t)s2 * cap + (uint32_t)r2];
+    const uint32_t lab = (uint32_t)(s_lab[rec_lut2(m2) * 256 + rec_cfg(m2)] >> (4 * e)) & 0xFu;
+    if (lab == 0xFu) { ok = false; return -1; }
+    return (int32_t)(a.seg_vq[s2].x + rec_lvb(m2) + lab - vol.vbase);
+  };
+  // the +x neighbour of (segment s2, rank r2, x offset xo)
+  auto next_x = [&](uint32_t s2, int32_t r2, uint32_t xo, uint32_t &s_out) -> int32_t {
+    if (r2 < 0) return -1;
+    if (xo + 1u < cap) {
+      s_out = s2;
+      const int32_t r = r2 + 1;
+      if ((uint32_t)r >= cap || rec_x(a.slots[(size_t)s2 * cap + (uint32_t)r]) != xo + 1u) return -1;
+      return r;
+    }
+    s_out = s2 + 1u;  // same row, next x-tile (in the grid since x + 1 < cx)
+    if (rec_x(a.slots[(size_t)s_out * cap]) != 0u || (a.seg_counts[s_out].x & 0xFFFFu) == 0u) return -1;
+    return 0;
+  };
+  const uint32_t s_y = s + vol.XT, s_z = s + vol.XT * g.cy, s_yz = s_z + vol.XT;
+  const int32_t r_y = (e6 || e10) ? rank_in_segment(a, s_y, xl) : -1;
+  const int32_t r_z = (e9 || e10) ? rank_in_segment(a, s_z, xl) : -1;
+  int4 *qo = reinterpret_cast<int4 *>(vol.quads) + qb;
+  uint32_t qn = 0;
+  const int32_t self = (int32_t)vb;
+  if (e6) {
+    uint32_t sx, sxy;
+    const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
+    const int32_t rxy = next_x(s_y, r_y, xl, sxy);
+    const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
+    const int32_t m1 = vertex_of(sx, rx, 7);
+    const int32_t m2 = vertex_of(sxy, rxy, 4);
+    const int32_t m3 = vertex_of(s_y, r_y, 5);
+    const bool ccw = v[2] <= iso;
+    qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+  }
+  if (e9) {
+    uint32_t sx, sxz;
+    const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
+    const int32_t rxz = next_x(s_z, r_z, xl, sxz);
+    const int32_t m0 = self + (int32_t)((labels >> 36) & 0xFu);
+    const int32_t m1 = vertex_of(s_z, r_z, 1);
+    const int32_t m2 = vertex_of(sxz, rxz, 3);
+    const int32_t m3 = vertex_of(sx, rx, 11);
+    const bool ccw = v[5] <= iso;
+    qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+  }
+  if (e10) {
+    const int32_t r_yz = rank_in_segment(a, s_yz, xl);
+    const int32_t m0 = self + (int32_t)((labels >> 40) & 0xFu);
+    const int32_t m1 = vertex_of(s_y, r_y, 8);
+    const int32_t m2 = vertex_of(s_yz, r_yz, 0);
+    const int32_t m3 = vertex_of(s_z, r_z, 2);
+    const bool ccw = v[7] <= iso;
+    qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+  }
+  if (!ok) atomicOr(vol.status, kStatusLookup);
+}
+
+// Segment and rank of active cell ch*64 + lane: the chunk's first entry of
+// the non-empty list comes from the scan, and a window of 64 entries is
+// searched with shuffles (all lanes take part).  False for lanes past the
+// last cell and for inconsistent inputs (reported in the status word).
+__device__ __forceinline__ bool locate_cell(const EmitArgs &a, const Totals &tot, uint32_t num_ne,
+                                            uint32_t ch, uint32_t lane, uint32_t &s, uint32_t &rank) {
+  const uint32_t act = ch * 64u + lane;
+  const uint32_t u0 = a.chunk_ne[ch];
+  const uint2 wl = (u0 < num_ne && u0 + lane <= num_ne) ? a.ne[u0 + lane] : make_uint2(0u, 0xFFFFFFFFu);
+  uint32_t m = 0;
+#pragma unroll
+  for (uint32_t step = 32; step >= 1; step >>= 1) {
+    const uint32_t cand = m + step;
+    const uint32_t wc = (uint32_t)__shfl((int)wl.y, (int)(cand & 63u), kWave);
+    if (cand < 64u && wc <= act) m = cand;
+  }
+  const uint32_t ap = (uint32_t)__shfl((int)wl.y, (int)m, kWave);
+  s = (uint32_t)__shfl((int)wl.x, (int)m, kWave);
+  if (act >= tot.num_active) return false;
+  rank = act - ap;
+  if (u0 >= num_ne || u0 + m >= num_ne || s >= a.nsegs || act < ap ||
+      rank >= min(a.seg_counts[s].x & 0xFFFFu, a.cap)) {
+    atomicOr(a.v.status, kStatusInternal);
+    return false;
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   __syncthreads();
-
   const uint32_t lane = lane_id(), wave = wave_id();
-  const FieldDesc g = a.g;
-  const float iso = g.iso;
-  const size_t sy = g.X, sz = (size_t)g.X * g.Y;
-  const uint32_t cap = a.cap;
   const Totals tot = *a.totals;
   // Counts are clamped to buffer capacities so that no access can leave the
   // scratch buffers even if the count pass produced inconsistent data.
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
-
   for (uint32_t ch = blockIdx.x * (kEmitThreads / kWave) + wave; ch < num_chunks;
        ch += gridDim.x * (kEmitThreads / kWave)) {
-    {
-      const uint32_t act = ch * 64u + lane;
-      const uint32_t u0 = a.chunk_ne[ch];
-      const uint2 wl = (u0 < num_ne && u0 + lane <= num_ne) ? a.ne[u0 + lane] : make_uint2(0u, 0xFFFFFFFFu);
-      // segment of this lane's cell: the last window entry starting at or
-      // before `act` (all lanes take part in the shuffles)
-      uint32_t m = 0;
-#pragma unroll
-      for (uint32_t step = 32; step >= 1; step >>= 1) {
-        const uint32_t cand = m + step;
-        const uint32_t wc = (uint32_t)__shfl((int)wl.y, (int)(cand & 63u), kWave);
-        if (cand < 64u && wc <= act) m = cand;
-      }
-      const uint32_t ap = (uint32_t)__shfl((int)wl.y, (int)m, kWave);
-      const uint32_t s = (uint32_t)__shfl((int)wl.x, (int)m, kWave);
-      if (act >= tot.num_active) continue;
-      // Inputs come from the count pass; a broken invariant is reported in the
-      // status word instead of being followed out of bounds.
-      const uint32_t rank = act - ap;
-      if (u0 >= num_ne || u0 + m >= num_ne || s >= a.nsegs || act < ap ||
-          rank >= min(a.seg_counts[s].x & 0xFFFFu, cap)) {
-        atomicOr(a.status, kStatusInternal);
-        continue;
-      }
-      const uint64_t rec = a.slots[(size_t)s * cap + rank];
-      const uint32_t xl = rec_x(rec), cfg = rec_cfg(rec), lut2 = rec_lut2(rec);
-      const uint32_t xt = s % a.XT, row = s / a.XT;
-      const uint32_t j = row % g.cy, k = row / g.cy;
-      const uint32_t x = xt * cap + xl;
-      if (x >= g.cx || xl >= cap) { atomicOr(a.status, kStatusInternal); continue; }
-      const uint2 vq = a.seg_vq[s];
-      const uint32_t vb = vq.x + rec_lvb(rec);
-      const uint32_t qb = vq.y + rec_lqb(rec);
+    uint32_t s, rank;
+    if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
+    emit_cell(a, a.v, s, rank, s_lab);
+  }
+}
 
-      const float *p = g.f + (size_t)k * sz + (size_t)j * sy + x;
-      float v[8];
-      v[0] = p[0];
-      v[1] = p[1];
-      v[2] = p[sy + 1];
-      v[3] = p[sy];
-      v[4] = p[sz];
-      v[5] = p[sz + 1];
-      v[6] = p[sz + sy + 1];
-      v[7] = p[sz + sy];
 
-      const float X0 = coordf(x, g.cx, a.rgx, a.mnx), X1 = coordf(x + 1, g.cx, a.rgx, a.mnx);
-      const float Y0 = coordf(j, g.cy, a.rgy, a.mny), Y1 = coordf(j + 1, g.cy, a.rgy, a.mny);
-      const float Z0 = coordf(k, g.cz, a.rgz, a.mnz), Z1 = coordf(k + 1, g.cz, a.rgz, a.mnz);
+// ---------------------------------------------------------------------------
+// Batch count and emit.
 
-      // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
-      // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
-      // point is bitwise the one the owner writes (dmc.cu:1468-1514).
-      constexpr uint8_t EF[12] = {1, 1, 2, 0, 0, 1, 2, 3, 5, 5, 6, 4};
-      constexpr uint8_t ET[12] = {0, 2, 3, 3, 4, 5, 6, 7, 4, 6, 7, 7};
-      // Owner of each edge lies at a non-positive offset; an edge is skipped
-      // when its owner would be outside the grid (dmc.cu:1539-1545).
-      // bit 0: needs x>0, bit 1: needs y>0, bit 2: needs z>0
-      constexpr uint8_t NEED[12] = {6, 4, 4, 5, 3, 2, 0, 1, 2, 0, 0, 1};
-      const uint32_t have = (x > 0 ? 1u : 0u) | (j > 0 ? 2u : 0u) | (k > 0 ? 4u : 0u);
+struct BatchCountArgs {
+  const BatchVol *vols;
+  uint32_t nvol;
+  const uint2 *seg_counts;
+  const uint2 *seg_vq;
+  uint2 *base;                  // per volume: global vertex / quad number of its first
+  const uint32_t *batch_counts;  // the batch scan's counts (status bits in [3])
+};
 
-      const uint64_t labels = s_lab[lut2 * 256 + cfg];
-      float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
-      uint32_t cnt[4] = {0, 0, 0, 0};
-      float ex[3] = {0, 0, 0}, ey[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
-#pragma unroll
-      for (int e = 0; e < 12; ++e) {
-        const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
-        if (lab == 0xFu) continue;
-        const int f = EF[e], to = ET[e];
-        const float vf = v[f], vt = v[to];
-        const float tt = (iso - vf) / (vt - vf);
-        const float omt = 1.0f - tt;
-        const float fx = (f == 1 || f == 2 || f == 5 || f == 6) ? X1 : X0;
-        const float fy = (f == 2 || f == 3 || f == 6 || f == 7) ? Y1 : Y0;
-        const float fz = (f >= 4) ? Z1 : Z0;
-        const float tx = (to == 1 || to == 2 || to == 5 || to == 6) ? X1 : X0;
-        const float ty = (to == 2 || to == 3 || to == 6 || to == 7) ? Y1 : Y0;
-        const float tz = (to >= 4) ? Z1 : Z0;
-        const float px = fx * omt + tx * tt;
-        const float py = fy * omt + ty * tt;
-        const float pz = fz * omt + tz * tt;
-        if (e == 6) { ex[0] = px; ey[0] = py; ez[0] = pz; }
-        if (e == 9) { ex[1] = px; ey[1] = py; ez[1] = pz; }
-        if (e == 10) { ex[2] = px; ey[2] = py; ez[2] = pz; }
-        if ((NEED[e] & have) != NEED[e]) continue;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (lab == (uint32_t)q) {
-            if (cnt[q] == 0) { ax[q] = px; ay[q] = py; az[q] = pz; }
-            else { ax[q] = ax[q] + px; ay[q] = ay[q] + py; az[q] = az[q] + pz; }
-            ++cnt[q];
-          }
-        }
+// One workgroup per volume: its totals and the bases of its numbering.
+__global__ __launch_bounds__(256) void k_batch_counts(BatchCountArgs b) {
+  __shared__ uint32_t s_part[256 / kWave];
+  for (uint32_t v = blockIdx.x; v < b.nvol; v += gridDim.x) {
+    const BatchVol &bv = b.vols[v];
+    uint32_t na = 0;
+    for (uint32_t i = threadIdx.x; i < bv.nsegs; i += 256) na += b.seg_counts[bv.seg_off + i].x & 0xFFFFu;
+    na = wave_incl_scan_dpp(na);
+    __syncthreads();
+    if (lane_id() == kWave - 1) s_part[wave_id()] = na;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t A = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+      const uint2 v0 = b.seg_vq[bv.seg_off], v1 = b.seg_vq[bv.seg_off + bv.nsegs];
+      b.base[v] = v0;
+      if (bv.counts) {
+        bv.counts[0] = A;
+        bv.counts[1] = v1.x - v0.x;
+        bv.counts[2] = v1.y - v0.y;
+        bv.counts[3] = b.batch_counts[3];
       }
-      const uint16_t info = c_cfg_info[cfg];
-      const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
-      const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
-      const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
-      const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
-      const uint32_t nv = niso + b6 + b9 + b10;
-      if ((uint64_t)vb + nv <= a.vcap) {
-        float *out = a.verts + 3ull * vb;
-        const float qnan = __int_as_float(0x7FC00000);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if ((uint32_t)q < niso) {
-            const float d = (float)cnt[q];
-            out[3 * q + 0] = cnt[q] ? ax[q] / d : qnan;
-            out[3 * q + 1] = cnt[q] ? ay[q] / d : qnan;
-            out[3 * q + 2] = cnt[q] ? az[q] / d : qnan;
-          }
-        }
-        uint32_t wv = niso;
-        if (b6) { out[3 * wv] = ex[0]; out[3 * wv + 1] = ey[0]; out[3 * wv + 2] = ez[0]; ++wv; }
-        if (b9) { out[3 * wv] = ex[1]; out[3 * wv + 1] = ey[1]; out[3 * wv + 2] = ez[1]; ++wv; }
-        if (b10) { out[3 * wv] = ex[2]; out[3 * wv + 1] = ey[2]; out[3 * wv + 2] = ez[2]; ++wv; }
-      } else {
-        atomicOr(a.status, kStatusVertexOverflow);
-      }
-
-      // Quads (dmc.cu:1983-2047): rings in ccw order, reversed (0,3,2,1) when
-      // the owned edge's first corner is not above iso.
-      const bool xin = x + 1 < g.cx, yin = j + 1 < g.cy, zin = k + 1 < g.cz;
-      const bool e6 = b6 && xin && yin, e9 = b9 && xin && zin, e10 = b10 && yin && zin;
-      if (!(e6 || e9 || e10)) continue;
-      const uint32_t nq = (uint32_t)e6 + (uint32_t)e9 + (uint32_t)e10;
-      if ((uint64_t)qb + nq > a.qcap) { atomicOr(a.status, kStatusQuadOverflow); continue; }
-      bool ok = true;
-      // Cell (x, row of segment s2) has rank r2 in s2; the cell one step in +x
-      // is rank r2 + 1 of s2 or, past the x-tile, rank 0 of the next x-tile's
-      // segment.  Returns its vertex for local edge e.
-      auto vertex_of = [&](uint32_t s2, int32_t r2, int e) -> int32_t {
-        if (r2 < 0 || (uint32_t)r2 >= cap) { ok = false; return -1; }
-        const uint32_t m2 = (uint32_t)a.slots[(size_t)s2 * cap + (uint32_t)r2];
-        const uint32_t lab = (uint32_t)(s_lab[rec_lut2(m2) * 256 + rec_cfg(m2)] >> (4 * e)) & 0xFu;
-        if (lab == 0xFu) { ok = false; return -1; }
-        return (int32_t)(a.seg_vq[s2].x + rec_lvb(m2) + lab);
-      };
-      // the +x neighbour of (segment s2, rank r2, x offset xo)
-      auto next_x = [&](uint32_t s2, int32_t r2, uint32_t xo, uint32_t &s_out) -> int32_t {
-        if (r2 < 0) return -1;
-        if (xo + 1u < cap) {
-          s_out = s2;
-          const int32_t r = r2 + 1;
-          if ((uint32_t)r >= cap || rec_x(a.slots[(size_t)s2 * cap + (uint32_t)r]) != xo + 1u) return -1;
-          return r;
-        }
-        s_out = s2 + 1u;  // same row, next x-tile (in the grid since x + 1 < cx)
-        if (rec_x(a.slots[(size_t)s_out * cap]) != 0u || (a.seg_counts[s_out].x & 0xFFFFu) == 0u) return -1;
-        return 0;
-      };
-      const uint32_t s_y = s + a.XT, s_z = s + a.XT * g.cy, s_yz = s_z + a.XT;
-      const int32_t r_y = (e6 || e10) ? rank_in_segment(a, s_y, xl) : -1;
-      const int32_t r_z = (e9 || e10) ? rank_in_segment(a, s_z, xl) : -1;
-      int4 *qo = reinterpret_cast<int4 *>(a.quads) + qb;
-      uint32_t qn = 0;
-      const int32_t self = (int32_t)vb;
-      if (e6) {
-        uint32_t sx, sxy;
-        const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
-        const int32_t rxy = next_x(s_y, r_y, xl, sxy);
-        const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
-        const int32_t m1 = vertex_of(sx, rx, 7);
-        const int32_t m2 = vertex_of(sxy, rxy, 4);
-        const int32_t m3 = vertex_of(s_y, r_y, 5);
-        const bool ccw = v[2] <= iso;
-        qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
-      }
-      if (e9) {
-        uint32_t sx, sxz;
-        const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
-        const int32_t rxz = next_x(s_z, r_z, xl, sxz);
-        const int32_t m0 = self + (int32_t)((labels >> 36) & 0xFu);
-        const int32_t m1 = vertex_of(s_z, r_z, 1);
-        const int32_t m2 = vertex_of(sxz, rxz, 3);
-        const int32_t m3 = vertex_of(sx, rx, 11);
-        const bool ccw = v[5] <= iso;
-        qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
-      }
-      if (e10) {
-        const int32_t r_yz = rank_in_segment(a, s_yz, xl);
-        const int32_t m0 = self + (int32_t)((labels >> 40) & 0xFu);
-        const int32_t m1 = vertex_of(s_y, r_y, 8);
-        const int32_t m2 = vertex_of(s_yz, r_yz, 0);
-        const int32_t m3 = vertex_of(s_z, r_z, 2);
-        const bool ccw = v[7] <= iso;
-        qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
-      }
-      if (!ok) atomicOr(a.status, kStatusLookup);
     }
+  }
+}
+
+struct BatchEmitArgs {
+  EmitArgs a;  // arrays of the whole batch; a.v.status is the batch status word
+  const BatchVol *vols;
+  const uint2 *base;
+  uint32_t nvol;
+};
+
+// Emit over the active cells of all volumes; a lane finds its volume by
+// binary search over the volumes' first segments.
+__global__ __launch_bounds__(kEmitThreads) void k_emit_batch(BatchEmitArgs b) {
+  __shared__ uint64_t s_lab[2 * 256];
+  __shared__ uint32_t s_soff[kMaxBatch + 1];
+  for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
+  for (uint32_t i = threadIdx.x; i <= b.nvol; i += kEmitThreads)
+    s_soff[i] = i < b.nvol ? b.vols[i].seg_off : b.a.nsegs;
+  __syncthreads();
+  const EmitArgs &a = b.a;
+  const uint32_t lane = lane_id(), wave = wave_id();
+  const Totals tot = *a.totals;
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
+  for (uint32_t ch = blockIdx.x * (kEmitThreads / kWave) + wave; ch < num_chunks;
+       ch += gridDim.x * (kEmitThreads / kWave)) {
+    uint32_t s, rank;
+    if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
+    // the last volume whose first segment is <= s (empty volumes share it)
+    uint32_t lo = 0, hi = b.nvol;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_soff[mid] <= s) lo = mid; else hi = mid;
+    }
+    const BatchVol &bv = b.vols[lo];
+    EmitVol e;
+    e.g = bv.c.g;
+    e.mnx = bv.mnx; e.mny = bv.mny; e.mnz = bv.mnz;
+    e.rgx = bv.rgx; e.rgy = bv.rgy; e.rgz = bv.rgz;
+    e.XT = bv.c.XT;
+    e.seg_off = bv.seg_off;
+    const uint2 bs = b.base[lo];
+    e.vbase = bs.x;
+    e.qbase = bs.y;
+    e.verts = bv.verts;
+    e.vcap = bv.verts ? bv.vcap : 0;
+    e.quads = bv.quads;
+    e.qcap = bv.quads ? bv.qcap : 0;
+    e.status = bv.counts ? bv.counts + 3 : a.v.status;
+    emit_cell(a, e, s, rank, s_lab);
   }
 }
 

@@ -58,6 +58,18 @@ _F3 = ctypes.c_float * 3
 _VP = ctypes.c_void_p
 
 
+class Volume(ctypes.Structure):
+    """mdmc_volume: one volume of a batch (device pointers)."""
+    _fields_ = [("field", _VP), ("dims", ctypes.c_uint32 * 3), ("iso", ctypes.c_float),
+                ("xyz_min", ctypes.c_float * 3), ("xyz_max", ctypes.c_float * 3),
+                ("vertices", _VP), ("vertex_capacity", ctypes.c_uint64),
+                ("quads", _VP), ("quad_capacity", ctypes.c_uint64)]
+
+
+MAX_BATCH = 1024
+OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS_PER_CU = 0, 1, 2
+
+
 def declared_functions() -> list[str]:
     """Names of the functions include/mdmc.h declares."""
     text = open(HEADER_PATH).read()
@@ -99,6 +111,12 @@ def lib():
     L.mdmc_field_blobs_device.restype = ctypes.c_int
     L.mdmc_field_gyroid_device.argtypes = [_VP, _VP, _U3, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, _VP]
     L.mdmc_field_gyroid_device.restype = ctypes.c_int
+    for name in ("mdmc_batch_count_device", "mdmc_batch_emit_device", "mdmc_batch_extract_device"):
+        f = getattr(L, name)
+        f.argtypes = [_VP, ctypes.POINTER(Volume), ctypes.c_uint32, _VP, _VP]
+        f.restype = ctypes.c_int
+    L.mdmc_ctx_set_option.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
+    L.mdmc_ctx_set_option.restype = ctypes.c_int
     L.mdmc_ctx_profile.argtypes = [_VP, ctypes.c_int]
     L.mdmc_ctx_profile.restype = ctypes.c_int
     L.mdmc_ctx_profile_read.argtypes = [_VP, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32)]
@@ -303,3 +321,84 @@ class Context:
         _check(lib().mdmc_field_gyroid_device(self._h, _VP(out.data_ptr()), _U3(X, Y, Z), float(period), int(seed),
                                               float(noise_amp), self._stream(stream)), "mdmc_field_gyroid_device")
         return out
+
+
+def _volume_table(fields, isos, boxes=None, outputs=None):
+    """ctypes array of Volume for torch fields shaped (Z, Y, X)."""
+    n = len(fields)
+    arr = (Volume * n)()
+    for i, f in enumerate(fields):
+        if f.dtype.is_floating_point is False or f.dim() != 3 or not f.is_contiguous():
+            raise ValueError("batch fields must be contiguous 3-D float32 CUDA tensors")
+        Z, Y, X = f.shape
+        v = arr[i]
+        v.field = f.data_ptr()
+        v.dims[0], v.dims[1], v.dims[2] = X, Y, Z
+        v.iso = float(isos[i])
+        mn, mx = boxes[i] if boxes is not None else _default_box((X, Y, Z))
+        for k in range(3):
+            v.xyz_min[k] = float(mn[k])
+            v.xyz_max[k] = float(mx[k])
+        if outputs is not None and outputs[i] is not None:
+            verts, quads = outputs[i]
+            v.vertices = verts.data_ptr() if verts is not None and verts.numel() else None
+            v.vertex_capacity = verts.numel() // 3 if verts is not None else 0
+            v.quads = quads.data_ptr() if quads is not None and quads.numel() else None
+            v.quad_capacity = quads.numel() // 4 if quads is not None else 0
+    return arr
+
+
+def _batch_methods():
+    def set_option(self, option: int, value: int) -> None:
+        _check(lib().mdmc_ctx_set_option(self._h, int(option), int(value)), "mdmc_ctx_set_option")
+
+    def batch_counts_tensor(self, n: int):
+        return self._torch.zeros((n, 4), dtype=self._torch.int32, device="cuda")
+
+    def batch_count(self, fields, isos, counts, boxes=None, stream=None) -> None:
+        """Classify + scan many volumes in one launch sequence (counts: (n, 4) int32 CUDA tensor)."""
+        arr = _volume_table(fields, isos, boxes)
+        _check(lib().mdmc_batch_count_device(self._h, arr, len(fields), _VP(counts.data_ptr()),
+                                             self._stream(stream)), "mdmc_batch_count_device")
+
+    def batch_emit(self, fields, isos, outputs, counts, boxes=None, stream=None) -> None:
+        arr = _volume_table(fields, isos, boxes, outputs)
+        _check(lib().mdmc_batch_emit_device(self._h, arr, len(fields), _VP(counts.data_ptr()),
+                                            self._stream(stream)), "mdmc_batch_emit_device")
+
+    def batch_extract_into(self, fields, isos, outputs, counts, boxes=None, stream=None) -> None:
+        """Count + emit into preallocated outputs [(verts, quads), ...] without a host sync."""
+        arr = _volume_table(fields, isos, boxes, outputs)
+        _check(lib().mdmc_batch_extract_device(self._h, arr, len(fields), _VP(counts.data_ptr()),
+                                               self._stream(stream)), "mdmc_batch_extract_device")
+
+    def batch_extract(self, fields, isos, boxes=None, stream=None):
+        """Count, size every output exactly, emit.  Returns [(verts, quads, counts dict), ...]."""
+        torch = self._torch
+        n = len(fields)
+        counts = self.batch_counts_tensor(n)
+        self.batch_count(fields, isos, counts, boxes, stream)
+        (stream or torch.cuda.current_stream()).synchronize()
+        c = counts.cpu().numpy().view(np.uint32)
+        outs = []
+        for i in range(n):
+            V, Q = int(c[i, 1]), int(c[i, 2])
+            outs.append((torch.empty((V, 3), dtype=torch.float32, device="cuda"),
+                         torch.empty((Q, 4), dtype=torch.int32, device="cuda")))
+        self.batch_emit(fields, isos, outs, counts, boxes, stream)
+        (stream or torch.cuda.current_stream()).synchronize()
+        c = counts.cpu().numpy().view(np.uint32)
+        res = []
+        for i in range(n):
+            st = int(c[i, 3])
+            if st:
+                raise MDMCError(-7, f"volume {i}: status {st}")
+            res.append((outs[i][0], outs[i][1], {"num_active": int(c[i, 0]), "num_vertices": int(c[i, 1]),
+                                                 "num_quads": int(c[i, 2]), "status": st}))
+        return res
+
+    for f in (set_option, batch_counts_tensor, batch_count, batch_emit, batch_extract_into, batch_extract):
+        setattr(Context, f.__name__, f)
+
+
+_batch_methods()

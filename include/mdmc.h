@@ -134,6 +134,48 @@ int mdmc_ctx_profile(mdmc_ctx *ctx, int enable);
  * launch count per kernel id, and clears the record. */
 int mdmc_ctx_profile_read(mdmc_ctx *ctx, double ms_total[4], uint32_t launches[4]);
 
+/* ---- batches (no counterpart in the reference, which extracts one grid per
+ * call; for callers with many volumes: one launch sequence for all of them,
+ * capturable in a HIP graph after a first call has sized the buffers) ---- */
+
+#define MDMC_MAX_BATCH 1024
+
+/* One volume of a batch.  `field` is device memory; the outputs may be NULL
+ * for mdmc_batch_count_device.  Vertex and quad numbers are local to the
+ * volume (each volume's mesh is exactly what mdmc_extract_device gives). */
+typedef struct mdmc_volume {
+  const float *field;
+  uint32_t dims[3];
+  float iso;
+  float xyz_min[3];
+  float xyz_max[3];
+  float *vertices;          /* 3 * vertex_capacity floats */
+  uint64_t vertex_capacity;
+  int32_t *quads;           /* 4 * quad_capacity int32 */
+  uint64_t quad_capacity;
+} mdmc_volume;
+
+/* Classify + scan all volumes; d_counts[v] (device, n entries) receives
+ * volume v's counts.  n <= MDMC_MAX_BATCH. */
+int mdmc_batch_count_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                            void *stream);
+/* Emit the batch last counted on ctx (same fields, dims and isovalues, in the
+ * same order) into the outputs named in vols. */
+int mdmc_batch_emit_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                           void *stream);
+/* Count + emit with the capacities given in vols (overflow is flagged per
+ * volume in d_counts[v].status, never written). */
+int mdmc_batch_extract_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                              void *stream);
+
+/* Tuning options of a context (0 restores the default). */
+enum {
+  MDMC_OPT_BATCH_PLANES = 0,          /* cell planes per batch classify tile */
+  MDMC_OPT_CLASSIFY_BLOCKS_PER_CU = 1, /* persistent batch classify grid */
+  MDMC_OPT_EMIT_BLOCKS_PER_CU = 2      /* emit grid */
+};
+int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value);
+
 /* ---- synthetic fields on the device ------------------------------------- */
 
 /* Sum of `nblobs` Gaussian blobs (centres uniform in the grid, sigma uniform

@@ -1,0 +1,183 @@
+"""Batched extraction (many volumes in one launch sequence) against the
+single-volume path and the oracle (GPU).
+
+Each volume of a batch must give exactly the mesh the single-volume entry
+gives for it alone: same counts, quads and vertex bits.  The single-volume
+path is itself checked against the oracle, and the small volumes here are
+also compared with the oracle directly.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import assert_same_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    torch.cuda.set_device(0)
+    return torch
+
+
+@pytest.fixture(scope="module")
+def fa_mod(torch_cuda):
+    import framework_amd as fa
+
+    fa.setup()
+    return fa
+
+
+def _mixed_batch(fa, torch, n, seed, max_dim=96):
+    """Volumes of uneven dims (log-uniform per axis), alternating blob sums and
+    gyroid+noise, isovalues in [-0.2, 0.2]."""
+    rng = np.random.default_rng(seed)
+    gen = fa.Context()
+    fields, isos = [], []
+    for i in range(n):
+        dims = tuple(int(round(np.exp(rng.uniform(np.log(8), np.log(max_dim))))) for _ in range(3))
+        if i % 2 == 0:
+            f = gen.field_blobs(dims, seed=seed * 1000 + i, nblobs=int(rng.integers(8, 33)), sigma=(2.0, 10.0))
+        else:
+            f = gen.field_gyroid(dims, period=float(rng.uniform(6, 14)), seed=seed * 1000 + i, noise_amp=0.3)
+        fields.append(f)
+        isos.append(float(rng.uniform(-0.2, 0.2)))
+    torch.cuda.synchronize()
+    gen.close()
+    return fields, isos
+
+
+def _single(ctx, f, iso):
+    v, q, c = ctx.extract(f, iso)
+    return v.cpu().numpy(), q.cpu().numpy(), c
+
+
+def test_batch_matches_single_volume(fa_mod, torch_cuda):
+    fields, isos = _mixed_batch(fa_mod, torch_cuda, 24, seed=3)
+    bctx, sctx = fa_mod.Context(), fa_mod.Context()
+    res = bctx.batch_extract(fields, isos)
+    for i, (f, iso) in enumerate(zip(fields, isos)):
+        v, q, c = _single(sctx, f, iso)
+        bv, bq, bc = res[i]
+        assert (bc["num_active"], bc["num_vertices"], bc["num_quads"]) == \
+            (c["num_active"], c["num_vertices"], c["num_quads"]), i
+        assert np.array_equal(bq.cpu().numpy(), q), i
+        assert np.array_equal(bv.cpu().numpy().view(np.uint32), v.view(np.uint32)), i
+    bctx.close()
+    sctx.close()
+
+
+def test_batch_against_oracle(fa_mod, torch_cuda):
+    fields, isos = _mixed_batch(fa_mod, torch_cuda, 6, seed=11, max_dim=48)
+    ctx = fa_mod.Context()
+    res = ctx.batch_extract(fields, isos)
+    for i, (f, iso) in enumerate(zip(fields, isos)):
+        ref = oracle.run(f.cpu().numpy(), iso)
+        v, q, c = res[i]
+        assert c["num_vertices"] == ref["num_vertices"] and c["num_quads"] == ref["num_quads"]
+        assert_same_mesh(v.cpu().numpy(), q.cpu().numpy(), ref)
+    ctx.close()
+
+
+def test_batch_with_empty_and_flat_volumes(fa_mod, torch_cuda):
+    """Degenerate dims and volumes without a surface inside a batch."""
+    torch = torch_cuda
+    ctx = fa_mod.Context()
+    a = ctx.field_blobs((40, 33, 27), seed=1, nblobs=10, sigma=(3.0, 8.0))
+    flat = torch.full((20, 21, 22), 5.0, device="cuda")
+    tiny = torch.zeros((1, 7, 7), device="cuda")
+    b = ctx.field_gyroid((37, 41, 29), period=9.0, seed=2, noise_amp=0.2)
+    torch.cuda.synchronize()
+    fields, isos = [tiny, a, flat, tiny, b, tiny], [0.0, 0.1, 0.0, 0.0, 0.0, 0.0]
+    res = ctx.batch_extract(fields, isos)
+    sctx = fa_mod.Context()
+    for i, (f, iso) in enumerate(zip(fields, isos)):
+        v, q, c = _single(sctx, f, iso)
+        assert res[i][2]["num_vertices"] == c["num_vertices"] and res[i][2]["num_quads"] == c["num_quads"]
+        assert np.array_equal(res[i][1].cpu().numpy(), q)
+        assert np.array_equal(res[i][0].cpu().numpy().view(np.uint32), v.view(np.uint32))
+    assert res[2][2]["num_vertices"] == 0 and res[0][2]["num_vertices"] == 0
+    # all volumes empty
+    res = ctx.batch_extract([tiny, flat], [0.0, 0.0])
+    assert all(r[2]["num_vertices"] == 0 for r in res)
+    ctx.close()
+    sctx.close()
+
+
+def test_batch_capacity_overflow_is_flagged_per_volume(fa_mod, torch_cuda):
+    torch = torch_cuda
+    fields, isos = _mixed_batch(fa_mod, torch, 4, seed=5, max_dim=40)
+    ctx = fa_mod.Context()
+    exact = ctx.batch_extract(fields, isos)
+    outs = []
+    for i, (v, q, c) in enumerate(exact):
+        nv = c["num_vertices"] - (3 if i == 1 else 0)
+        outs.append((torch.empty((max(nv, 0), 3), device="cuda"), torch.empty((c["num_quads"], 4), dtype=torch.int32,
+                                                                              device="cuda")))
+    counts = ctx.batch_counts_tensor(len(fields))
+    ctx.batch_extract_into(fields, isos, outs, counts)
+    torch.cuda.synchronize()
+    st = counts.cpu().numpy().view(np.uint32)[:, 3]
+    assert st[1] & 1, st
+    assert st[0] == 0 and st[2] == 0 and st[3] == 0, st
+    ctx.close()
+
+
+def test_batch_graph_replay(fa_mod, torch_cuda):
+    """The batch sequence captured in a HIP graph, replayed several times with
+    eager work in between; every replay equals the eager result."""
+    torch = torch_cuda
+    fields, isos = _mixed_batch(fa_mod, torch, 8, seed=7, max_dim=64)
+    ctx = fa_mod.Context()
+    exact = ctx.batch_extract(fields, isos)
+    outs = [(torch.empty_like(v), torch.empty_like(q)) for v, q, _ in exact]
+    counts = ctx.batch_counts_tensor(len(fields))
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx.batch_extract_into(fields, isos, outs, counts, stream=stream)  # sizes the buffers
+    stream.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        ctx.batch_extract_into(fields, isos, outs, counts, stream=torch.cuda.current_stream())
+    for rep in range(3):
+        for v, q in outs:
+            v.fill_(-1.0)
+            q.fill_(-1)
+        counts.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        c = counts.cpu().numpy().view(np.uint32)
+        for i, (v, q, cnt) in enumerate(exact):
+            assert c[i, 1] == cnt["num_vertices"] and c[i, 2] == cnt["num_quads"] and c[i, 3] == 0, (rep, i)
+            assert torch.equal(outs[i][1], q), (rep, i)
+            assert torch.equal(outs[i][0].view(torch.int32), v.view(torch.int32)), (rep, i)
+        _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+    ctx.close()
+
+
+def test_zmarch_and_plane_tile_kernels_agree(fa_mod, torch_cuda):
+    """The two classify kernels produce identical meshes (row lengths with and
+    without 16-byte row loads)."""
+    torch = torch_cuda
+    ctx = fa_mod.Context()
+    for dims in [(131, 40, 30), (132, 40, 30), (300, 20, 17), (700, 9, 9)]:
+        f = ctx.field_gyroid(dims, period=9.0, seed=4, noise_amp=0.5)
+        torch.cuda.synchronize()
+        v0, q0, _ = ctx.extract(f, 0.02)
+        os.environ["MDMC_CLASSIFY"] = "zmarch"
+        try:
+            v1, q1, _ = ctx.extract(f, 0.02)
+        finally:
+            del os.environ["MDMC_CLASSIFY"]
+        assert torch.equal(q0, q1), dims
+        assert torch.equal(v0.view(torch.int32), v1.view(torch.int32)), dims
+    ctx.close()
