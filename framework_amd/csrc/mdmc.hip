@@ -145,6 +145,7 @@ struct mdmc_ctx {
   int num_cus = 256;
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  bool scratch_external = false;  // memory given by mdmc_ctx_set_scratch
   mdmc::Scratch s{};
   mdmc::Plan plan{};
   bool counted = false;
@@ -165,6 +166,14 @@ struct mdmc_ctx {
   uint32_t batch_n = 0, batch_W4 = 0, batch_KP = 0, batch_tiles = 0;
   uint64_t batch_key = 0;
   mdmc::Plan batch_plan{};
+  // batched field generation: pinned staging + device copy of GenVol table and
+  // the per-axis tables
+  void *gdev = nullptr;
+  size_t gdev_bytes = 0;
+  void *ghost = nullptr;
+  size_t ghost_bytes = 0;
+  hipEvent_t ghost_ev = nullptr;
+  bool ghost_ev_used = false;
   // tuning options (0 = default)
   int opt[4] = {0, 0, 0, 0};
   // per-kernel event timing
@@ -202,24 +211,40 @@ struct ProfScope {
 
 static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
+struct ScratchLayout {
+  size_t counts, vq, ne, chunk, state, ticket, totals, slots, gmap, total;
+};
+
+ScratchLayout scratch_layout(const Plan &p) {
+  ScratchLayout L{};
   size_t off = 0;
-  Scratch s{};
   auto take = [&](size_t bytes) {
     size_t o = off;
     off += align256(bytes);
     return o;
   };
   const size_t seg_padded = (size_t)p.ntiles * kScanTile;  // the scan reads whole tiles
-  const size_t o_counts = take(seg_padded * sizeof(uint2));
-  const size_t o_vq = take(seg_padded * sizeof(uint2));
-  const size_t o_ne = take((p.nsegs + 1) * sizeof(uint2));
-  const size_t o_chunk = take(p.max_chunks * sizeof(uint32_t));
-  const size_t o_state = take((size_t)p.ntiles * 4 * sizeof(unsigned long long));
-  const size_t o_ticket = take(sizeof(uint32_t));
-  const size_t o_totals = take(sizeof(Totals));
-  const size_t o_slots = take(p.nslots * sizeof(uint64_t));
-  const size_t o_gmap = take(p.nsegs * (p.cap / 64) * sizeof(uint4));
+  L.counts = take(seg_padded * sizeof(uint2));
+  L.vq = take(seg_padded * sizeof(uint2));
+  L.ne = take((p.nsegs + 1) * sizeof(uint2));
+  L.chunk = take(p.max_chunks * sizeof(uint32_t));
+  L.state = take((size_t)p.ntiles * 4 * sizeof(unsigned long long));
+  L.ticket = take(sizeof(uint32_t));
+  L.totals = take(sizeof(Totals));
+  L.slots = take(p.nslots * sizeof(uint64_t));
+  L.gmap = take(p.nsegs * (p.cap / 64) * sizeof(uint4));
+  L.total = off;
+  return L;
+}
+
+int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
+  Scratch s{};
+  const ScratchLayout lay = scratch_layout(p);
+  const size_t off = lay.total;
+  const size_t o_counts = lay.counts, o_vq = lay.vq, o_ne = lay.ne, o_chunk = lay.chunk, o_state = lay.state,
+               o_ticket = lay.ticket, o_totals = lay.totals, o_slots = lay.slots, o_gmap = lay.gmap;
+  if (off > ctx->scratch_bytes && ctx->scratch_external)
+    return fail(MDMC_ERR_CAPACITY, "the scratch memory given to the context is too small");
   if (off > ctx->scratch_bytes) {
     if (ctx->scratch) {
       hipError_t e = hipFree(ctx->scratch);
@@ -520,30 +545,52 @@ int fill_batch_table(mdmc_ctx *ctx, int slot, const mdmc_volume *vols, uint32_t 
   return MDMC_OK;
 }
 
-int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts, hipStream_t st) {
+// x-tile width and the aggregate plan (what the scratch buffers and the scan
+// see) of a batch.
+int batch_layout(const mdmc_volume *vols, uint32_t n, bool check_fields, Plan *agg, uint32_t *W4_out,
+                 uint32_t *KP_out) {
   uint32_t W4 = 1;
-  uint64_t nseg_est = 0, tiles_est = 0;
   for (uint32_t v = 0; v < n; ++v) {
     const mdmc_volume &mv = vols[v];
-    const bool empty = mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2;
-    if (empty) continue;
-    if (!mv.field) return fail(MDMC_ERR_INVALID, "null field in batch");
+    if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
+    if (check_fields && !mv.field) return fail(MDMC_ERR_INVALID, "null field in batch");
     if ((uint64_t)mv.dims[0] * mv.dims[1] * mv.dims[2] > 0xFFFFFFFFull)
       return fail(MDMC_ERR_TOO_LARGE, "grid has more than 2^32-1 points");
     const uint32_t cx = mv.dims[0] - 1;
     while (W4 < 4 && 256u * W4 < cx) W4 <<= 1;
   }
-  const uint32_t KP = batch_planes(ctx, W4);
+  const uint32_t KP = pt_planes(W4);
   const uint32_t cap = 256u * W4;
+  uint64_t nseg = 0, tiles = 0, cells = 0;
   for (uint32_t v = 0; v < n; ++v) {
     const mdmc_volume &mv = vols[v];
     if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
     const uint64_t XT = (mv.dims[0] - 1 + cap - 1) / cap;
-    nseg_est += XT * (mv.dims[1] - 1) * (uint64_t)(mv.dims[2] - 1);
-    tiles_est += XT * ((mv.dims[1] - 1 + kBatchRows - 1) / kBatchRows) * ((mv.dims[2] - 1 + KP - 1) / KP);
+    nseg += XT * (mv.dims[1] - 1) * (uint64_t)(mv.dims[2] - 1);
+    tiles += XT * ((mv.dims[1] - 1 + kBatchRows - 1) / kBatchRows) * ((mv.dims[2] - 1 + KP - 1) / KP);
+    cells += (uint64_t)(mv.dims[0] - 1) * (mv.dims[1] - 1) * (mv.dims[2] - 1);
   }
-  if (nseg_est >= 0x7FFFFFFFull || tiles_est >= 0x7FFFFFFFull)
-    return fail(MDMC_ERR_TOO_LARGE, "batch has too many cell rows");
+  if (nseg >= 0x7FFFFFFFull || tiles >= 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "batch has too many cell rows");
+  *agg = Plan{};
+  agg->cap = cap;
+  agg->nsegs = nseg;
+  agg->nslots = nseg * cap;
+  agg->ntiles = (uint32_t)(nseg / kScanTile + 1);
+  agg->max_chunks = (cells + 63) / 64 + 1;
+  agg->classify_blocks = tiles;
+  agg->empty = nseg == 0;
+  *W4_out = W4;
+  *KP_out = KP;
+  return MDMC_OK;
+}
+
+int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts, hipStream_t st) {
+  Plan agg{};
+  uint32_t W4 = 1, KP = 1;
+  int rc0 = batch_layout(vols, n, true, &agg, &W4, &KP);
+  if (rc0 != MDMC_OK) return rc0;
+  KP = batch_planes(ctx, W4);
+  const uint64_t nseg_est = agg.nsegs;
   int rc = ensure_batch_buffers(ctx, n);
   if (rc != MDMC_OK) return rc;
   ctx->batch_counted = false;
@@ -558,22 +605,8 @@ int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts 
     ctx->batch_counted = true;
     return MDMC_OK;
   }
-  // aggregate plan: what the scratch buffers and the scan see
-  Plan agg{};
-  agg.cap = cap;
-  agg.nsegs = nseg_est;
-  agg.nslots = nseg_est * cap;
-  agg.ntiles = (uint32_t)(nseg_est / kScanTile + 1);
   rc = staging_acquire(ctx, 0, st);
   if (rc != MDMC_OK) return rc;
-  // cells bound the chunk index; computed from the table below
-  uint64_t cells = 0;
-  for (uint32_t v = 0; v < n; ++v) {
-    const mdmc_volume &mv = vols[v];
-    if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
-    cells += (uint64_t)(mv.dims[0] - 1) * (mv.dims[1] - 1) * (mv.dims[2] - 1);
-  }
-  agg.max_chunks = (cells + 63) / 64 + 1;
   rc = ensure_scratch(ctx, agg);
   if (rc != MDMC_OK) return rc;
   const Scratch &S = ctx->s;
@@ -794,9 +827,12 @@ int mdmc_ctx_profile_read(mdmc_ctx *ctx, double ms_total[4], uint32_t launches[4
 void mdmc_ctx_destroy(mdmc_ctx *ctx) {
   if (!ctx) return;
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
-  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->scratch && !ctx->scratch_external) (void)hipFree(ctx->scratch);
   if (ctx->gen) (void)hipFree(ctx->gen);
   if (ctx->bdev) (void)hipFree(ctx->bdev);
+  if (ctx->gdev) (void)hipFree(ctx->gdev);
+  if (ctx->ghost) (void)hipHostFree(ctx->ghost);
+  if (ctx->ghost_ev) (void)hipEventDestroy(ctx->ghost_ev);
   for (int i = 0; i < 2; ++i) {
     if (ctx->bhost[i]) (void)hipHostFree(ctx->bhost[i]);
     if (ctx->bhost_ev[i]) (void)hipEventDestroy(ctx->bhost_ev[i]);
@@ -855,6 +891,28 @@ static int check_batch_args(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, 
   if (!ctx || !vols || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (n == 0 || n > kMaxBatch) return fail(MDMC_ERR_INVALID, "batch size must be 1..1024");
   if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  return MDMC_OK;
+}
+
+int mdmc_batch_scratch_bytes(const mdmc_volume *vols, uint32_t n, uint64_t *bytes) {
+  if (!vols || !bytes) return fail(MDMC_ERR_INVALID, "null argument");
+  if (n == 0 || n > kMaxBatch) return fail(MDMC_ERR_INVALID, "batch size must be 1..1024");
+  Plan agg{};
+  uint32_t W4 = 1, KP = 1;
+  int rc = batch_layout(vols, n, false, &agg, &W4, &KP);
+  if (rc != MDMC_OK) return rc;
+  *bytes = agg.empty ? 0 : scratch_layout(agg).total;
+  return MDMC_OK;
+}
+
+int mdmc_ctx_set_scratch(mdmc_ctx *ctx, void *d_scratch, uint64_t bytes) {
+  if (!ctx) return fail(MDMC_ERR_INVALID, "null ctx");
+  if (ctx->scratch && !ctx->scratch_external) HIP_TRY(hipFree(ctx->scratch));
+  ctx->scratch = d_scratch;
+  ctx->scratch_bytes = d_scratch ? bytes : 0;
+  ctx->scratch_external = d_scratch != nullptr;
+  ctx->counted = false;
+  ctx->batch_counted = false;
   return MDMC_OK;
 }
 
@@ -1056,6 +1114,113 @@ int mdmc_field_gyroid_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3]
   const uint64_t n = (uint64_t)X * Y * Z;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
   hipLaunchKernelGGL(k_field_gyroid, dim3(blocks), dim3(256), 0, st, ga);
+  HIP_TRY(hipGetLastError());
+  return MDMC_OK;
+}
+
+int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32_t n, void *stream) {
+  if (!ctx || !specs) return fail(MDMC_ERR_INVALID, "null argument");
+  if (n == 0 || n > MDMC_MAX_BATCH) return fail(MDMC_ERR_INVALID, "batch size must be 1..1024");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // layout: GenVol[n] then the doubles of every volume's tables
+  size_t ndoubles = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const mdmc_gen_spec &g = specs[v];
+    if (g.kind == MDMC_FIELD_BLOBS)
+      ndoubles += (size_t)g.nblobs * ((size_t)g.dims[0] + g.dims[1] + g.dims[2] + 1);
+    else if (g.kind == MDMC_FIELD_GYROID)
+      ndoubles += 2 * ((size_t)g.dims[0] + g.dims[1] + g.dims[2]);
+    else
+      return fail(MDMC_ERR_INVALID, "unknown field kind");
+    if (!g.out && (uint64_t)g.dims[0] * g.dims[1] * g.dims[2]) return fail(MDMC_ERR_INVALID, "null output");
+  }
+  const size_t head = align256(n * sizeof(GenVol));
+  const size_t bytes = head + ndoubles * sizeof(double);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(st, &cs));
+  if (cs == hipStreamCaptureStatusNone && ctx->ghost_ev_used) HIP_TRY(hipEventSynchronize(ctx->ghost_ev));
+  if (bytes > ctx->ghost_bytes) {
+    if (ctx->ghost) (void)hipHostFree(ctx->ghost);
+    ctx->ghost = nullptr;
+    ctx->ghost_bytes = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 4, 1u << 20);
+    if (hipHostMalloc(&ctx->ghost, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(MDMC_ERR_NOMEM, "pinned staging allocation failed");
+    }
+    ctx->ghost_bytes = want;
+  }
+  if (bytes > ctx->gdev_bytes) {
+    if (ctx->gdev) (void)hipFree(ctx->gdev);
+    ctx->gdev = nullptr;
+    ctx->gdev_bytes = 0;
+    if (hipMalloc(&ctx->gdev, bytes + bytes / 4) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(MDMC_ERR_NOMEM, "generator table allocation failed");
+    }
+    ctx->gdev_bytes = bytes + bytes / 4;
+  }
+  if (!ctx->ghost_ev) HIP_TRY(hipEventCreateWithFlags(&ctx->ghost_ev, hipEventDisableTiming));
+  GenVol *hv = static_cast<GenVol *>(ctx->ghost);
+  double *ht = reinterpret_cast<double *>(static_cast<char *>(ctx->ghost) + head);
+  const double *dt = reinterpret_cast<const double *>(static_cast<char *>(ctx->gdev) + head);
+  size_t off = 0;
+  uint32_t blocks = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    const mdmc_gen_spec &g = specs[v];
+    GenVol &G = hv[v];
+    std::memset(&G, 0, sizeof(G));
+    const uint32_t X = g.dims[0], Y = g.dims[1], Z = g.dims[2];
+    G.out = g.out;
+    G.X = X;
+    G.Y = Y;
+    G.Z = Z;
+    G.kind = (uint32_t)g.kind;
+    G.seed = g.seed;
+    const uint64_t npts = (uint64_t)X * Y * Z;
+    G.block_off = blocks;
+    G.nblocks = npts ? (uint32_t)std::min<uint64_t>((npts + 255) / 256, 4096) : 0;
+    blocks += G.nblocks;
+    double *h = ht + off;
+    const double *d = dt + off;
+    if (g.kind == MDMC_FIELD_BLOBS) {
+      const uint32_t nb = g.nblobs;
+      G.nblobs = nb;
+      double *ex = h, *ey = ex + (size_t)nb * X, *ez = ey + (size_t)nb * Y, *amp = ez + (size_t)nb * Z;
+      for (uint32_t b = 0; b < nb; ++b) {
+        mdmc_blob bp = mdmc_blob_params(g.seed, b, g.dims, g.sigma_lo, g.sigma_hi, g.amp_lo, g.amp_hi);
+        mdmc_gauss_1d(ex + (size_t)b * X, X, bp.cx, bp.sigma);
+        mdmc_gauss_1d(ey + (size_t)b * Y, Y, bp.cy, bp.sigma);
+        mdmc_gauss_1d(ez + (size_t)b * Z, Z, bp.cz, bp.sigma);
+        amp[b] = bp.amp;
+      }
+      G.t[0] = d;
+      G.t[1] = d + (size_t)nb * X;
+      G.t[2] = d + (size_t)nb * (X + Y);
+      G.t[3] = d + (size_t)nb * (X + Y + Z);
+      off += (size_t)nb * ((size_t)X + Y + Z + 1);
+    } else {
+      double *sx = h, *cx = sx + X, *sy = cx + X, *cy = sy + Y, *sz = cy + Y, *cz = sz + Z;
+      mdmc_trig_1d(sx, cx, X, g.period);
+      mdmc_trig_1d(sy, cy, Y, g.period);
+      mdmc_trig_1d(sz, cz, Z, g.period);
+      G.t[0] = d;
+      G.t[1] = d + X;
+      G.t[2] = d + 2 * (size_t)X;
+      G.t[3] = d + 2 * (size_t)X + Y;
+      G.t[4] = d + 2 * (size_t)X + 2 * (size_t)Y;
+      G.t[5] = d + 2 * (size_t)X + 2 * (size_t)Y + Z;
+      G.amp = g.noise_amp;
+      off += 2 * ((size_t)X + Y + Z);
+    }
+  }
+  if (blocks == 0) return MDMC_OK;
+  HIP_TRY(hipMemcpyAsync(ctx->gdev, ctx->ghost, bytes, hipMemcpyHostToDevice, st));
+  if (cs == hipStreamCaptureStatusNone) {
+    HIP_TRY(hipEventRecord(ctx->ghost_ev, st));
+    ctx->ghost_ev_used = true;
+  }
+  hipLaunchKernelGGL(k_field_batch, dim3(blocks), dim3(256), 0, st, static_cast<const GenVol *>(ctx->gdev), n);
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
 }

@@ -1329,4 +1329,44 @@ __global__ __launch_bounds__(256) void k_field_gyroid(GyroidArgs a) {
   }
 }
 
+// Many fields in one launch (the generators above, same arithmetic).  Block
+// b belongs to the volume whose block range holds it; the volume's blocks
+// stride over its points.
+struct GenVol {
+  float *out;
+  uint32_t X, Y, Z, kind;  // kind 1: blobs, 2: gyroid
+  uint32_t nblobs, block_off, nblocks, pad;
+  const double *t[6];      // blobs: ex, ey, ez, amp; gyroid: sx, cx, sy, cy, sz, cz
+  uint64_t seed;
+  double amp;              // gyroid noise amplitude
+};
+
+__global__ __launch_bounds__(256) void k_field_batch(const GenVol *vols, uint32_t nvol) {
+  uint32_t lo = 0, hi = nvol;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (vols[mid].block_off <= blockIdx.x) lo = mid; else hi = mid;
+  }
+  const GenVol &v = vols[__builtin_amdgcn_readfirstlane(lo)];
+  const size_t n = (size_t)v.X * v.Y * v.Z;
+  for (size_t id = (size_t)(blockIdx.x - v.block_off) * blockDim.x + threadIdx.x; id < n;
+       id += (size_t)v.nblocks * blockDim.x) {
+    const uint32_t x = (uint32_t)(id % v.X);
+    const size_t r = id / v.X;
+    const uint32_t y = (uint32_t)(r % v.Y), z = (uint32_t)(r / v.Y);
+    if (v.kind == 1) {
+      double acc = 0.0;
+      for (uint32_t b = 0; b < v.nblobs; ++b)
+        acc = acc + ((v.t[3][b] * v.t[0][(size_t)b * v.X + x]) * v.t[1][(size_t)b * v.Y + y]) *
+                        v.t[2][(size_t)b * v.Z + z];
+      v.out[id] = (float)acc;
+    } else {
+      const double gval = v.t[0][x] * v.t[3][y] + v.t[2][y] * v.t[5][z] + v.t[4][z] * v.t[1][x];
+      const uint64_t h = mix64(v.seed + ((uint64_t)id + 1) * 0x9E3779B97F4A7C15ull);
+      const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+      v.out[id] = (float)(gval + v.amp * (2.0 * u - 1.0));
+    }
+  }
+}
+
 }  // namespace mdmc

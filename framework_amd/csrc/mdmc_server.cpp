@@ -1,0 +1,557 @@
+// Request loop for serving extractions (mdmc_server_* in include/mdmc.h).
+//
+// Memory: one device allocation of the budget, carved by a first-fit pool.
+// A request's footprint is its field (when the server generates it), its
+// share of the batch scratch and an output estimate; admission keeps the sum
+// of the footprints of unfinished requests within the budget.  Outputs are
+// allocated with their exact sizes once the count pass of their batch has
+// finished (the scan gives the sizes).
+//
+// Scheduling: `num_streams` lanes, each with its own stream and extraction
+// context.  An idle lane takes the queued requests in arrival order (up to
+// max_batch, as long as their footprints fit), generates their fields, and
+// runs the batch count; the next poll that sees the count finished allocates
+// the outputs and runs the batch emit; the poll that sees the emit finished
+// frees fields and scratch and hands out the results.  Lanes run
+// independently, so a batch started for late arrivals overlaps with the
+// batches already on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mdmc.h"
+#include "mdmc_pool.h"
+
+namespace {
+
+constexpr uint64_t kNoOffset = ~0ull;
+
+struct Req {
+  uint64_t id = 0;
+  mdmc_request spec{};
+  int status = MDMC_OK;
+  int state = 0;  // 0 queued, 1 in a batch, 2 finished, 3 returned to the caller
+  double t_submit = 0, t_start = 0, t_done = 0;
+  uint64_t field_off = kNoOffset, vert_off = kNoOffset, quad_off = kNoOffset;
+  uint64_t footprint = 0, scratch_est = 0, out_est = 0;
+  const float *field = nullptr;
+  float *verts = nullptr;
+  int32_t *quads = nullptr;
+  mdmc_counts counts{};
+};
+
+struct Batch {
+  std::vector<Req *> reqs;
+  std::vector<mdmc_volume> vols;
+  uint64_t scratch_off = kNoOffset;
+  int phase = 0;  // 1 counting, 2 waiting for output memory, 3 emitting
+};
+
+struct Lane {
+  hipStream_t stream = nullptr;
+  mdmc_ctx *ctx = nullptr;
+  mdmc_counts *d_counts = nullptr;  // device, MDMC_MAX_BATCH entries
+  mdmc_counts *h_counts = nullptr;  // pinned
+  hipEvent_t ev = nullptr;          // end of the current phase
+  Batch *batch = nullptr;
+};
+
+// MDMC_SERVER_TRACE=1: one line per scheduling event on stderr.
+bool trace_on() {
+  static const bool on = [] {
+    const char *e = std::getenv("MDMC_SERVER_TRACE");
+    return e && e[0] && e[0] != '0';
+  }();
+  return on;
+}
+#define TRACE(...)                              \
+  do {                                          \
+    if (trace_on()) {                           \
+      std::fprintf(stderr, "[mdmc_server] " __VA_ARGS__); \
+      std::fputc('\n', stderr);                \
+    }                                           \
+  } while (0)
+
+uint64_t field_bytes(const mdmc_request &r) {
+  return 4ull * r.dims[0] * r.dims[1] * r.dims[2];
+}
+
+}  // namespace
+
+struct mdmc_server {
+  mdmc_server_config cfg{};
+  char *base = nullptr;
+  mdmc::Pool pool;
+  std::vector<Lane> lanes;
+  std::deque<Req *> queue;
+  std::unordered_map<uint64_t, Req *> reqs;
+  std::deque<Req *> finished;  // not yet returned by poll
+  uint64_t next_id = 1;
+  uint64_t reserved = 0;       // footprints of admitted, unfinished requests
+  uint64_t held = 0;           // outputs of finished results not yet released
+  double out_ratio = 1.0;      // output bytes per field byte used for estimates
+  uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, peak = 0;
+  double busy = 0, busy_since = -1;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  int active_lanes() const {
+    int a = 0;
+    for (const Lane &l : lanes) a += l.batch != nullptr;
+    return a;
+  }
+  void note_busy() {
+    const bool b = active_lanes() > 0;
+    const double t = now();
+    if (b && busy_since < 0) busy_since = t;
+    if (!b && busy_since >= 0) {
+      busy += t - busy_since;
+      busy_since = -1;
+    }
+  }
+};
+
+namespace {
+
+bool pool_alloc(mdmc_server *s, uint64_t bytes, uint64_t *off) {
+  if (!s->pool.alloc(std::max<uint64_t>(bytes, 256), off)) return false;
+  s->peak = std::max(s->peak, s->pool.in_use());
+  return true;
+}
+
+void pool_free(mdmc_server *s, uint64_t *off) {
+  if (*off != kNoOffset) s->pool.release(*off);
+  *off = kNoOffset;
+}
+
+// Scratch the count pass needs for one volume alone (the estimate admission
+// uses; a batch's exact scratch is allocated when it starts).
+uint64_t scratch_estimate(const mdmc_request &r) {
+  mdmc_volume v{};
+  std::memcpy(v.dims, r.dims, sizeof(v.dims));
+  uint64_t b = 0;
+  if (mdmc_batch_scratch_bytes(&v, 1, &b) != MDMC_OK) return 0;
+  return b;
+}
+
+void finish(mdmc_server *s, Req *r, int status) {
+  if (r->status == MDMC_OK) r->status = status;
+  r->state = 2;
+  r->t_done = s->now();
+  s->finished.push_back(r);
+  s->completed += 1;
+  if (r->status == MDMC_ERR_REJECTED) s->rejected += 1;
+}
+
+// Frees everything a batch holds except the outputs of successful requests.
+void retire_batch(mdmc_server *s, Lane &lane) {
+  Batch *b = lane.batch;
+  for (Req *r : b->reqs) {
+    pool_free(s, &r->field_off);
+    s->reserved -= std::min(s->reserved, r->footprint);
+    if (r->status != MDMC_OK) {
+      pool_free(s, &r->vert_off);
+      pool_free(s, &r->quad_off);
+      r->verts = nullptr;
+      r->quads = nullptr;
+    } else {
+      s->held += (uint64_t)(12ull * r->counts.num_vertices + 16ull * r->counts.num_quads);
+    }
+  }
+  pool_free(s, &b->scratch_off);
+  delete b;
+  lane.batch = nullptr;
+}
+
+// Starts a batch on an idle lane from the head of the queue.
+int start_batch(mdmc_server *s, Lane &lane) {
+  Batch *b = new (std::nothrow) Batch();
+  if (!b) return MDMC_ERR_NOMEM;
+  std::vector<mdmc_gen_spec> gens;
+  while (!s->queue.empty() && b->reqs.size() < s->cfg.max_batch) {
+    Req *r = s->queue.front();
+    if (s->reserved + s->held + r->footprint > s->pool.size()) break;  // wait for memory (arrival order)
+    if (r->spec.field_kind != MDMC_FIELD_DEVICE) {
+      if (!pool_alloc(s, field_bytes(r->spec), &r->field_off)) break;
+      r->field = reinterpret_cast<const float *>(s->base + r->field_off);
+      mdmc_gen_spec g{};
+      g.out = const_cast<float *>(r->field);
+      std::memcpy(g.dims, r->spec.dims, sizeof(g.dims));
+      g.kind = r->spec.field_kind;
+      g.seed = r->spec.seed;
+      g.nblobs = r->spec.nblobs;
+      g.sigma_lo = r->spec.sigma_lo;
+      g.sigma_hi = r->spec.sigma_hi;
+      g.amp_lo = r->spec.amp_lo;
+      g.amp_hi = r->spec.amp_hi;
+      g.period = r->spec.period;
+      g.noise_amp = r->spec.noise_amp;
+      gens.push_back(g);
+    } else {
+      r->field = r->spec.field;
+    }
+    s->queue.pop_front();
+    s->reserved += r->footprint;
+    r->state = 1;
+    r->t_start = s->now();
+    b->reqs.push_back(r);
+    mdmc_volume v{};
+    v.field = r->field;
+    std::memcpy(v.dims, r->spec.dims, sizeof(v.dims));
+    v.iso = r->spec.iso;
+    std::memcpy(v.xyz_min, r->spec.xyz_min, sizeof(v.xyz_min));
+    std::memcpy(v.xyz_max, r->spec.xyz_max, sizeof(v.xyz_max));
+    b->vols.push_back(v);
+  }
+  const uint32_t n = (uint32_t)b->reqs.size();
+  if (n == 0) {
+    delete b;
+    return MDMC_OK;
+  }
+  TRACE("start batch of %u on stream %p (reserved %llu, in use %llu)", n, (void *)lane.stream,
+        (unsigned long long)s->reserved, (unsigned long long)s->pool.in_use());
+  lane.batch = b;
+  s->batches += 1;
+  auto fail_batch = [&](int rc) {
+    for (Req *r : b->reqs) finish(s, r, rc);
+    retire_batch(s, lane);
+    return MDMC_OK;
+  };
+  uint64_t sbytes = 0;
+  int rc = mdmc_batch_scratch_bytes(b->vols.data(), n, &sbytes);
+  if (rc != MDMC_OK) return fail_batch(rc);
+  TRACE("  scratch %llu bytes, %zu fields to generate", (unsigned long long)sbytes, gens.size());
+  if (sbytes) {
+    if (!pool_alloc(s, sbytes, &b->scratch_off)) {
+      // fragmentation: give the requests back to the queue, in order
+      for (auto it = b->reqs.rbegin(); it != b->reqs.rend(); ++it) {
+        Req *r = *it;
+        pool_free(s, &r->field_off);
+        s->reserved -= std::min(s->reserved, r->footprint);
+        r->state = 0;
+        s->queue.push_front(r);
+      }
+      delete b;
+      lane.batch = nullptr;
+      s->batches -= 1;
+      return MDMC_OK;
+    }
+    rc = mdmc_ctx_set_scratch(lane.ctx, s->base + b->scratch_off, sbytes);
+    if (rc != MDMC_OK) return fail_batch(rc);
+  }
+  if (!gens.empty()) {
+    rc = mdmc_batch_generate_device(lane.ctx, gens.data(), (uint32_t)gens.size(), lane.stream);
+    TRACE("  generate rc %d", rc);
+    if (rc != MDMC_OK) return fail_batch(rc);
+  }
+  rc = mdmc_batch_count_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
+  TRACE("  count rc %d", rc);
+  if (rc != MDMC_OK) return fail_batch(rc);
+  if (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
+                     lane.stream) != hipSuccess ||
+      hipEventRecord(lane.ev, lane.stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail_batch(MDMC_ERR_HIP);
+  }
+  b->phase = 1;
+  return MDMC_OK;
+}
+
+// Allocates exact outputs and launches the emit; false if memory is short.
+bool start_emit(mdmc_server *s, Lane &lane) {
+  Batch *b = lane.batch;
+  const uint32_t n = (uint32_t)b->reqs.size();
+  std::vector<uint64_t> got;
+  bool ok = true;
+  for (uint32_t i = 0; i < n && ok; ++i) {
+    Req *r = b->reqs[i];
+    mdmc_volume &v = b->vols[i];
+    if (r->status != MDMC_OK) continue;
+    const uint64_t V = r->counts.num_vertices, Q = r->counts.num_quads;
+    if (V && r->vert_off == kNoOffset) {
+      if (pool_alloc(s, 12 * V, &r->vert_off)) got.push_back(r->vert_off); else ok = false;
+    }
+    if (ok && Q && r->quad_off == kNoOffset) {
+      if (pool_alloc(s, 16 * Q, &r->quad_off)) got.push_back(r->quad_off); else ok = false;
+    }
+    if (!ok) break;
+    r->verts = V ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
+    r->quads = Q ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
+    v.vertices = r->verts;
+    v.vertex_capacity = V;
+    v.quads = r->quads;
+    v.quad_capacity = Q;
+  }
+  if (!ok) {
+    for (Req *r : b->reqs) {
+      pool_free(s, &r->vert_off);
+      pool_free(s, &r->quad_off);
+    }
+    return false;
+  }
+  int rc = mdmc_batch_emit_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
+  TRACE("emit batch of %u rc %d", n, rc);
+  if (rc == MDMC_OK &&
+      (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
+                      lane.stream) != hipSuccess ||
+       hipEventRecord(lane.ev, lane.stream) != hipSuccess)) {
+    (void)hipGetLastError();
+    rc = MDMC_ERR_HIP;
+  }
+  if (rc != MDMC_OK) {
+    for (Req *r : b->reqs) finish(s, r, rc);
+    retire_batch(s, lane);
+    return true;
+  }
+  b->phase = 3;
+  return true;
+}
+
+// One pass over the lanes; returns true if anything changed.
+bool pump(mdmc_server *s) {
+  bool progress = false;
+  for (Lane &lane : s->lanes) {
+    Batch *b = lane.batch;
+    if (!b) continue;
+    if (b->phase == 1 || b->phase == 3) {
+      const hipError_t q = hipEventQuery(lane.ev);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) {
+        (void)hipGetLastError();
+        for (Req *r : b->reqs) finish(s, r, MDMC_ERR_HIP);
+        retire_batch(s, lane);
+        progress = true;
+        continue;
+      }
+      for (size_t i = 0; i < b->reqs.size(); ++i) b->reqs[i]->counts = lane.h_counts[i];
+      if (b->phase == 1) {
+        for (Req *r : b->reqs)
+          if (r->counts.status) r->status = MDMC_ERR_INTERNAL;
+        b->phase = 2;
+      } else {
+        uint64_t fb = 0, ob = 0;
+        for (Req *r : b->reqs) {
+          if (r->counts.status && r->status == MDMC_OK) r->status = MDMC_ERR_INTERNAL;
+          fb += field_bytes(r->spec);
+          ob += 12ull * r->counts.num_vertices + 16ull * r->counts.num_quads;
+          finish(s, r, MDMC_OK);
+        }
+        TRACE("batch of %zu done", b->reqs.size());
+        if (fb) s->out_ratio = std::max(0.05, 0.75 * s->out_ratio + 0.25 * 1.5 * (double)ob / (double)fb);
+        retire_batch(s, lane);
+        progress = true;
+        continue;
+      }
+      progress = true;
+    }
+    if (b->phase == 2) {
+      if (start_emit(s, lane)) {
+        progress = true;
+      } else if (s->held == 0 && std::all_of(s->lanes.begin(), s->lanes.end(), [](const Lane &l) {
+                   return !l.batch || l.batch->phase == 2;
+                 })) {
+        // Nothing in flight or held by unreleased results will free memory:
+        // the outputs cannot fit the budget.
+        for (Req *r : b->reqs) finish(s, r, MDMC_ERR_REJECTED);
+        retire_batch(s, lane);
+        progress = true;
+      }
+    }
+  }
+  for (Lane &lane : s->lanes) {
+    if (lane.batch || s->queue.empty()) continue;
+    if (start_batch(s, lane) != MDMC_OK) break;
+    if (lane.batch) progress = true;
+  }
+  s->note_busy();
+  return progress;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out) {
+  if (!cfg || !out) return MDMC_ERR_INVALID;
+  *out = nullptr;
+  if (cfg->max_batch == 0 || cfg->max_batch > MDMC_MAX_BATCH || cfg->num_streams == 0 || cfg->num_streams > 8 ||
+      cfg->memory_budget < (1ull << 20))
+    return MDMC_ERR_INVALID;
+  int rc = mdmc_setup();
+  if (rc != MDMC_OK) return rc;
+  mdmc_server *s = new (std::nothrow) mdmc_server();
+  if (!s) return MDMC_ERR_NOMEM;
+  s->cfg = *cfg;
+  if (hipMalloc(reinterpret_cast<void **>(&s->base), cfg->memory_budget) != hipSuccess) {
+    (void)hipGetLastError();
+    delete s;
+    return MDMC_ERR_NOMEM;
+  }
+  s->pool.reset(cfg->memory_budget);
+  s->lanes.resize(cfg->num_streams);
+  for (Lane &l : s->lanes) {
+    bool ok = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&l.ev, hipEventDisableTiming) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void **>(&l.d_counts), MDMC_MAX_BATCH * sizeof(mdmc_counts)) == hipSuccess &&
+              hipHostMalloc(reinterpret_cast<void **>(&l.h_counts), MDMC_MAX_BATCH * sizeof(mdmc_counts)) ==
+                  hipSuccess &&
+              mdmc_ctx_create(&l.ctx) == MDMC_OK;
+    if (!ok) {
+      (void)hipGetLastError();
+      mdmc_server_destroy(s);
+      return MDMC_ERR_NOMEM;
+    }
+  }
+  *out = s;
+  return MDMC_OK;
+}
+
+void mdmc_server_destroy(mdmc_server *s) {
+  if (!s) return;
+  for (Lane &l : s->lanes) {
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
+    if (l.batch) delete l.batch;
+    if (l.ctx) mdmc_ctx_destroy(l.ctx);
+    if (l.d_counts) (void)hipFree(l.d_counts);
+    if (l.h_counts) (void)hipHostFree(l.h_counts);
+    if (l.ev) (void)hipEventDestroy(l.ev);
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+  }
+  for (auto &kv : s->reqs) delete kv.second;
+  if (s->base) (void)hipFree(s->base);
+  delete s;
+}
+
+int mdmc_server_submit(mdmc_server *s, const mdmc_request *req, uint64_t *id) {
+  if (!s || !req || !id) return MDMC_ERR_INVALID;
+  if (req->field_kind != MDMC_FIELD_DEVICE && req->field_kind != MDMC_FIELD_BLOBS &&
+      req->field_kind != MDMC_FIELD_GYROID)
+    return MDMC_ERR_INVALID;
+  if (req->field_kind == MDMC_FIELD_DEVICE && !req->field &&
+      (uint64_t)req->dims[0] * req->dims[1] * req->dims[2] > 0)
+    return MDMC_ERR_INVALID;
+  Req *r = new (std::nothrow) Req();
+  if (!r) return MDMC_ERR_NOMEM;
+  r->id = s->next_id++;
+  TRACE("submit %llu dims %u %u %u kind %d", (unsigned long long)r->id, req->dims[0], req->dims[1], req->dims[2],
+        req->field_kind);
+  r->spec = *req;
+  r->t_submit = s->now();
+  const uint64_t fb = field_bytes(*req);
+  r->scratch_est = scratch_estimate(*req);
+  r->out_est = (uint64_t)(s->out_ratio * (double)fb);
+  r->footprint = (req->field_kind != MDMC_FIELD_DEVICE ? fb : 0) + r->scratch_est + r->out_est;
+  s->reqs[r->id] = r;
+  s->submitted += 1;
+  *id = r->id;
+  if (r->footprint > s->pool.size()) {
+    finish(s, r, MDMC_ERR_REJECTED);
+    return MDMC_OK;
+  }
+  s->queue.push_back(r);
+  pump(s);
+  return MDMC_OK;
+}
+
+int mdmc_server_poll(mdmc_server *s, int wait, mdmc_result *out, uint32_t max, uint32_t *n) {
+  if (!s || !n || (max && !out)) return MDMC_ERR_INVALID;
+  *n = 0;
+  for (;;) {
+    while (pump(s)) {
+    }
+    if (!s->finished.empty() || !wait) break;
+    // block on a phase in flight; with none (empty, or waiting for results
+    // to be released) there is nothing to wait for
+    bool waited = false;
+    for (Lane &l : s->lanes)
+      if (l.batch && (l.batch->phase == 1 || l.batch->phase == 3)) {
+        (void)hipEventSynchronize(l.ev);
+        waited = true;
+        break;
+      }
+    if (!waited) break;
+  }
+  while (*n < max && !s->finished.empty()) {
+    Req *r = s->finished.front();
+    s->finished.pop_front();
+    r->state = 3;
+    mdmc_result &o = out[(*n)++];
+    std::memset(&o, 0, sizeof(o));
+    o.id = r->id;
+    o.tag = r->spec.tag;
+    o.status = r->status;
+    o.counts = r->counts;
+    o.vertices = r->verts;
+    o.quads = r->quads;
+    o.t_submit = r->t_submit;
+    o.t_start = r->t_start;
+    o.t_done = r->t_done;
+  }
+  return MDMC_OK;
+}
+
+int mdmc_server_copy_result(mdmc_server *s, uint64_t id, float *h_vertices, int32_t *h_quads) {
+  if (!s) return MDMC_ERR_INVALID;
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end() || it->second->state < 2) return MDMC_ERR_INVALID;
+  Req *r = it->second;
+  if (r->status != MDMC_OK) return r->status;
+  if (h_vertices && r->verts &&
+      hipMemcpy(h_vertices, r->verts, 12ull * r->counts.num_vertices, hipMemcpyDeviceToHost) != hipSuccess)
+    return MDMC_ERR_HIP;
+  if (h_quads && r->quads &&
+      hipMemcpy(h_quads, r->quads, 16ull * r->counts.num_quads, hipMemcpyDeviceToHost) != hipSuccess)
+    return MDMC_ERR_HIP;
+  return MDMC_OK;
+}
+
+int mdmc_server_release(mdmc_server *s, uint64_t id) {
+  if (!s) return MDMC_ERR_INVALID;
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end() || it->second->state < 2) return MDMC_ERR_INVALID;
+  Req *r = it->second;
+  if (r->vert_off != kNoOffset || r->quad_off != kNoOffset)
+    s->held -= std::min<uint64_t>(s->held, 12ull * r->counts.num_vertices + 16ull * r->counts.num_quads);
+  pool_free(s, &r->vert_off);
+  pool_free(s, &r->quad_off);
+  if (r->state == 2) {  // still in the finished list: drop it there too
+    auto f = std::find(s->finished.begin(), s->finished.end(), r);
+    if (f != s->finished.end()) s->finished.erase(f);
+  }
+  s->reqs.erase(it);
+  delete r;
+  return MDMC_OK;
+}
+
+int mdmc_server_get_stats(mdmc_server *s, mdmc_server_stats *st) {
+  if (!s || !st) return MDMC_ERR_INVALID;
+  s->note_busy();
+  std::memset(st, 0, sizeof(*st));
+  st->submitted = s->submitted;
+  st->completed = s->completed;
+  st->rejected = s->rejected;
+  st->batches = s->batches;
+  st->queued = s->queue.size();
+  uint64_t inflight = 0;
+  for (const Lane &l : s->lanes) inflight += l.batch ? l.batch->reqs.size() : 0;
+  st->in_flight = inflight;
+  st->memory_in_use = s->pool.in_use();
+  st->memory_peak = s->peak;
+  st->memory_budget = s->pool.size();
+  st->busy_seconds = s->busy + (s->busy_since >= 0 ? s->now() - s->busy_since : 0.0);
+  return MDMC_OK;
+}
+
+double mdmc_server_now(mdmc_server *s) { return s ? s->now() : 0.0; }
+
+}  // extern "C"

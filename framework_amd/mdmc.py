@@ -66,6 +66,14 @@ class Volume(ctypes.Structure):
                 ("quads", _VP), ("quad_capacity", ctypes.c_uint64)]
 
 
+class GenSpec(ctypes.Structure):
+    """mdmc_gen_spec: one synthetic field of a batched generation."""
+    _fields_ = [("out", _VP), ("dims", ctypes.c_uint32 * 3), ("kind", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("nblobs", ctypes.c_uint32), ("sigma_lo", ctypes.c_double), ("sigma_hi", ctypes.c_double),
+                ("amp_lo", ctypes.c_double), ("amp_hi", ctypes.c_double), ("period", ctypes.c_double),
+                ("noise_amp", ctypes.c_double)]
+
+
 MAX_BATCH = 1024
 OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS_PER_CU = 0, 1, 2
 
@@ -115,6 +123,8 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [_VP, ctypes.POINTER(Volume), ctypes.c_uint32, _VP, _VP]
         f.restype = ctypes.c_int
+    L.mdmc_batch_generate_device.argtypes = [_VP, ctypes.POINTER(GenSpec), ctypes.c_uint32, _VP]
+    L.mdmc_batch_generate_device.restype = ctypes.c_int
     L.mdmc_ctx_set_option.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     L.mdmc_ctx_set_option.restype = ctypes.c_int
     L.mdmc_ctx_profile.argtypes = [_VP, ctypes.c_int]
@@ -397,7 +407,32 @@ def _batch_methods():
                                                  "num_quads": int(c[i, 2]), "status": st}))
         return res
 
-    for f in (set_option, batch_counts_tensor, batch_count, batch_emit, batch_extract_into, batch_extract):
+    def batch_generate(self, specs, stream=None):
+        """Fields for many volumes in one launch.  specs: dicts with dims (X, Y, Z), kind ("blobs" /
+        "gyroid") and the generator parameters; returns the (Z, Y, X) tensors."""
+        torch = self._torch
+        arr = (GenSpec * len(specs))()
+        outs = []
+        for i, sp in enumerate(specs):
+            X, Y, Z = sp["dims"]
+            o = torch.empty((Z, Y, X), dtype=torch.float32, device="cuda")
+            outs.append(o)
+            g = arr[i]
+            g.out = o.data_ptr()
+            g.dims[0], g.dims[1], g.dims[2] = X, Y, Z
+            g.kind = 1 if sp["kind"] == "blobs" else 2
+            g.seed = int(sp.get("seed", 0))
+            g.nblobs = int(sp.get("nblobs", 16))
+            g.sigma_lo, g.sigma_hi = map(float, sp.get("sigma", (8.0, 32.0)))
+            g.amp_lo, g.amp_hi = map(float, sp.get("amp", (-1.0, 1.0)))
+            g.period = float(sp.get("period", 24.0))
+            g.noise_amp = float(sp.get("noise_amp", 0.05))
+        _check(lib().mdmc_batch_generate_device(self._h, arr, len(specs), self._stream(stream)),
+               "mdmc_batch_generate_device")
+        return outs
+
+    for f in (set_option, batch_counts_tensor, batch_count, batch_emit, batch_extract_into, batch_extract,
+              batch_generate):
         setattr(Context, f.__name__, f)
 
 

@@ -168,6 +168,29 @@ int mdmc_batch_emit_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, m
 int mdmc_batch_extract_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
                               void *stream);
 
+/* Synthetic fields for many volumes in one launch (bitwise the fields of
+ * mdmc_field_blobs_device / mdmc_field_gyroid_device with the same
+ * parameters). */
+enum { MDMC_FIELD_DEVICE = 0, MDMC_FIELD_BLOBS = 1, MDMC_FIELD_GYROID = 2 };
+typedef struct mdmc_gen_spec {
+  float *out;        /* device, dims[0]*dims[1]*dims[2] floats, x fastest */
+  uint32_t dims[3];
+  int kind;          /* MDMC_FIELD_BLOBS or MDMC_FIELD_GYROID */
+  uint64_t seed;
+  uint32_t nblobs;   /* blobs */
+  double sigma_lo, sigma_hi, amp_lo, amp_hi;
+  double period, noise_amp;  /* gyroid */
+} mdmc_gen_spec;
+int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32_t n, void *stream);
+
+/* Device scratch bytes mdmc_batch_count_device needs for this batch (fields
+ * may be NULL here; only dims are read). */
+int mdmc_batch_scratch_bytes(const mdmc_volume *vols, uint32_t n, uint64_t *bytes);
+/* Make the context use caller-owned device memory as scratch (NULL: back to
+ * its own allocations).  A call that needs more than `bytes` returns
+ * MDMC_ERR_CAPACITY instead of allocating. */
+int mdmc_ctx_set_scratch(mdmc_ctx *ctx, void *d_scratch, uint64_t bytes);
+
 /* Tuning options of a context (0 restores the default). */
 enum {
   MDMC_OPT_BATCH_PLANES = 0,          /* cell planes per batch classify tile */
@@ -175,6 +198,67 @@ enum {
   MDMC_OPT_EMIT_BLOCKS_PER_CU = 2      /* emit grid */
 };
 int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value);
+
+/* ---- serving (no counterpart in the reference): a request loop that admits
+ * extraction jobs by their device-memory footprint, batches the admitted ones
+ * (continuous batching over a few streams, so that jobs arriving while
+ * others run are started as memory and streams free up) and keeps fields and
+ * results resident.  One host thread drives a server. ---------------------- */
+
+typedef struct mdmc_server mdmc_server;
+
+typedef struct mdmc_server_config {
+  uint64_t memory_budget;  /* bytes of device memory for fields, scratch and results */
+  uint32_t max_batch;      /* volumes per launch sequence, 1..MDMC_MAX_BATCH */
+  uint32_t num_streams;    /* launch sequences in flight, 1..8 */
+} mdmc_server_config;
+
+typedef struct mdmc_request {
+  uint32_t dims[3];
+  float iso;
+  float xyz_min[3];
+  float xyz_max[3];
+  int field_kind;        /* MDMC_FIELD_DEVICE (caller's field, kept alive until the
+                            result is returned) or a generator kind */
+  const float *field;    /* MDMC_FIELD_DEVICE */
+  uint64_t seed;         /* generators (see mdmc_gen_spec) */
+  uint32_t nblobs;
+  double sigma_lo, sigma_hi, amp_lo, amp_hi;
+  double period, noise_amp;
+  uint64_t tag;          /* returned with the result */
+} mdmc_request;
+
+typedef struct mdmc_result {
+  uint64_t id, tag;
+  int status;            /* MDMC_OK, MDMC_ERR_REJECTED (footprint over budget), ... */
+  mdmc_counts counts;
+  const float *vertices; /* device memory of the server, valid until release */
+  const int32_t *quads;
+  double t_submit, t_start, t_done;  /* seconds on the server clock */
+} mdmc_result;
+
+typedef struct mdmc_server_stats {
+  uint64_t submitted, completed, rejected, batches;
+  uint64_t queued, in_flight;
+  uint64_t memory_in_use, memory_peak, memory_budget;
+  double busy_seconds;   /* time with at least one launch sequence in flight */
+} mdmc_server_stats;
+
+int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out);
+void mdmc_server_destroy(mdmc_server *srv);
+/* Queues a request; *id identifies its result. */
+int mdmc_server_submit(mdmc_server *srv, const mdmc_request *req, uint64_t *id);
+/* Advances the loop (admission, launches, completions) and returns up to
+ * `max` finished results.  wait != 0: block until at least one result is
+ * ready (or nothing is queued or in flight). */
+int mdmc_server_poll(mdmc_server *srv, int wait, mdmc_result *out, uint32_t max, uint32_t *n);
+/* Copies a finished result's mesh to host memory (sizes from its counts). */
+int mdmc_server_copy_result(mdmc_server *srv, uint64_t id, float *h_vertices, int32_t *h_quads);
+/* Frees a finished result's device memory. */
+int mdmc_server_release(mdmc_server *srv, uint64_t id);
+int mdmc_server_get_stats(mdmc_server *srv, mdmc_server_stats *st);
+/* Seconds on the server clock (the clock of t_submit / t_done). */
+double mdmc_server_now(mdmc_server *srv);
 
 /* ---- synthetic fields on the device ------------------------------------- */
 

@@ -181,3 +181,22 @@ def test_zmarch_and_plane_tile_kernels_agree(fa_mod, torch_cuda):
         assert torch.equal(q0, q1), dims
         assert torch.equal(v0.view(torch.int32), v1.view(torch.int32)), dims
     ctx.close()
+
+
+def test_batch_generator_matches_single_generators(fa_mod, torch_cuda):
+    torch = torch_cuda
+    ctx = fa_mod.Context()
+    specs = [{"dims": (43, 43, 43), "kind": "gyroid", "seed": 15843, "period": 15.7, "noise_amp": 0.1},
+             {"dims": (32, 30, 31), "kind": "blobs", "seed": 9, "nblobs": 12, "sigma": (2.0, 10.0)},
+             {"dims": (7, 90, 5), "kind": "gyroid", "seed": 1, "period": 6.0, "noise_amp": 0.5}]
+    for rep in range(3):  # staging reuse across calls
+        outs = ctx.batch_generate(specs[rep:] + specs[:rep])
+        torch.cuda.synchronize()
+        for sp, o in zip(specs[rep:] + specs[:rep], outs):
+            if sp["kind"] == "blobs":
+                ref = ctx.field_blobs(sp["dims"], sp["seed"], sp["nblobs"], sp["sigma"])
+            else:
+                ref = ctx.field_gyroid(sp["dims"], sp["period"], sp["seed"], sp["noise_amp"])
+            torch.cuda.synchronize()
+            assert torch.equal(o.view(torch.int32), ref.view(torch.int32)), (rep, sp)
+    ctx.close()

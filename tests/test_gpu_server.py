@@ -1,0 +1,121 @@
+"""The request loop (GPU): results alone and in a mix are identical, admission
+by memory footprint, no request is lost under a tight budget."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    torch.cuda.set_device(0)
+    return torch
+
+
+@pytest.fixture(scope="module")
+def fa_mod(torch_cuda):
+    import framework_amd as fa
+
+    fa.setup()
+    return fa
+
+
+def _requests(n, seed, dmin=24, dmax=96):
+    from framework_amd.server import make_request
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        d = int(round(np.exp(rng.uniform(np.log(dmin), np.log(dmax)))))
+        iso = float(rng.uniform(-0.2, 0.2))
+        if i % 2 == 0:
+            reqs.append(make_request((d, d, d), iso, "blobs", seed=seed * 7919 + i, nblobs=int(rng.integers(8, 33)),
+                                     sigma=(2.0, 10.0), tag=i))
+        else:
+            reqs.append(make_request((d, d, d), iso, "gyroid", seed=seed * 7919 + i, period=float(rng.uniform(8, 16)),
+                                     noise_amp=0.1, tag=i))
+    return reqs
+
+
+def _single(fa, r):
+    """The same request through the single-volume entry points."""
+    ctx = fa.Context()
+    X, Y, Z = r.dims[0], r.dims[1], r.dims[2]
+    if r.field_kind == 1:
+        f = ctx.field_blobs((X, Y, Z), r.seed, r.nblobs, (r.sigma_lo, r.sigma_hi), (r.amp_lo, r.amp_hi))
+    else:
+        f = ctx.field_gyroid((X, Y, Z), r.period, r.seed, r.noise_amp)
+    v, q, c = ctx.extract(f, r.iso, tuple(r.xyz_min), tuple(r.xyz_max))
+    out = v.cpu().numpy(), q.cpu().numpy()
+    ctx.close()
+    return out
+
+
+def test_result_alone_equals_result_in_mix(fa_mod, torch_cuda):
+    from framework_amd.server import Server, run_stream
+
+    reqs = _requests(40, seed=2)
+    probe = [5, 17, 30]
+    alone = {}
+    srv = Server(2 << 30, max_batch=8, num_streams=2)
+    for k in probe:
+        res = run_stream(srv, [reqs[k]], np.zeros(1), keep={0})
+        alone[k] = res["meshes"][0]
+    srv.close()
+    srv = Server(2 << 30, max_batch=8, num_streams=3)
+    arrivals = np.cumsum(np.random.default_rng(0).exponential(2e-4, len(reqs)))
+    res = run_stream(srv, reqs, arrivals, keep=set(probe))
+    st = srv.stats()
+    srv.close()
+    assert (res["status"] == 0).all()
+    assert st["batches"] >= 2 and st["completed"] == len(reqs)
+    for k in probe:
+        va, qa = alone[k]
+        vm, qm = res["meshes"][k]
+        vs, qs = _single(fa_mod, reqs[k])
+        assert np.array_equal(qa, qm) and np.array_equal(qa, qs), k
+        assert np.array_equal(va.view(np.uint32), vm.view(np.uint32)), k
+        assert np.array_equal(va.view(np.uint32), vs.view(np.uint32)), k
+
+
+def test_admission_under_a_tight_budget(fa_mod, torch_cuda):
+    """Requests larger than the budget are rejected; everything else finishes
+    with the peak memory within the budget."""
+    from framework_amd.server import Server, make_request, run_stream
+
+    budget = 96 << 20
+    reqs = _requests(30, seed=4, dmin=24, dmax=80)
+    reqs.insert(7, make_request((400, 400, 400), 0.0, "gyroid", seed=1, tag=99))  # 256 MB field
+    srv = Server(budget, max_batch=6, num_streams=2)
+    res = run_stream(srv, reqs, np.zeros(len(reqs)))
+    st = srv.stats()
+    srv.close()
+    assert res["status"][7] == -8  # MDMC_ERR_REJECTED
+    ok = np.delete(res["status"], 7)
+    assert (ok == 0).all(), ok
+    assert st["memory_peak"] <= budget
+    assert st["completed"] == len(reqs)
+
+
+def test_device_field_requests(fa_mod, torch_cuda):
+    """Caller-owned fields are extracted in place."""
+    from framework_amd.server import Server, make_request, run_stream
+
+    ctx = fa_mod.Context()
+    fields = [ctx.field_blobs((40 + 3 * i, 33, 29), seed=i, nblobs=9, sigma=(3.0, 9.0)) for i in range(5)]
+    torch_cuda.cuda.synchronize()
+    reqs = [make_request((f.shape[2], f.shape[1], f.shape[0]), 0.05, field=f, tag=i) for i, f in enumerate(fields)]
+    srv = Server(512 << 20, max_batch=4, num_streams=2)
+    res = run_stream(srv, reqs, np.zeros(len(reqs)), keep=set(range(5)))
+    srv.close()
+    for i, f in enumerate(fields):
+        v, q, c = ctx.extract(f, 0.05)
+        vm, qm = res["meshes"][i]
+        assert np.array_equal(q.cpu().numpy(), qm)
+        assert np.array_equal(v.cpu().numpy().view(np.uint32), vm.view(np.uint32))
+    ctx.close()
