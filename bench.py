@@ -11,6 +11,17 @@ Prints ONE JSON line on stdout.  `value` is Gvoxels/s (field samples per
 second).  `roofline` is for the dominant kernel (longest average launch),
 timed live with HIP events the library records around each launch on its
 stream.  `cpu_baseline` is the C oracle (single thread) on the same field.
+
+Two more legs go into the same line:
+  "batched": 256 volumes, dims per axis log-uniform in [32, 256], alternating
+      blob sums (8-32 blobs) and gyroid+noise, isovalues U[-0.2, 0.2]; the
+      plain per-volume loop (count, read sizes, allocate, emit, per volume)
+      against one batched launch sequence and its HIP-graph replay.
+  "served": a Poisson stream of requests (cubic dims log-uniform in
+      [48, 384], blob or gyroid fields generated from per-request seeds,
+      isovalues U[-0.2, 0.2]) through the request loop under a 64 GB budget
+      at 0.8x its measured capacity: volumes/s, p50/p99 latency, and the
+      plain per-volume loop on the same requests.
 """
 from __future__ import annotations
 
@@ -39,6 +50,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy --steps times (a kernel of known "
                          "HBM traffic, used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -138,6 +152,9 @@ def main() -> int:
                "sample": f"full 512^3 blob field, {len(times)} run(s), median {tmed:.3f} s "
                          f"(C oracle restating dmc::run_dmc, one thread, host of the GPU box)"}
 
+    batched = None if args.skip_batched else bench_batched(fa, torch)
+    served = None if args.skip_served else bench_served(fa, torch, args.served_requests)
+
     out = {
         "metric": "Gvoxels/s + %HBM peak at 512^3; served volumes/s with p50/p99 latency",
         "value": N / (ms_step * 1e-3) / 1e9,
@@ -160,10 +177,151 @@ def main() -> int:
         "pipeline": {"algorithmic_bytes": B, "achieved_gbs": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
                      "kernel_avg_ms": per_kernel},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def _log_uniform_int(rng, lo, hi):
+    import numpy as np
+
+    return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+
+
+def bench_batched(fa, torch, nvol: int = 256, reps: int = 5) -> dict:
+    """256 uneven volumes: plain per-volume loop vs one batched launch sequence."""
+    import numpy as np
+
+    rng = np.random.default_rng(1)
+    specs, isos = [], []
+    for i in range(nvol):
+        dims = tuple(_log_uniform_int(rng, 32, 256) for _ in range(3))
+        if i % 2 == 0:
+            specs.append({"dims": dims, "kind": "blobs", "seed": 1000 + i, "nblobs": int(rng.integers(8, 33)),
+                          "sigma": (4.0, 16.0)})
+        else:
+            specs.append({"dims": dims, "kind": "gyroid", "seed": 1000 + i, "period": 24.0, "noise_amp": 0.05})
+        isos.append(float(rng.uniform(-0.2, 0.2)))
+    ctx = fa.Context()
+    log("batched: generating 256 fields")
+    fields = ctx.batch_generate(specs)
+    torch.cuda.synchronize()
+    points = sum(int(f.numel()) for f in fields)
+    # plain loop: the single-volume protocol per volume
+    sctx = fa.Context()
+    for f, iso in zip(fields[:4], isos[:4]):
+        sctx.extract(f, iso)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    V = Q = 0
+    for f, iso in zip(fields, isos):
+        v, q, c = sctx.extract(f, iso)
+        V += c["num_vertices"]
+        Q += c["num_quads"]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t
+    # batched: count all, one sync, allocate, emit all
+    ctx.batch_extract(fields, isos)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        res = ctx.batch_extract(fields, isos)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    t_batch = sorted(times)[len(times) // 2]
+    assert sum(r[2]["num_vertices"] for r in res) == V and sum(r[2]["num_quads"] for r in res) == Q
+    # graph replay of count + emit into fixed outputs
+    outs = [(r[0], r[1]) for r in res]
+    counts = ctx.batch_counts_tensor(nvol)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        ctx.batch_extract_into(fields, isos, outs, counts, stream=stream)
+    stream.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        ctx.batch_extract_into(fields, isos, outs, counts, stream=torch.cuda.current_stream())
+    g.replay()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    t_graph = sorted(times)[len(times) // 2]
+    st = counts.cpu().numpy().view(np.uint32)
+    assert (st[:, 3] == 0).all() and int(st[:, 1].sum()) == V
+    ctx.close()
+    sctx.close()
+    out = {"volumes": nvol, "points": points, "vertices": V, "quads": Q,
+           "plain_loop_volumes_per_s": nvol / t_plain, "batched_volumes_per_s": nvol / t_batch,
+           "graph_volumes_per_s": nvol / t_graph, "speedup_batched": t_plain / t_batch,
+           "speedup_graph": t_plain / t_graph, "batched_gvoxels_per_s": points / t_batch / 1e9,
+           "graph_gvoxels_per_s": points / t_graph / 1e9}
+    log(f"batched: {out}")
+    return out
+
+
+def _served_requests(n: int, seed: int):
+    import numpy as np
+    from framework_amd.server import make_request
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        d = _log_uniform_int(rng, 48, 384)
+        iso = float(rng.uniform(-0.2, 0.2))
+        if rng.random() < 0.5:
+            reqs.append(make_request((d, d, d), iso, "blobs", seed=seed * 1_000_003 + i,
+                                     nblobs=int(rng.integers(8, 33)), sigma=(4.0, 16.0), tag=i))
+        else:
+            reqs.append(make_request((d, d, d), iso, "gyroid", seed=seed * 1_000_003 + i, period=24.0,
+                                     noise_amp=0.05, tag=i))
+    return reqs
+
+
+def bench_served(fa, torch, nreq: int, budget: int = 64 << 30) -> dict:
+    """Poisson stream through the request loop at 0.8x its measured capacity."""
+    import numpy as np
+    from framework_amd.server import Server, run_plain_loop, run_stream
+
+    reqs = _served_requests(nreq, seed=5)
+    srv = Server(budget, max_batch=32, num_streams=4)
+    # capacity: a saturating burst (everything arrives at once) on a sample
+    ncap = min(1000, nreq)
+    log(f"served: capacity burst of {ncap}")
+    run_stream(srv, reqs[: min(50, nreq)], np.zeros(min(50, nreq)))  # warm the lanes
+    cap = run_stream(srv, reqs[:ncap], np.zeros(ncap))
+    capacity = ncap / cap["wall"]
+    rate = 0.8 * capacity
+    arrivals = np.cumsum(np.random.default_rng(6).exponential(1.0 / rate, nreq))
+    log(f"served: {nreq} requests at {rate:.0f}/s")
+    res = run_stream(srv, reqs, arrivals)
+    st = srv.stats()
+    srv.close()
+    ok = res["status"] == 0
+    lat = np.sort(res["latency"][ok])
+    # plain loop on a sample of the same requests
+    nplain = min(300, nreq)
+    ctx = fa.Context()
+    run_plain_loop(ctx, reqs[:5])
+    plain = run_plain_loop(ctx, reqs[:nplain])
+    ctx.close()
+    plain_vps = nplain / plain["wall"]
+    out = {"requests": nreq, "completed_ok": int(ok.sum()), "rejected": int((res["status"] == -8).sum()),
+           "offered_rate": rate, "capacity_volumes_per_s": capacity,
+           "volumes_per_s": nreq / res["wall"], "p50_ms": float(np.percentile(lat, 50) * 1e3),
+           "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": st["batches"],
+           "memory_peak_gb": st["memory_peak"] / 1e9, "budget_gb": budget / 1e9,
+           "plain_loop_volumes_per_s": plain_vps, "plain_loop_sample": nplain,
+           "plain_loop_p50_ms": float(np.percentile(plain["latency"], 50) * 1e3),
+           "speedup_vs_plain_loop": capacity / plain_vps}
+    log(f"served: {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -111,6 +111,7 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   p->nslots = p->nsegs * p->cap;
   const uint64_t cells = (uint64_t)p->cx * p->cy * p->cz;
   p->max_chunks = (cells + 63) / 64 + 1;
+  p->ncoord = (uint64_t)p->X + p->Y + p->Z;
   // whole scan tiles with at least one padding segment (its vertex / quad
   // prefix is the grand total)
   p->ntiles = (uint32_t)(p->nsegs / kScanTile + 1);
@@ -212,7 +213,7 @@ struct ProfScope {
 static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 struct ScratchLayout {
-  size_t counts, vq, ne, chunk, state, ticket, totals, slots, gmap, total;
+  size_t counts, vq, ne, chunk, state, ticket, totals, slots, gmap, coords, total;
 };
 
 ScratchLayout scratch_layout(const Plan &p) {
@@ -233,6 +234,7 @@ ScratchLayout scratch_layout(const Plan &p) {
   L.totals = take(sizeof(Totals));
   L.slots = take(p.nslots * sizeof(uint64_t));
   L.gmap = take(p.nsegs * (p.cap / 64) * sizeof(uint4));
+  L.coords = take(p.ncoord * sizeof(float));
   L.total = off;
   return L;
 }
@@ -242,7 +244,8 @@ int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
   const ScratchLayout lay = scratch_layout(p);
   const size_t off = lay.total;
   const size_t o_counts = lay.counts, o_vq = lay.vq, o_ne = lay.ne, o_chunk = lay.chunk, o_state = lay.state,
-               o_ticket = lay.ticket, o_totals = lay.totals, o_slots = lay.slots, o_gmap = lay.gmap;
+               o_ticket = lay.ticket, o_totals = lay.totals, o_slots = lay.slots, o_gmap = lay.gmap,
+               o_coords = lay.coords;
   if (off > ctx->scratch_bytes && ctx->scratch_external)
     return fail(MDMC_ERR_CAPACITY, "the scratch memory given to the context is too small");
   if (off > ctx->scratch_bytes) {
@@ -275,6 +278,7 @@ int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
   s.totals = reinterpret_cast<Totals *>(b + o_totals);
   s.slots = reinterpret_cast<uint64_t *>(b + o_slots);
   s.gmap = reinterpret_cast<uint4 *>(b + o_gmap);
+  s.coords = reinterpret_cast<float *>(b + o_coords);
   ctx->s = s;
   return MDMC_OK;
 }
@@ -351,13 +355,12 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   const Scratch &s = ctx->s;
   EmitArgs ea{};
   ea.v.g = FieldDesc{d_field, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
-  ea.v.mnx = mn[0];
-  ea.v.mny = mn[1];
-  ea.v.mnz = mn[2];
   // xyz_range = xyz_max - xyz_min in float (dmc.cu:2149)
-  ea.v.rgx = mx[0] - mn[0];
-  ea.v.rgy = mx[1] - mn[1];
-  ea.v.rgz = mx[2] - mn[2];
+  CoordArgs co{s.coords, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, mn[0], mn[1], mn[2],
+               mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+  ea.v.xt_c = s.coords;
+  ea.v.yt_c = s.coords + p.X;
+  ea.v.zt_c = s.coords + p.X + p.Y;
   ea.v.XT = p.XT;
   ea.cap = p.cap;
   ea.nsegs = (uint32_t)p.nsegs;
@@ -376,6 +379,8 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
   {
     ProfScope ps(ctx, 2, st);
+    hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0, st,
+                       co);
     // one wave per chunk of 64 active cells, at most 8 workgroups per CU
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * 8));
@@ -501,7 +506,7 @@ int fill_batch_table(mdmc_ctx *ctx, int slot, const mdmc_volume *vols, uint32_t 
                      uint32_t W4, uint32_t KP, const Scratch &S, uint64_t *nsegs_total, uint64_t *cells_total,
                      uint32_t *tiles_total) {
   const uint32_t cap = 256u * W4;
-  uint64_t seg_off = 0, tile_off = 0, cells = 0;
+  uint64_t seg_off = 0, tile_off = 0, cells = 0, coord_off = 0;
   for (uint32_t v = 0; v < n; ++v) {
     const mdmc_volume &mv = vols[v];
     BatchVol &b = ctx->bhost[slot][v];
@@ -535,6 +540,8 @@ int fill_batch_table(mdmc_ctx *ctx, int slot, const mdmc_volume *vols, uint32_t 
     b.quads = mv.quads;
     b.qcap = mv.quads ? mv.quad_capacity : 0;
     b.counts = reinterpret_cast<uint32_t *>(d_counts + v);
+    b.coords = empty ? nullptr : S.coords + coord_off;
+    if (!empty) coord_off += (uint64_t)X + Y + Z;
     seg_off += nsegs;
     tile_off += (uint64_t)XT * NRB * NKP;
     cells += (uint64_t)cx * cy * cz;
@@ -561,7 +568,7 @@ int batch_layout(const mdmc_volume *vols, uint32_t n, bool check_fields, Plan *a
   }
   const uint32_t KP = pt_planes(W4);
   const uint32_t cap = 256u * W4;
-  uint64_t nseg = 0, tiles = 0, cells = 0;
+  uint64_t nseg = 0, tiles = 0, cells = 0, ncoord = 0;
   for (uint32_t v = 0; v < n; ++v) {
     const mdmc_volume &mv = vols[v];
     if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
@@ -569,6 +576,7 @@ int batch_layout(const mdmc_volume *vols, uint32_t n, bool check_fields, Plan *a
     nseg += XT * (mv.dims[1] - 1) * (uint64_t)(mv.dims[2] - 1);
     tiles += XT * ((mv.dims[1] - 1 + kBatchRows - 1) / kBatchRows) * ((mv.dims[2] - 1 + KP - 1) / KP);
     cells += (uint64_t)(mv.dims[0] - 1) * (mv.dims[1] - 1) * (mv.dims[2] - 1);
+    ncoord += (uint64_t)mv.dims[0] + mv.dims[1] + mv.dims[2];
   }
   if (nseg >= 0x7FFFFFFFull || tiles >= 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "batch has too many cell rows");
   *agg = Plan{};
@@ -578,6 +586,7 @@ int batch_layout(const mdmc_volume *vols, uint32_t n, bool check_fields, Plan *a
   agg->ntiles = (uint32_t)(nseg / kScanTile + 1);
   agg->max_chunks = (cells + 63) / 64 + 1;
   agg->classify_blocks = tiles;
+  agg->ncoord = ncoord;
   agg->empty = nseg == 0;
   *W4_out = W4;
   *KP_out = KP;
@@ -713,6 +722,8 @@ int batch_emit(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *
     const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
     const uint64_t want = (agg.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
+    hipLaunchKernelGGL(k_coords_batch, dim3(std::min<uint32_t>(n, (uint32_t)ctx->num_cus * 4)), dim3(256), 0, st,
+                       static_cast<const BatchVol *>(batch_table_dev(ctx)), n);
     hipLaunchKernelGGL(k_emit_batch, dim3(blocks), dim3(kEmitThreads), 0, st, be);
   }
   HIP_TRY(hipGetLastError());
@@ -1087,8 +1098,11 @@ int mdmc_field_blobs_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3],
   BlobArgs ba{d_out, X, Y, Z, nblobs, d, d + (size_t)nblobs * X, d + (size_t)nblobs * (X + Y),
               d + (size_t)nblobs * (X + Y + Z)};
   const uint64_t n = (uint64_t)X * Y * Z;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
-  hipLaunchKernelGGL(k_field_blobs, dim3(blocks), dim3(256), 0, st, ba);
+  const uint64_t blocks = nblobs <= kBlobRegs
+                              ? (uint64_t)((X + 255) / 256) * (((uint64_t)Y * Z + kBlobRows - 1) / kBlobRows)
+                              : std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
+  if (blocks > 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "field too large for the generator grid");
+  hipLaunchKernelGGL(k_field_blobs, dim3((uint32_t)blocks), dim3(256), 0, st, ba);
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
 }
@@ -1179,7 +1193,10 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
     G.seed = g.seed;
     const uint64_t npts = (uint64_t)X * Y * Z;
     G.block_off = blocks;
-    G.nblocks = npts ? (uint32_t)std::min<uint64_t>((npts + 255) / 256, 4096) : 0;
+    if (g.kind == MDMC_FIELD_BLOBS && g.nblobs <= kBlobRegs)
+      G.nblocks = npts ? (uint32_t)(((uint64_t)X + 255) / 256 * (((uint64_t)Y * Z + kBlobRows - 1) / kBlobRows)) : 0;
+    else
+      G.nblocks = npts ? (uint32_t)std::min<uint64_t>((npts + 255) / 256, 4096) : 0;
     blocks += G.nblocks;
     double *h = ht + off;
     const double *d = dt + off;

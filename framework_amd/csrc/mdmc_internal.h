@@ -16,6 +16,7 @@ struct Plan {
   uint32_t X, Y, Z, cx, cy, cz;
   uint32_t W, R, Kc, cap, XT, NRB, NKC, ntiles;
   uint64_t nsegs, nslots, max_chunks, classify_blocks;
+  uint64_t ncoord;  // point coordinates kept for the emit: X + Y + Z (summed over a batch)
   int empty;
   int vec4;    // rows are a multiple of 4 points
   int zmarch;  // use the z-marching classify kernel
@@ -34,6 +35,7 @@ struct Scratch {
   Totals *totals;
   uint64_t *slots;
   uint4 *gmap;
+  float *coords;
 };
 
 int plan_tiling(const uint32_t dims[3], Plan *p);

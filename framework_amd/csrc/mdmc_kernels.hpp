@@ -648,6 +648,7 @@ struct BatchVol {
   uint64_t qcap;
   uint32_t *counts;       // the volume's mdmc_counts
   uint32_t vbase, qbase;  // written by k_batch_counts: prefix at seg_off
+  float *coords;          // X + Y + Z point coordinates (filled before the emit)
 };
 
 struct BatchArgs {
@@ -889,7 +890,7 @@ constexpr int kEmitThreads = 256;
 // Per-volume part of the emit arguments.
 struct EmitVol {
   FieldDesc g;
-  float mnx, mny, mnz, rgx, rgy, rgz;  // xyz_min and xyz_max - xyz_min
+  const float *xt_c, *yt_c, *zt_c;     // coordinate of every point index per axis
   uint32_t XT, seg_off;                // x-tiles; first segment of the volume
   uint32_t vbase, qbase;               // global vertex / quad number of its first
   float *verts;
@@ -918,6 +919,29 @@ __device__ __forceinline__ float coordf(uint32_t i, uint32_t n, float range, flo
   return (float)i / (float)n * range + mn;
 }
 
+// Coordinates of the point indices of a volume, per axis (X, then Y, then Z
+// entries): the same three float operations per entry as the reference.
+struct CoordArgs {
+  float *out;
+  uint32_t X, Y, Z, cx, cy, cz;
+  float mnx, mny, mnz, rgx, rgy, rgz;
+};
+
+__device__ __forceinline__ void coords_fill(const CoordArgs &c, uint32_t i0, uint32_t stride) {
+  const uint32_t n = c.X + c.Y + c.Z;
+  for (uint32_t i = i0; i < n; i += stride) {
+    float v;
+    if (i < c.X) v = coordf(i, c.cx, c.rgx, c.mnx);
+    else if (i < c.X + c.Y) v = coordf(i - c.X, c.cy, c.rgy, c.mny);
+    else v = coordf(i - c.X - c.Y, c.cz, c.rgz, c.mnz);
+    c.out[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_coords(CoordArgs c) {
+  coords_fill(c, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // Rank of the active cell at x offset xl in segment s, or -1 if the group
 // map says it is not active.
 __device__ __forceinline__ int32_t rank_in_segment(const EmitArgs &a, uint32_t s, uint32_t xl) {
@@ -931,7 +955,7 @@ __device__ __forceinline__ int32_t rank_in_segment(const EmitArgs &a, uint32_t s
 // One active cell (segment s, rank within it) of volume vol: its vertices and
 // the quads of its owned edges.
 __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
-                                          const uint64_t *s_lab) {
+                                          const uint64_t *s_lab, float *s_acc) {
   const FieldDesc &g = vol.g;
   const float iso = g.iso;
   const size_t sy = g.X, sz = (size_t)g.X * g.Y;
@@ -958,9 +982,9 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   v[6] = p[sz + sy + 1];
   v[7] = p[sz + sy];
 
-  const float X0 = coordf(x, g.cx, vol.rgx, vol.mnx), X1 = coordf(x + 1, g.cx, vol.rgx, vol.mnx);
-  const float Y0 = coordf(j, g.cy, vol.rgy, vol.mny), Y1 = coordf(j + 1, g.cy, vol.rgy, vol.mny);
-  const float Z0 = coordf(k, g.cz, vol.rgz, vol.mnz), Z1 = coordf(k + 1, g.cz, vol.rgz, vol.mnz);
+  const float X0 = vol.xt_c[x], X1 = vol.xt_c[x + 1];
+  const float Y0 = vol.yt_c[j], Y1 = vol.yt_c[j + 1];
+  const float Z0 = vol.zt_c[k], Z1 = vol.zt_c[k + 1];
 
   // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
   // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
@@ -974,8 +998,22 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   const uint32_t have = (x > 0 ? 1u : 0u) | (j > 0 ? 2u : 0u) | (k > 0 ? 4u : 0u);
 
   const uint64_t labels = s_lab[lut2 * 256 + cfg];
-  float ax[4] = {0, 0, 0, 0}, ay[4] = {0, 0, 0, 0}, az[4] = {0, 0, 0, 0};
-  uint32_t cnt[4] = {0, 0, 0, 0};
+  const uint16_t info = c_cfg_info[cfg];
+  const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
+  // Group sums live in LDS, component c of group q at s_acc[(3q + c) * T]
+  // (T = threads per block; the caller offsets s_acc by the thread index).
+  // They start at -0.0, so the first addition yields the point itself
+  // bitwise, as the reference's "assign the first, add the rest" does.
+  constexpr uint32_t T = kEmitThreads;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((uint32_t)q < niso) {
+      s_acc[(3 * q + 0) * T] = -0.0f;
+      s_acc[(3 * q + 1) * T] = -0.0f;
+      s_acc[(3 * q + 2) * T] = -0.0f;
+    }
+  }
+  uint32_t cnt = 0;  // 8 bits per group
   float ex[3] = {0, 0, 0}, ey[3] = {0, 0, 0}, ez[3] = {0, 0, 0};
 #pragma unroll
   for (int e = 0; e < 12; ++e) {
@@ -998,17 +1036,12 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
     if (e == 9) { ex[1] = px; ey[1] = py; ez[1] = pz; }
     if (e == 10) { ex[2] = px; ey[2] = py; ez[2] = pz; }
     if ((NEED[e] & have) != NEED[e]) continue;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (lab == (uint32_t)q) {
-        if (cnt[q] == 0) { ax[q] = px; ay[q] = py; az[q] = pz; }
-        else { ax[q] = ax[q] + px; ay[q] = ay[q] + py; az[q] = az[q] + pz; }
-        ++cnt[q];
-      }
-    }
+    float *acc = s_acc + 3 * lab * T;
+    acc[0] = acc[0] + px;
+    acc[T] = acc[T] + py;
+    acc[2 * T] = acc[2 * T] + pz;
+    cnt += 1u << (8 * lab);
   }
-  const uint16_t info = c_cfg_info[cfg];
-  const uint32_t niso = lut2 ? ((info >> 3) & 7u) : (info & 7u);
   const uint32_t b6 = ((cfg >> 2) ^ (cfg >> 6)) & 1u;
   const uint32_t b9 = ((cfg >> 5) ^ (cfg >> 6)) & 1u;
   const uint32_t b10 = ((cfg >> 7) ^ (cfg >> 6)) & 1u;
@@ -1019,10 +1052,11 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if ((uint32_t)q < niso) {
-        const float d = (float)cnt[q];
-        out[3 * q + 0] = cnt[q] ? ax[q] / d : qnan;
-        out[3 * q + 1] = cnt[q] ? ay[q] / d : qnan;
-        out[3 * q + 2] = cnt[q] ? az[q] / d : qnan;
+        const uint32_t n = (cnt >> (8 * q)) & 0xFFu;
+        const float d = (float)n;
+        out[3 * q + 0] = n ? s_acc[(3 * q + 0) * T] / d : qnan;
+        out[3 * q + 1] = n ? s_acc[(3 * q + 1) * T] / d : qnan;
+        out[3 * q + 2] = n ? s_acc[(3 * q + 2) * T] / d : qnan;
       }
     }
     uint32_t wv = niso;
@@ -1134,6 +1168,7 @@ __device__ __forceinline__ bool locate_cell(const EmitArgs &a, const Totals &tot
 
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
+  __shared__ float s_acc[12 * kEmitThreads];
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   __syncthreads();
   const uint32_t lane = lane_id(), wave = wave_id();
@@ -1146,7 +1181,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
        ch += gridDim.x * (kEmitThreads / kWave)) {
     uint32_t s, rank;
     if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
-    emit_cell(a, a.v, s, rank, s_lab);
+    emit_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
   }
 }
 
@@ -1188,6 +1223,17 @@ __global__ __launch_bounds__(256) void k_batch_counts(BatchCountArgs b) {
   }
 }
 
+// One workgroup per volume: its coordinate tables.
+__global__ __launch_bounds__(256) void k_coords_batch(const BatchVol *vols, uint32_t nvol) {
+  for (uint32_t v = blockIdx.x; v < nvol; v += gridDim.x) {
+    const BatchVol &bv = vols[v];
+    if (bv.nsegs == 0) continue;
+    CoordArgs c{bv.coords, bv.c.g.X, bv.c.g.Y, bv.c.g.Z, bv.c.g.cx, bv.c.g.cy, bv.c.g.cz,
+                bv.mnx, bv.mny, bv.mnz, bv.rgx, bv.rgy, bv.rgz};
+    coords_fill(c, threadIdx.x, blockDim.x);
+  }
+}
+
 struct BatchEmitArgs {
   EmitArgs a;  // arrays of the whole batch; a.v.status is the batch status word
   const BatchVol *vols;
@@ -1199,6 +1245,7 @@ struct BatchEmitArgs {
 // binary search over the volumes' first segments.
 __global__ __launch_bounds__(kEmitThreads) void k_emit_batch(BatchEmitArgs b) {
   __shared__ uint64_t s_lab[2 * 256];
+  __shared__ float s_acc[12 * kEmitThreads];
   __shared__ uint32_t s_soff[kMaxBatch + 1];
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   for (uint32_t i = threadIdx.x; i <= b.nvol; i += kEmitThreads)
@@ -1222,8 +1269,9 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_batch(BatchEmitArgs b) {
     const BatchVol &bv = b.vols[lo];
     EmitVol e;
     e.g = bv.c.g;
-    e.mnx = bv.mnx; e.mny = bv.mny; e.mnz = bv.mnz;
-    e.rgx = bv.rgx; e.rgy = bv.rgy; e.rgz = bv.rgz;
+    e.xt_c = bv.coords;
+    e.yt_c = bv.coords + bv.c.g.X;
+    e.zt_c = bv.coords + bv.c.g.X + bv.c.g.Y;
     e.XT = bv.c.XT;
     e.seg_off = bv.seg_off;
     const uint2 bs = b.base[lo];
@@ -1234,7 +1282,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_batch(BatchEmitArgs b) {
     e.quads = bv.quads;
     e.qcap = bv.quads ? bv.qcap : 0;
     e.status = bv.counts ? bv.counts + 3 : a.v.status;
-    emit_cell(a, e, s, rank, s_lab);
+    emit_cell(a, e, s, rank, s_lab, s_acc + threadIdx.x);
   }
 }
 
@@ -1286,10 +1334,36 @@ struct BlobArgs {
   const double *ex, *ey, *ez, *amp;  // [nblobs][X], [nblobs][Y], [nblobs][Z], [nblobs]
 };
 
-__global__ __launch_bounds__(256) void k_field_blobs(BlobArgs a) {
+// Blob sums, column-major: a thread owns one column x and keeps
+// amp[b] * ex[b][x] for every blob in registers; for each row (y, z) of its
+// workgroup's row range it adds ((amp*ex) * ey) * ez blob by blob, the same
+// operations in the same order as the CPU generator.  Rows are uniform per
+// workgroup, so ey and ez come through scalar loads.
+constexpr uint32_t kBlobRegs = 32;  // blobs kept in registers (more: per-point path)
+constexpr uint32_t kBlobRows = 16;  // rows per workgroup
+
+__device__ __forceinline__ void blob_rows(const BlobArgs &a, uint32_t x0, uint32_t r0) {
+  const uint32_t x = x0 + threadIdx.x;
+  const uint32_t xc = x < a.X ? x : a.X - 1u;
+  double ax[kBlobRegs];
+#pragma unroll
+  for (uint32_t b = 0; b < kBlobRegs; ++b) ax[b] = b < a.nblobs ? a.amp[b] * a.ex[(size_t)b * a.X + xc] : 0.0;
+  const uint32_t nrows = a.Y * a.Z;
+  const uint32_t r1 = min(r0 + kBlobRows, nrows);
+  for (uint32_t r = r0; r < r1; ++r) {
+    const uint32_t y = r % a.Y, z = r / a.Y;
+    double acc = 0.0;
+#pragma unroll
+    for (uint32_t b = 0; b < kBlobRegs; ++b)
+      if (b < a.nblobs) acc = acc + (ax[b] * a.ey[(size_t)b * a.Y + y]) * a.ez[(size_t)b * a.Z + z];
+    if (x < a.X) a.out[(size_t)r * a.X + x] = (float)acc;
+  }
+}
+
+// Per-point path (any number of blobs).
+__device__ __forceinline__ void blob_points(const BlobArgs &a, size_t i0, size_t stride) {
   const size_t n = (size_t)a.X * a.Y * a.Z;
-  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
-       id += (size_t)gridDim.x * blockDim.x) {
+  for (size_t id = i0; id < n; id += stride) {
     const uint32_t x = (uint32_t)(id % a.X);
     const size_t r = id / a.X;
     const uint32_t y = (uint32_t)(r % a.Y), z = (uint32_t)(r / a.Y);
@@ -1298,6 +1372,17 @@ __global__ __launch_bounds__(256) void k_field_blobs(BlobArgs a) {
       acc = acc + ((a.amp[b] * a.ex[(size_t)b * a.X + x]) * a.ey[(size_t)b * a.Y + y]) *
                       a.ez[(size_t)b * a.Z + z];
     a.out[id] = (float)acc;
+  }
+}
+
+// Grid: ceil(X / 256) column blocks x ceil(Y * Z / kBlobRows) row blocks when
+// nblobs <= kBlobRegs, else any grid over points.
+__global__ __launch_bounds__(256) void k_field_blobs(BlobArgs a) {
+  if (a.nblobs <= kBlobRegs) {
+    const uint32_t xcb = (a.X + 255u) / 256u;
+    blob_rows(a, (blockIdx.x % xcb) * 256u, (blockIdx.x / xcb) * kBlobRows);
+  } else {
+    blob_points(a, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
   }
 }
 
@@ -1348,24 +1433,26 @@ __global__ __launch_bounds__(256) void k_field_batch(const GenVol *vols, uint32_
     if (vols[mid].block_off <= blockIdx.x) lo = mid; else hi = mid;
   }
   const GenVol &v = vols[__builtin_amdgcn_readfirstlane(lo)];
+  const uint32_t local = blockIdx.x - v.block_off;
+  if (v.kind == 1) {
+    const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, v.t[0], v.t[1], v.t[2], v.t[3]};
+    if (v.nblobs <= kBlobRegs) {
+      const uint32_t xcb = (v.X + 255u) / 256u;
+      blob_rows(ba, (local % xcb) * 256u, (local / xcb) * kBlobRows);
+    } else {
+      blob_points(ba, (size_t)local * blockDim.x + threadIdx.x, (size_t)v.nblocks * blockDim.x);
+    }
+    return;
+  }
   const size_t n = (size_t)v.X * v.Y * v.Z;
-  for (size_t id = (size_t)(blockIdx.x - v.block_off) * blockDim.x + threadIdx.x; id < n;
-       id += (size_t)v.nblocks * blockDim.x) {
+  for (size_t id = (size_t)local * blockDim.x + threadIdx.x; id < n; id += (size_t)v.nblocks * blockDim.x) {
     const uint32_t x = (uint32_t)(id % v.X);
     const size_t r = id / v.X;
     const uint32_t y = (uint32_t)(r % v.Y), z = (uint32_t)(r / v.Y);
-    if (v.kind == 1) {
-      double acc = 0.0;
-      for (uint32_t b = 0; b < v.nblobs; ++b)
-        acc = acc + ((v.t[3][b] * v.t[0][(size_t)b * v.X + x]) * v.t[1][(size_t)b * v.Y + y]) *
-                        v.t[2][(size_t)b * v.Z + z];
-      v.out[id] = (float)acc;
-    } else {
-      const double gval = v.t[0][x] * v.t[3][y] + v.t[2][y] * v.t[5][z] + v.t[4][z] * v.t[1][x];
-      const uint64_t h = mix64(v.seed + ((uint64_t)id + 1) * 0x9E3779B97F4A7C15ull);
-      const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
-      v.out[id] = (float)(gval + v.amp * (2.0 * u - 1.0));
-    }
+    const double gval = v.t[0][x] * v.t[3][y] + v.t[2][y] * v.t[5][z] + v.t[4][z] * v.t[1][x];
+    const uint64_t h = mix64(v.seed + ((uint64_t)id + 1) * 0x9E3779B97F4A7C15ull);
+    const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+    v.out[id] = (float)(gval + v.amp * (2.0 * u - 1.0));
   }
 }
 
