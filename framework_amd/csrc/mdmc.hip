@@ -152,9 +152,6 @@ struct mdmc_ctx {
   bool counted = false;
   const float *counted_field = nullptr;
   float iso = 0.f;
-  // generator tables
-  void *gen = nullptr;
-  size_t gen_bytes = 0;
   // batches: device table of BatchVol + bases + the batch's own counts, and
   // two pinned staging tables (count, emit) uploaded with async copies
   void *bdev = nullptr;
@@ -839,7 +836,6 @@ void mdmc_ctx_destroy(mdmc_ctx *ctx) {
   if (!ctx) return;
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->scratch && !ctx->scratch_external) (void)hipFree(ctx->scratch);
-  if (ctx->gen) (void)hipFree(ctx->gen);
   if (ctx->bdev) (void)hipFree(ctx->bdev);
   if (ctx->gdev) (void)hipFree(ctx->gdev);
   if (ctx->ghost) (void)hipHostFree(ctx->ghost);
@@ -1060,76 +1056,36 @@ void mdmc_mesh_free(mdmc_mesh *mesh) {
 
 // ---- synthetic fields ------------------------------------------------------
 
-static int ensure_gen(mdmc_ctx *ctx, size_t bytes) {
-  if (bytes <= ctx->gen_bytes) return MDMC_OK;
-  if (ctx->gen) (void)hipFree(ctx->gen);
-  ctx->gen = nullptr;
-  ctx->gen_bytes = 0;
-  if (hipMalloc(&ctx->gen, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(MDMC_ERR_NOMEM, "generator table allocation failed");
-  }
-  ctx->gen_bytes = bytes;
-  return MDMC_OK;
-}
-
 int mdmc_field_blobs_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], uint64_t seed,
                             uint32_t nblobs, double sigma_lo, double sigma_hi, double amp_lo,
                             double amp_hi, void *stream) {
   if (!ctx || !d_out || !dims) return fail(MDMC_ERR_INVALID, "null argument");
-  const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
-  if (!X || !Y || !Z) return MDMC_OK;
-  std::vector<double> h((size_t)nblobs * (X + Y + Z + 1));
-  double *ex = h.data(), *ey = ex + (size_t)nblobs * X, *ez = ey + (size_t)nblobs * Y,
-         *amp = ez + (size_t)nblobs * Z;
-  for (uint32_t b = 0; b < nblobs; ++b) {
-    mdmc_blob bp = mdmc_blob_params(seed, b, dims, sigma_lo, sigma_hi, amp_lo, amp_hi);
-    mdmc_gauss_1d(ex + (size_t)b * X, X, bp.cx, bp.sigma);
-    mdmc_gauss_1d(ey + (size_t)b * Y, Y, bp.cy, bp.sigma);
-    mdmc_gauss_1d(ez + (size_t)b * Z, Z, bp.cz, bp.sigma);
-    amp[b] = bp.amp;
-  }
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipStreamSynchronize(st));  // the table buffer may still be in use
-  int rc = ensure_gen(ctx, std::max<size_t>(8, h.size() * sizeof(double)));
-  if (rc != MDMC_OK) return rc;
-  HIP_TRY(hipMemcpy(ctx->gen, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-  double *d = static_cast<double *>(ctx->gen);
-  BlobArgs ba{d_out, X, Y, Z, nblobs, d, d + (size_t)nblobs * X, d + (size_t)nblobs * (X + Y),
-              d + (size_t)nblobs * (X + Y + Z)};
-  const uint64_t n = (uint64_t)X * Y * Z;
-  const uint64_t blocks = nblobs <= kBlobRegs
-                              ? (uint64_t)((X + 255) / 256) * (((uint64_t)Y * Z + kBlobRows - 1) / kBlobRows)
-                              : std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
-  if (blocks > 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "field too large for the generator grid");
-  hipLaunchKernelGGL(k_field_blobs, dim3((uint32_t)blocks), dim3(256), 0, st, ba);
-  HIP_TRY(hipGetLastError());
-  return MDMC_OK;
+  if (!dims[0] || !dims[1] || !dims[2]) return MDMC_OK;
+  mdmc_gen_spec g{};
+  g.out = d_out;
+  std::memcpy(g.dims, dims, sizeof(g.dims));
+  g.kind = MDMC_FIELD_BLOBS;
+  g.seed = seed;
+  g.nblobs = nblobs;
+  g.sigma_lo = sigma_lo;
+  g.sigma_hi = sigma_hi;
+  g.amp_lo = amp_lo;
+  g.amp_hi = amp_hi;
+  return mdmc_batch_generate_device(ctx, &g, 1, stream);
 }
 
 int mdmc_field_gyroid_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], double period,
                              uint64_t seed, double noise_amp, void *stream) {
   if (!ctx || !d_out || !dims) return fail(MDMC_ERR_INVALID, "null argument");
-  const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
-  if (!X || !Y || !Z) return MDMC_OK;
-  std::vector<double> h(2 * ((size_t)X + Y + Z));
-  double *sx = h.data(), *cx = sx + X, *sy = cx + X, *cy = sy + Y, *sz = cy + Y, *cz = sz + Z;
-  mdmc_trig_1d(sx, cx, X, period);
-  mdmc_trig_1d(sy, cy, Y, period);
-  mdmc_trig_1d(sz, cz, Z, period);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipStreamSynchronize(st));
-  int rc = ensure_gen(ctx, h.size() * sizeof(double));
-  if (rc != MDMC_OK) return rc;
-  HIP_TRY(hipMemcpy(ctx->gen, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-  double *d = static_cast<double *>(ctx->gen);
-  GyroidArgs ga{d_out, X, Y, Z, d, d + X, d + 2 * X, d + 2 * X + Y, d + 2 * X + 2 * Y,
-                d + 2 * X + 2 * Y + Z, seed, noise_amp};
-  const uint64_t n = (uint64_t)X * Y * Z;
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, (uint64_t)ctx->num_cus * 16);
-  hipLaunchKernelGGL(k_field_gyroid, dim3(blocks), dim3(256), 0, st, ga);
-  HIP_TRY(hipGetLastError());
-  return MDMC_OK;
+  if (!dims[0] || !dims[1] || !dims[2]) return MDMC_OK;
+  mdmc_gen_spec g{};
+  g.out = d_out;
+  std::memcpy(g.dims, dims, sizeof(g.dims));
+  g.kind = MDMC_FIELD_GYROID;
+  g.seed = seed;
+  g.period = period;
+  g.noise_amp = noise_amp;
+  return mdmc_batch_generate_device(ctx, &g, 1, stream);
 }
 
 int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32_t n, void *stream) {
@@ -1141,7 +1097,7 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
   for (uint32_t v = 0; v < n; ++v) {
     const mdmc_gen_spec &g = specs[v];
     if (g.kind == MDMC_FIELD_BLOBS)
-      ndoubles += (size_t)g.nblobs * ((size_t)g.dims[0] + g.dims[1] + g.dims[2] + 1);
+      ndoubles += (size_t)blob_ldb(g.nblobs) * ((size_t)g.dims[0] + g.dims[1] + g.dims[2] + 1);
     else if (g.kind == MDMC_FIELD_GYROID)
       ndoubles += 2 * ((size_t)g.dims[0] + g.dims[1] + g.dims[2]);
     else
@@ -1193,29 +1149,38 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
     G.seed = g.seed;
     const uint64_t npts = (uint64_t)X * Y * Z;
     G.block_off = blocks;
-    if (g.kind == MDMC_FIELD_BLOBS && g.nblobs <= kBlobRegs)
-      G.nblocks = npts ? (uint32_t)(((uint64_t)X + 255) / 256 * (((uint64_t)Y * Z + kBlobRows - 1) / kBlobRows)) : 0;
-    else
-      G.nblocks = npts ? (uint32_t)std::min<uint64_t>((npts + 255) / 256, 4096) : 0;
+    uint64_t nb64 = 0;
+    if (npts && g.kind == MDMC_FIELD_BLOBS && g.nblobs > kBlobRegs)
+      nb64 = std::min<uint64_t>((npts + 255) / 256, 4096);
+    else if (npts)
+      nb64 = (gen_tasks(X, Y, Z) + kGenWaves - 1) / kGenWaves;
+    if (nb64 + blocks > 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "fields too large for the generator grid");
+    G.nblocks = (uint32_t)nb64;
     blocks += G.nblocks;
     double *h = ht + off;
     const double *d = dt + off;
     if (g.kind == MDMC_FIELD_BLOBS) {
-      const uint32_t nb = g.nblobs;
+      // ex [ldb][X]; ey [Y][ldb] and ez [Z][ldb] transposed; amp [ldb]; zero padded
+      const uint32_t nb = g.nblobs, ldb = blob_ldb(nb);
       G.nblobs = nb;
-      double *ex = h, *ey = ex + (size_t)nb * X, *ez = ey + (size_t)nb * Y, *amp = ez + (size_t)nb * Z;
+      const size_t tab = (size_t)ldb * ((size_t)X + Y + Z + 1);
+      double *ex = h, *ey = ex + (size_t)ldb * X, *ez = ey + (size_t)ldb * Y, *amp = ez + (size_t)ldb * Z;
+      std::memset(h, 0, sizeof(double) * tab);
+      std::vector<double> line(std::max(Y, Z));
       for (uint32_t b = 0; b < nb; ++b) {
         mdmc_blob bp = mdmc_blob_params(g.seed, b, g.dims, g.sigma_lo, g.sigma_hi, g.amp_lo, g.amp_hi);
         mdmc_gauss_1d(ex + (size_t)b * X, X, bp.cx, bp.sigma);
-        mdmc_gauss_1d(ey + (size_t)b * Y, Y, bp.cy, bp.sigma);
-        mdmc_gauss_1d(ez + (size_t)b * Z, Z, bp.cz, bp.sigma);
+        mdmc_gauss_1d(line.data(), Y, bp.cy, bp.sigma);
+        for (uint32_t y = 0; y < Y; ++y) ey[(size_t)y * ldb + b] = line[y];
+        mdmc_gauss_1d(line.data(), Z, bp.cz, bp.sigma);
+        for (uint32_t z = 0; z < Z; ++z) ez[(size_t)z * ldb + b] = line[z];
         amp[b] = bp.amp;
       }
       G.t[0] = d;
-      G.t[1] = d + (size_t)nb * X;
-      G.t[2] = d + (size_t)nb * (X + Y);
-      G.t[3] = d + (size_t)nb * (X + Y + Z);
-      off += (size_t)nb * ((size_t)X + Y + Z + 1);
+      G.t[1] = d + (size_t)ldb * X;
+      G.t[2] = d + (size_t)ldb * ((size_t)X + Y);
+      G.t[3] = d + (size_t)ldb * ((size_t)X + Y + Z);
+      off += tab;
     } else {
       double *sx = h, *cx = sx + X, *sy = cx + X, *cy = sy + Y, *sz = cy + Y, *cz = sz + Z;
       mdmc_trig_1d(sx, cx, X, g.period);
@@ -1233,12 +1198,14 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
   }
   if (blocks == 0) return MDMC_OK;
   HIP_TRY(hipMemcpyAsync(ctx->gdev, ctx->ghost, bytes, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_field_batch, dim3(blocks), dim3(kGenThreads), 0, st, static_cast<const GenVol *>(ctx->gdev),
+                     n);
+  HIP_TRY(hipGetLastError());
+  // the next call rewrites both table copies once this kernel has read them
   if (cs == hipStreamCaptureStatusNone) {
     HIP_TRY(hipEventRecord(ctx->ghost_ev, st));
     ctx->ghost_ev_used = true;
   }
-  hipLaunchKernelGGL(k_field_batch, dim3(blocks), dim3(256), 0, st, static_cast<const GenVol *>(ctx->gdev), n);
-  HIP_TRY(hipGetLastError());
   return MDMC_OK;
 }
 

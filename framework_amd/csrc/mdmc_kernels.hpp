@@ -1328,35 +1328,108 @@ __global__ __launch_bounds__(256) void k_cells(CellsArgs a) {
 // ---------------------------------------------------------------------------
 // Synthetic fields (definitions in mdmc_fields.h).
 
+// Blob tables, ldb = blob_ldb(nblobs) entries per blob dimension, entries
+// past nblobs zero: ex is [ldb][X] (read per lane), ey and ez are transposed,
+// [Y][ldb] and [Z][ldb], so that the factors of one row are contiguous and
+// come in through a few wide scalar loads; amp has ldb entries.
 struct BlobArgs {
   float *out;
-  uint32_t X, Y, Z, nblobs;
-  const double *ex, *ey, *ez, *amp;  // [nblobs][X], [nblobs][Y], [nblobs][Z], [nblobs]
+  uint32_t X, Y, Z, nblobs, ldb;
+  const double *ex, *ey, *ez, *amp;
 };
 
-// Blob sums, column-major: a thread owns one column x and keeps
-// amp[b] * ex[b][x] for every blob in registers; for each row (y, z) of its
-// workgroup's row range it adds ((amp*ex) * ey) * ez blob by blob, the same
-// operations in the same order as the CPU generator.  Rows are uniform per
-// workgroup, so ey and ez come through scalar loads.
+// Generators work in wave tasks: 64 consecutive columns x of kGenRows
+// consecutive rows r = y + Y*z.  A lane owns one column; the row loop is
+// uniform over the wave, so per-row values are scalar loads and the stores
+// of a row are one contiguous 256-byte run.
+constexpr uint32_t kGenRows = 16;
+constexpr uint32_t kGenThreads = 256;
+constexpr uint32_t kGenWaves = kGenThreads / kWave;
 constexpr uint32_t kBlobRegs = 32;  // blobs kept in registers (more: per-point path)
-constexpr uint32_t kBlobRows = 16;  // rows per workgroup
 
-__device__ __forceinline__ void blob_rows(const BlobArgs &a, uint32_t x0, uint32_t r0) {
-  const uint32_t x = x0 + threadIdx.x;
-  const uint32_t xc = x < a.X ? x : a.X - 1u;
+// blob table row stride for a blob count
+__host__ __device__ inline uint32_t blob_ldb(uint32_t nblobs) {
+  return nblobs <= kBlobRegs ? kBlobRegs : nblobs;
+}
+
+__host__ __device__ inline uint64_t gen_tasks(uint32_t X, uint32_t Y, uint32_t Z) {
+  return (uint64_t)((X + 63u) / 64u) * (((uint64_t)Y * Z + kGenRows - 1u) / kGenRows);
+}
+
+// Wave-uniform reads through the constant address space compile to scalar
+// loads (the tables are not written while a generator runs).
+typedef const __attribute__((address_space(4))) double *uniform_dptr;
+__device__ __forceinline__ uniform_dptr as_uniform(const double *p) { return (uniform_dptr)p; }
+
+// First row of a task as (y0, z0); row r0 + i is (y0 + i, z0) carried into z.
+struct GenTask {
+  uint32_t x, nr, y0, z0;
+  uint64_t r0;
+};
+
+__device__ __forceinline__ GenTask gen_task(uint32_t X, uint32_t Y, uint32_t Z, uint64_t task) {
+  const uint32_t ncc = (X + 63u) / 64u;
+  GenTask t;
+  t.x = (uint32_t)(task % ncc) * 64u + (threadIdx.x & 63u);
+  t.r0 = (task / ncc) * kGenRows;
+  const uint64_t nrows = (uint64_t)Y * Z;
+  t.nr = t.r0 < nrows ? (uint32_t)min<uint64_t>(kGenRows, nrows - t.r0) : 0u;
+  t.y0 = (uint32_t)(t.r0 % Y);
+  t.z0 = (uint32_t)(t.r0 / Y);
+  return t;
+}
+
+__device__ __forceinline__ void gen_row_yz(const GenTask &t, uint32_t Y, uint32_t i, uint32_t &y, uint32_t &z) {
+  y = t.y0 + i;
+  z = t.z0;
+  while (y >= Y) {
+    y -= Y;
+    ++z;
+  }
+}
+
+// Blob sums: a lane keeps amp[b] * ex[b][x] for every blob in registers and,
+// row by row, adds ((amp*ex) * ey) * ez blob by blob, the same operations in
+// the same order as the CPU generator.  Blobs go in groups of 8 (one group
+// of row factors is one wide scalar load); the zero entries that pad the
+// last group add +0.0 to a sum that starts at +0.0 and so can never be -0.0,
+// which leaves it unchanged.  Needs ldb == kBlobRegs.
+__device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
+  const GenTask t = gen_task(a.X, a.Y, a.Z, task);
+  if (t.nr == 0) return;
+  const uint32_t xc = t.x < a.X ? t.x : a.X - 1u;
+  const uniform_dptr amp = as_uniform(a.amp);
   double ax[kBlobRegs];
 #pragma unroll
-  for (uint32_t b = 0; b < kBlobRegs; ++b) ax[b] = b < a.nblobs ? a.amp[b] * a.ex[(size_t)b * a.X + xc] : 0.0;
-  const uint32_t nrows = a.Y * a.Z;
-  const uint32_t r1 = min(r0 + kBlobRows, nrows);
-  for (uint32_t r = r0; r < r1; ++r) {
-    const uint32_t y = r % a.Y, z = r / a.Y;
+  for (uint32_t g = 0; g < kBlobRegs / 8; ++g) {
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) ax[8 * g + i] = 0.0;
+    if (8 * g < a.nblobs) {
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) ax[8 * g + i] = amp[8 * g + i] * a.ex[(size_t)(8 * g + i) * a.X + xc];
+    }
+  }
+  float *out = a.out + t.r0 * a.X + t.x;
+  for (uint32_t rr = 0; rr < t.nr; ++rr) {
+    uint32_t y, z;
+    gen_row_yz(t, a.Y, rr, y, z);
+    const uniform_dptr ey = as_uniform(a.ey + (size_t)y * kBlobRegs);
+    const uniform_dptr ez = as_uniform(a.ez + (size_t)z * kBlobRegs);
     double acc = 0.0;
 #pragma unroll
-    for (uint32_t b = 0; b < kBlobRegs; ++b)
-      if (b < a.nblobs) acc = acc + (ax[b] * a.ey[(size_t)b * a.Y + y]) * a.ez[(size_t)b * a.Z + z];
-    if (x < a.X) a.out[(size_t)r * a.X + x] = (float)acc;
+    for (uint32_t g = 0; g < kBlobRegs / 8; ++g) {
+      if (8 * g < a.nblobs) {
+        double fy[8], fz[8];
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) {
+          fy[i] = ey[8 * g + i];
+          fz[i] = ez[8 * g + i];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) acc = acc + (ax[8 * g + i] * fy[i]) * fz[i];
+      }
+    }
+    if (t.x < a.X) out[(size_t)rr * a.X] = (float)acc;
   }
 }
 
@@ -1369,18 +1442,18 @@ __device__ __forceinline__ void blob_points(const BlobArgs &a, size_t i0, size_t
     const uint32_t y = (uint32_t)(r % a.Y), z = (uint32_t)(r / a.Y);
     double acc = 0.0;
     for (uint32_t b = 0; b < a.nblobs; ++b)
-      acc = acc + ((a.amp[b] * a.ex[(size_t)b * a.X + x]) * a.ey[(size_t)b * a.Y + y]) *
-                      a.ez[(size_t)b * a.Z + z];
+      acc = acc + ((a.amp[b] * a.ex[(size_t)b * a.X + x]) * a.ey[(size_t)y * a.ldb + b]) *
+                      a.ez[(size_t)z * a.ldb + b];
     a.out[id] = (float)acc;
   }
 }
 
-// Grid: ceil(X / 256) column blocks x ceil(Y * Z / kBlobRows) row blocks when
-// nblobs <= kBlobRegs, else any grid over points.
-__global__ __launch_bounds__(256) void k_field_blobs(BlobArgs a) {
+// Grid: ceil(gen_tasks / kGenWaves) workgroups when nblobs <= kBlobRegs,
+// else any grid over points.
+__global__ __launch_bounds__(kGenThreads) void k_field_blobs(BlobArgs a) {
   if (a.nblobs <= kBlobRegs) {
-    const uint32_t xcb = (a.X + 255u) / 256u;
-    blob_rows(a, (blockIdx.x % xcb) * 256u, (blockIdx.x / xcb) * kBlobRows);
+    const uint64_t task = (uint64_t)blockIdx.x * kGenWaves + wave_id();
+    if (task < gen_tasks(a.X, a.Y, a.Z)) blob_task(a, task);
   } else {
     blob_points(a, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
   }
@@ -1400,23 +1473,33 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(256) void k_field_gyroid(GyroidArgs a) {
-  const size_t n = (size_t)a.X * a.Y * a.Z;
-  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n;
-       id += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t x = (uint32_t)(id % a.X);
-    const size_t r = id / a.X;
-    const uint32_t y = (uint32_t)(r % a.Y), z = (uint32_t)(r / a.Y);
-    const double gval = a.sx[x] * a.cy[y] + a.sy[y] * a.cz[z] + a.sz[z] * a.cx[x];
-    const uint64_t h = mix64(a.seed + ((uint64_t)id + 1) * 0x9E3779B97F4A7C15ull);
+// Gyroid plus hash noise of point id = x + X * r:
+// sx*cy + sy*cz + sz*cx from the 1-D tables, in that order, then the noise.
+__device__ __forceinline__ void gyroid_task(const GyroidArgs &a, uint64_t task) {
+  const GenTask t = gen_task(a.X, a.Y, a.Z, task);
+  if (t.x >= a.X) return;
+  const double sxx = a.sx[t.x], cxx = a.cx[t.x];
+  const uniform_dptr sy = as_uniform(a.sy), cy = as_uniform(a.cy), sz = as_uniform(a.sz), cz = as_uniform(a.cz);
+  for (uint32_t rr = 0; rr < t.nr; ++rr) {
+    uint32_t y, z;
+    gen_row_yz(t, a.Y, rr, y, z);
+    const double gval = sxx * cy[y] + sy[y] * cz[z] + sz[z] * cxx;
+    const uint64_t id = (t.r0 + rr) * a.X + t.x;
+    const uint64_t h = mix64(a.seed + (id + 1) * 0x9E3779B97F4A7C15ull);
     const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
     a.out[id] = (float)(gval + a.amp * (2.0 * u - 1.0));
   }
 }
 
-// Many fields in one launch (the generators above, same arithmetic).  Block
-// b belongs to the volume whose block range holds it; the volume's blocks
-// stride over its points.
+// Grid: ceil(gen_tasks / kGenWaves) workgroups.
+__global__ __launch_bounds__(kGenThreads) void k_field_gyroid(GyroidArgs a) {
+  const uint64_t task = (uint64_t)blockIdx.x * kGenWaves + wave_id();
+  if (task < gen_tasks(a.X, a.Y, a.Z)) gyroid_task(a, task);
+}
+
+// Many fields in one launch (the generators above, same arithmetic).
+// Workgroup b belongs to the volume whose range [block_off, block_off +
+// nblocks) holds it, so its waves share the volume and its kind.
 struct GenVol {
   float *out;
   uint32_t X, Y, Z, kind;  // kind 1: blobs, 2: gyroid
@@ -1426,7 +1509,7 @@ struct GenVol {
   double amp;              // gyroid noise amplitude
 };
 
-__global__ __launch_bounds__(256) void k_field_batch(const GenVol *vols, uint32_t nvol) {
+__global__ __launch_bounds__(kGenThreads) void k_field_batch(const GenVol *vols, uint32_t nvol) {
   uint32_t lo = 0, hi = nvol;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -1434,25 +1517,19 @@ __global__ __launch_bounds__(256) void k_field_batch(const GenVol *vols, uint32_
   }
   const GenVol &v = vols[__builtin_amdgcn_readfirstlane(lo)];
   const uint32_t local = blockIdx.x - v.block_off;
+  const uint64_t task = (uint64_t)local * kGenWaves + wave_id();
   if (v.kind == 1) {
-    const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, v.t[0], v.t[1], v.t[2], v.t[3]};
+    const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, blob_ldb(v.nblobs), v.t[0], v.t[1], v.t[2], v.t[3]};
     if (v.nblobs <= kBlobRegs) {
-      const uint32_t xcb = (v.X + 255u) / 256u;
-      blob_rows(ba, (local % xcb) * 256u, (local / xcb) * kBlobRows);
+      if (task < gen_tasks(v.X, v.Y, v.Z)) blob_task(ba, task);
     } else {
       blob_points(ba, (size_t)local * blockDim.x + threadIdx.x, (size_t)v.nblocks * blockDim.x);
     }
     return;
   }
-  const size_t n = (size_t)v.X * v.Y * v.Z;
-  for (size_t id = (size_t)local * blockDim.x + threadIdx.x; id < n; id += (size_t)v.nblocks * blockDim.x) {
-    const uint32_t x = (uint32_t)(id % v.X);
-    const size_t r = id / v.X;
-    const uint32_t y = (uint32_t)(r % v.Y), z = (uint32_t)(r / v.Y);
-    const double gval = v.t[0][x] * v.t[3][y] + v.t[2][y] * v.t[5][z] + v.t[4][z] * v.t[1][x];
-    const uint64_t h = mix64(v.seed + ((uint64_t)id + 1) * 0x9E3779B97F4A7C15ull);
-    const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
-    v.out[id] = (float)(gval + v.amp * (2.0 * u - 1.0));
+  if (task < gen_tasks(v.X, v.Y, v.Z)) {
+    const GyroidArgs ga{v.out, v.X, v.Y, v.Z, v.t[0], v.t[1], v.t[2], v.t[3], v.t[4], v.t[5], v.seed, v.amp};
+    gyroid_task(ga, task);
   }
 }
 
