@@ -942,16 +942,6 @@ __global__ __launch_bounds__(256) void k_coords(CoordArgs c) {
   coords_fill(c, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
-// Rank of the active cell at x offset xl in segment s, or -1 if the group
-// map says it is not active.
-__device__ __forceinline__ int32_t rank_in_segment(const EmitArgs &a, uint32_t s, uint32_t xl) {
-  const uint4 e = a.gmap[(size_t)s * (a.cap >> 6) + (xl >> 6)];
-  const uint64_t mask = (uint64_t)e.x | ((uint64_t)e.y << 32);
-  const uint32_t b = xl & 63u;
-  if (!((mask >> b) & 1ull)) return -1;
-  return (int32_t)(e.z + (uint32_t)__popcll(mask & ((1ull << b) - 1ull)));
-}
-
 // One active cell (segment s, rank within it) of volume vol: its vertices and
 // the quads of its owned edges.
 __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
@@ -1074,66 +1064,63 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   if (!(e6 || e9 || e10)) return;
   const uint32_t nq = (uint32_t)e6 + (uint32_t)e9 + (uint32_t)e10;
   if ((uint64_t)qb + nq > vol.qcap) { atomicOr(vol.status, kStatusQuadOverflow); return; }
+  // Ring cells n = 0 (+1,0,0), 1 (0,+1,0), 2 (0,0,+1), 3 (+1,+1,0),
+  // 4 (+1,0,+1), 5 (0,+1,+1).  A ring cell contains the crossing edge, so it
+  // is active and its group-map entry is current.  A cell the quads do not
+  // need is replaced by this cell, so that every load below is in bounds and
+  // all of them issue together: group maps and segment bases, then records.
+  const uint32_t need = (e6 ? 0x0Bu : 0u) | (e9 ? 0x15u : 0u) | (e10 ? 0x26u : 0u);
+  const bool wrap = xl + 1u == cap;  // +x lies in the next x-tile's segment
+  const uint32_t sx = wrap ? s + 1u : s, xl1 = wrap ? 0u : xl + 1u;
+  const uint32_t dy = vol.XT, dz = vol.XT * g.cy;
+  const uint32_t ns0[6] = {sx, s + dy, s + dz, sx + dy, sx + dz, s + dy + dz};
+  const uint32_t nx0[6] = {xl1, xl, xl, xl1, xl1, xl};
+  uint32_t ns[6], nxl[6], vq0[6];
+  uint4 gm[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const bool use = (need >> n) & 1u;
+    ns[n] = use ? ns0[n] : s;
+    nxl[n] = use ? nx0[n] : xl;
+    gm[n] = a.gmap[(size_t)ns[n] * (cap >> 6) + (nxl[n] >> 6)];
+    vq0[n] = a.seg_vq[ns[n]].x;
+  }
+  const uint32_t *slot_lo = reinterpret_cast<const uint32_t *>(a.slots);  // record bits 0..31
+  uint32_t nrec[6];
+  bool act[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const uint64_t mask = (uint64_t)gm[n].x | ((uint64_t)gm[n].y << 32);
+    const uint32_t bit = nxl[n] & 63u;
+    const uint32_t r = gm[n].z + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
+    act[n] = ((mask >> bit) & 1ull) && r < cap;
+    nrec[n] = slot_lo[2 * ((size_t)ns[n] * cap + (act[n] ? r : rank))];
+  }
   bool ok = true;
-  // Cell (x, row of segment s2) has rank r2 in s2; the cell one step in +x
-  // is rank r2 + 1 of s2 or, past the x-tile, rank 0 of the next x-tile's
-  // segment.  Returns its vertex for local edge e.
-  auto vertex_of = [&](uint32_t s2, int32_t r2, int e) -> int32_t {
-    if (r2 < 0 || (uint32_t)r2 >= cap) { ok = false; return -1; }
-    const uint32_t m2 = (uint32_t)a.slots[(size_t)s2 * cap + (uint32_t)r2];
+  // vertex of ring cell n for its local edge e
+  auto vtx = [&](int n, int e, bool used) -> int32_t {
+    const uint32_t m2 = nrec[n];
     const uint32_t lab = (uint32_t)(s_lab[rec_lut2(m2) * 256 + rec_cfg(m2)] >> (4 * e)) & 0xFu;
-    if (lab == 0xFu) { ok = false; return -1; }
-    return (int32_t)(a.seg_vq[s2].x + rec_lvb(m2) + lab - vol.vbase);
+    if (used && (!act[n] || lab == 0xFu)) ok = false;
+    return (int32_t)(vq0[n] + rec_lvb(m2) + lab - vol.vbase);
   };
-  // the +x neighbour of (segment s2, rank r2, x offset xo)
-  auto next_x = [&](uint32_t s2, int32_t r2, uint32_t xo, uint32_t &s_out) -> int32_t {
-    if (r2 < 0) return -1;
-    if (xo + 1u < cap) {
-      s_out = s2;
-      const int32_t r = r2 + 1;
-      if ((uint32_t)r >= cap || rec_x(a.slots[(size_t)s2 * cap + (uint32_t)r]) != xo + 1u) return -1;
-      return r;
-    }
-    s_out = s2 + 1u;  // same row, next x-tile (in the grid since x + 1 < cx)
-    if (rec_x(a.slots[(size_t)s_out * cap]) != 0u || (a.seg_counts[s_out].x & 0xFFFFu) == 0u) return -1;
-    return 0;
-  };
-  const uint32_t s_y = s + vol.XT, s_z = s + vol.XT * g.cy, s_yz = s_z + vol.XT;
-  const int32_t r_y = (e6 || e10) ? rank_in_segment(a, s_y, xl) : -1;
-  const int32_t r_z = (e9 || e10) ? rank_in_segment(a, s_z, xl) : -1;
   int4 *qo = reinterpret_cast<int4 *>(vol.quads) + qb;
   uint32_t qn = 0;
   const int32_t self = (int32_t)vb;
   if (e6) {
-    uint32_t sx, sxy;
-    const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
-    const int32_t rxy = next_x(s_y, r_y, xl, sxy);
     const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
-    const int32_t m1 = vertex_of(sx, rx, 7);
-    const int32_t m2 = vertex_of(sxy, rxy, 4);
-    const int32_t m3 = vertex_of(s_y, r_y, 5);
-    const bool ccw = v[2] <= iso;
-    qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+    const int32_t m1 = vtx(0, 7, true), m2 = vtx(3, 4, true), m3 = vtx(1, 5, true);
+    qo[qn++] = v[2] <= iso ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
   }
   if (e9) {
-    uint32_t sx, sxz;
-    const int32_t rx = next_x(s, (int32_t)rank, xl, sx);
-    const int32_t rxz = next_x(s_z, r_z, xl, sxz);
     const int32_t m0 = self + (int32_t)((labels >> 36) & 0xFu);
-    const int32_t m1 = vertex_of(s_z, r_z, 1);
-    const int32_t m2 = vertex_of(sxz, rxz, 3);
-    const int32_t m3 = vertex_of(sx, rx, 11);
-    const bool ccw = v[5] <= iso;
-    qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+    const int32_t m1 = vtx(2, 1, true), m2 = vtx(4, 3, true), m3 = vtx(0, 11, true);
+    qo[qn++] = v[5] <= iso ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
   }
   if (e10) {
-    const int32_t r_yz = rank_in_segment(a, s_yz, xl);
     const int32_t m0 = self + (int32_t)((labels >> 40) & 0xFu);
-    const int32_t m1 = vertex_of(s_y, r_y, 8);
-    const int32_t m2 = vertex_of(s_yz, r_yz, 0);
-    const int32_t m3 = vertex_of(s_z, r_z, 2);
-    const bool ccw = v[7] <= iso;
-    qo[qn++] = ccw ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
+    const int32_t m1 = vtx(1, 8, true), m2 = vtx(5, 0, true), m3 = vtx(2, 2, true);
+    qo[qn++] = v[7] <= iso ? make_int4(m0, m1, m2, m3) : make_int4(m0, m3, m2, m1);
   }
   if (!ok) atomicOr(vol.status, kStatusLookup);
 }
