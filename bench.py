@@ -50,6 +50,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner sweep)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--served-requests", type=int, default=10000)
@@ -91,6 +92,12 @@ def main() -> int:
     quads = torch.empty((Q, 4), dtype=torch.int32, device="cuda")
     counts = torch.zeros(4, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
+    tuned = None
+    if not args.no_tune:
+        from framework_amd import autotune
+
+        tuned = autotune.tune_single(ctx, field, ISO)
+        log(f"launch options {tuned['options']} ({tuned['ms']:.4f} ms)")
 
     for _ in range(args.warmup):
         ctx.extract_into(field, ISO, verts, quads, counts=counts)
@@ -152,7 +159,7 @@ def main() -> int:
                "sample": f"full 512^3 blob field, {len(times)} run(s), median {tmed:.3f} s "
                          f"(C oracle restating dmc::run_dmc, one thread, host of the GPU box)"}
 
-    batched = None if args.skip_batched else bench_batched(fa, torch)
+    batched = None if args.skip_batched else bench_batched(fa, torch, tune=not args.no_tune)
     served = None if args.skip_served else bench_served(fa, torch, args.served_requests)
 
     out = {
@@ -170,7 +177,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "plain DMC, 512^3 FP32 sum of 32 Gaussian blobs (seed 0, sigma U[8,32], "
                                "amp U[-1,1]), iso 0.1, field resident in HBM",
-                   "dims": list(DIMS), "iso": ISO, "active_cells": A, "vertices": V, "quads": Q},
+                   "dims": list(DIMS), "iso": ISO, "active_cells": A, "vertices": V, "quads": Q,
+                   "launch_options": None if tuned is None else
+                   {autotune.OPTION_NAMES[k]: v for k, v in tuned["options"].items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": dom_ms, "algorithmic_bytes": algo[dom]},
@@ -191,8 +200,9 @@ def _log_uniform_int(rng, lo, hi):
     return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
 
 
-def bench_batched(fa, torch, nvol: int = 256, reps: int = 5) -> dict:
-    """256 uneven volumes: plain per-volume loop vs one batched launch sequence."""
+def bench_batched(fa, torch, nvol: int = 256, reps: int = 5, tune: bool = True) -> dict:
+    """256 uneven volumes: plain per-volume loop vs one batched launch sequence
+    (launch shapes chosen by the autotuner unless tune is False)."""
     import numpy as np
 
     rng = np.random.default_rng(1)
@@ -224,6 +234,12 @@ def bench_batched(fa, torch, nvol: int = 256, reps: int = 5) -> dict:
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t
     # batched: count all, one sync, allocate, emit all
+    tuned = None
+    if tune:
+        from framework_amd import autotune
+
+        tuned = autotune.tune_batch(ctx, fields, isos)
+        log(f"batched: launch options {tuned['options']} ({tuned['ms']:.3f} ms)")
     ctx.batch_extract(fields, isos)
     torch.cuda.synchronize()
     times = []
@@ -261,7 +277,9 @@ def bench_batched(fa, torch, nvol: int = 256, reps: int = 5) -> dict:
            "plain_loop_volumes_per_s": nvol / t_plain, "batched_volumes_per_s": nvol / t_batch,
            "graph_volumes_per_s": nvol / t_graph, "speedup_batched": t_plain / t_batch,
            "speedup_graph": t_plain / t_graph, "batched_gvoxels_per_s": points / t_batch / 1e9,
-           "graph_gvoxels_per_s": points / t_graph / 1e9}
+           "graph_gvoxels_per_s": points / t_graph / 1e9,
+           "launch_options": None if tuned is None else
+           {autotune.OPTION_NAMES[k]: v for k, v in tuned["options"].items()}}
     log(f"batched: {out}")
     return out
 

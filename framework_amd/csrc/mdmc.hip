@@ -59,7 +59,7 @@ static int env_int(const char *name, int dflt) {
   return s ? std::atoi(s) : dflt;
 }
 
-int plan_tiling(const uint32_t dims[3], Plan *p) {
+int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt) {
   std::memset(p, 0, sizeof(*p));
   p->X = dims[0];
   p->Y = dims[1];
@@ -119,17 +119,13 @@ int plan_tiling(const uint32_t dims[3], Plan *p) {
   p->vec4 = (p->X % 4 == 0) && env_int("MDMC_VEC4", 1) != 0;
   p->ptR = pt_rows(W / 4);
   p->ptKP = pt_planes(W / 4);
-  if (const char *t = std::getenv("MDMC_PT_SHAPE")) {  // "R,KP" override for tuning
-    int r = 0, kp = 0;
-    if (std::sscanf(t, "%d,%d", &r, &kp) == 2 && pt_shape_supported(W / 4, r, kp)) {
-      p->ptR = (uint32_t)r;
-      p->ptKP = (uint32_t)kp;
-    }
-  }
+  // context options (0: default): cell planes per tile, XCD band
+  const int kp_opt = opt ? opt[MDMC_OPT_PT_PLANES] : 0;
+  if (kp_opt > 0 && pt_shape_supported(W / 4, (int)p->ptR, kp_opt)) p->ptKP = (uint32_t)kp_opt;
   const uint32_t rows = p->ptR;
   const uint32_t nrb = (p->cy + rows - 1) / rows;
-  p->band = (uint32_t)env_int("MDMC_PT_BAND", 1);
-  if (p->band < 1) p->band = 1;
+  const int band_opt = opt ? opt[MDMC_OPT_PT_BAND] : 0;
+  p->band = band_opt > 0 && band_opt <= 64 ? (uint32_t)band_opt : 1u;
   p->NS = (nrb + 8 * p->band - 1) / (8 * p->band);
   const uint32_t kp = p->ptKP;
   p->pt_blocks = 8ull * p->band * p->NS * p->XT * ((p->cz + kp - 1) / kp);
@@ -173,7 +169,7 @@ struct mdmc_ctx {
   hipEvent_t ghost_ev = nullptr;
   bool ghost_ev_used = false;
   // tuning options (0 = default)
-  int opt[4] = {0, 0, 0, 0};
+  int opt[mdmc::kNumOptions] = {};
   // per-kernel event timing
   bool profiling = false;
   std::vector<hipEvent_t> events;  // pool, used in (start, stop) pairs
@@ -380,7 +376,8 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
                        co);
     // one wave per chunk of 64 active cells, at most 8 workgroups per CU
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * 8));
+    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
     hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
   }
   HIP_TRY(hipGetLastError());
@@ -852,7 +849,7 @@ int mdmc_count_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3
   if (!ctx || !dims || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
   Plan p;
-  int rc = plan_tiling(dims, &p);
+  int rc = plan_tiling(dims, &p, ctx->opt);
   if (rc != MDMC_OK) return rc;
   if (!p.empty && !d_field) return fail(MDMC_ERR_INVALID, "null field");
   rc = ensure_scratch(ctx, p);
@@ -889,7 +886,7 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
 
 int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value) {
   if (!ctx) return fail(MDMC_ERR_INVALID, "null ctx");
-  if (option < 0 || option >= 4) return fail(MDMC_ERR_INVALID, "unknown option");
+  if (option < 0 || option >= kNumOptions) return fail(MDMC_ERR_INVALID, "unknown option");
   ctx->opt[option] = value;
   return MDMC_OK;
 }
@@ -989,7 +986,7 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
   }
   mdmc_ctx *ctx = g_default_ctx;
   Plan p;
-  rc = plan_tiling(dims, &p);
+  rc = plan_tiling(dims, &p, ctx->opt);
   if (rc != MDMC_OK) return rc;
   if (p.empty) return MDMC_OK;
   if (!field) return fail(MDMC_ERR_INVALID, "null field");

@@ -38,7 +38,9 @@ struct Scratch {
   float *coords;
 };
 
-int plan_tiling(const uint32_t dims[3], Plan *p);
+// opt: the context's option values (MDMC_OPT_*), or null for defaults.
+int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt);
+constexpr int kNumOptions = 5;
 
 // Tile shapes with a compiled classify kernel.
 inline bool tiling_supported(uint32_t W, uint32_t R) {
