@@ -146,6 +146,8 @@ struct mdmc_ctx {
   mdmc::Scratch s{};
   mdmc::Plan plan{};
   bool counted = false;
+  bool coords_valid = false;  // coordinate tables hold coords_box for the counted plan
+  float coords_box[6] = {};
   const float *counted_field = nullptr;
   float iso = 0.f;
   // batches: device table of BatchVol + bases + the batch's own counts, and
@@ -281,9 +283,20 @@ __global__ void k_zero_counts(uint32_t *c, Totals *t) {
   if (threadIdx.x < 8 && t) reinterpret_cast<uint32_t *>(t)[threadIdx.x] = 0u;
 }
 
+// Coordinate tables of a plan for a box: xyz_range = xyz_max - xyz_min in
+// float (dmc.cu:2149).
+CoordArgs coord_args(const Plan &p, const Scratch &s, const float mn[3], const float mx[3]) {
+  return CoordArgs{s.coords, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, mn[0], mn[1], mn[2],
+                   mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+}
+
+// box (optional): xyz_min, xyz_max; with it the scan also fills the
+// coordinate tables the emit reads.
 int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
-                 mdmc_counts *d_counts, hipStream_t st) {
+                 mdmc_counts *d_counts, hipStream_t st, const float *box_min = nullptr,
+                 const float *box_max = nullptr) {
   const Scratch &s = ctx->s;
+  ctx->coords_valid = false;
   if (p.empty) {
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st,
                        reinterpret_cast<uint32_t *>(d_counts), s.totals);
@@ -332,6 +345,13 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   sa.ticket = s.ticket;
   sa.totals = s.totals;
   sa.counts_out = reinterpret_cast<uint32_t *>(d_counts);
+  if (box_min && box_max) {
+    sa.co = coord_args(p, s, box_min, box_max);
+    sa.fill_coords = 1;
+    std::memcpy(ctx->coords_box, box_min, 3 * sizeof(float));
+    std::memcpy(ctx->coords_box + 3, box_max, 3 * sizeof(float));
+    ctx->coords_valid = true;
+  }
   {
     ProfScope ps(ctx, 1, st);
     hipLaunchKernelGGL(k_scan, dim3(p.ntiles), dim3(kScanThreads), 0, st, sa);
@@ -348,9 +368,10 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   const Scratch &s = ctx->s;
   EmitArgs ea{};
   ea.v.g = FieldDesc{d_field, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
-  // xyz_range = xyz_max - xyz_min in float (dmc.cu:2149)
-  CoordArgs co{s.coords, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, mn[0], mn[1], mn[2],
-               mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+  const CoordArgs co = coord_args(p, s, mn, mx);
+  // the count already filled the tables for this box
+  const bool have_coords = ctx->coords_valid && std::memcmp(ctx->coords_box, mn, 3 * sizeof(float)) == 0 &&
+                           std::memcmp(ctx->coords_box + 3, mx, 3 * sizeof(float)) == 0;
   ea.v.xt_c = s.coords;
   ea.v.yt_c = s.coords + p.X;
   ea.v.zt_c = s.coords + p.X + p.Y;
@@ -372,8 +393,9 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
   {
     ProfScope ps(ctx, 2, st);
-    hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0, st,
-                       co);
+    if (!have_coords)
+      hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0,
+                         st, co);
     // one wave per chunk of 64 active cells, at most 8 workgroups per CU
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
     const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
@@ -844,8 +866,8 @@ void mdmc_ctx_destroy(mdmc_ctx *ctx) {
   delete ctx;
 }
 
-int mdmc_count_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3], float iso,
-                      mdmc_counts *d_counts, void *stream) {
+static int count_impl(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3], float iso,
+                      mdmc_counts *d_counts, void *stream, const float *box_min, const float *box_max) {
   if (!ctx || !dims || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
   Plan p;
@@ -854,13 +876,18 @@ int mdmc_count_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3
   if (!p.empty && !d_field) return fail(MDMC_ERR_INVALID, "null field");
   rc = ensure_scratch(ctx, p);
   if (rc != MDMC_OK) return rc;
-  rc = launch_count(ctx, p, d_field, iso, d_counts, static_cast<hipStream_t>(stream));
+  rc = launch_count(ctx, p, d_field, iso, d_counts, static_cast<hipStream_t>(stream), box_min, box_max);
   if (rc != MDMC_OK) return rc;
   ctx->plan = p;
   ctx->counted = true;
   ctx->counted_field = d_field;
   ctx->iso = iso;
   return MDMC_OK;
+}
+
+int mdmc_count_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3], float iso,
+                      mdmc_counts *d_counts, void *stream) {
+  return count_impl(ctx, d_field, dims, iso, d_counts, stream, nullptr, nullptr);
 }
 
 int mdmc_emit_device(mdmc_ctx *ctx, const float *d_field, const float xyz_min[3],
@@ -878,7 +905,8 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
                         const float xyz_min[3], const float xyz_max[3], float iso,
                         float *d_vertices, uint64_t vertex_capacity, int32_t *d_quads,
                         uint64_t quad_capacity, mdmc_counts *d_counts, void *stream) {
-  int rc = mdmc_count_device(ctx, d_field, dims, iso, d_counts, stream);
+  if (!xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
+  int rc = count_impl(ctx, d_field, dims, iso, d_counts, stream, xyz_min, xyz_max);
   if (rc != MDMC_OK) return rc;
   return mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_vertices, vertex_capacity, d_quads,
                           quad_capacity, d_counts, stream);

@@ -689,6 +689,35 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
   }
 }
 
+// utils::ijk_to_xyz (include/utils.hpp:118-122): i / n * range + min, three
+// correctly rounded float operations.
+__device__ __forceinline__ float coordf(uint32_t i, uint32_t n, float range, float mn) {
+  return (float)i / (float)n * range + mn;
+}
+
+// Coordinates of the point indices of a volume, per axis (X, then Y, then Z
+// entries): the same three float operations per entry as the reference.
+struct CoordArgs {
+  float *out;
+  uint32_t X, Y, Z, cx, cy, cz;
+  float mnx, mny, mnz, rgx, rgy, rgz;
+};
+
+__device__ __forceinline__ void coords_fill(const CoordArgs &c, uint32_t i0, uint32_t stride) {
+  const uint32_t n = c.X + c.Y + c.Z;
+  for (uint32_t i = i0; i < n; i += stride) {
+    float v;
+    if (i < c.X) v = coordf(i, c.cx, c.rgx, c.mnx);
+    else if (i < c.X + c.Y) v = coordf(i - c.X, c.cy, c.rgy, c.mny);
+    else v = coordf(i - c.X - c.Y, c.cz, c.rgz, c.mnz);
+    c.out[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_coords(CoordArgs c) {
+  coords_fill(c, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Decoupled look-back scan over segment totals.  A tile is 4096 segments, 16
 // per thread read as eight 16-byte loads (the totals array is padded to a
@@ -715,6 +744,8 @@ struct ScanArgs {
   uint32_t *ticket;
   Totals *totals;
   uint32_t *counts_out;  // optional user mdmc_counts
+  CoordArgs co;          // coordinate tables, filled by the scan's workgroups
+  uint32_t fill_coords;  // when the box is known at count time
 };
 
 __device__ __forceinline__ unsigned long long ld_state(const unsigned long long *p) {
@@ -870,6 +901,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
     }
   }
   if (bad && a.counts_out) atomicOr(a.counts_out + 3, kStatusInternal);
+  if (a.fill_coords) coords_fill(a.co, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -912,35 +944,6 @@ struct EmitArgs {
   const uint32_t *chunk_ne;
   const Totals *totals;
 };
-
-// utils::ijk_to_xyz (include/utils.hpp:118-122): i / n * range + min, three
-// correctly rounded float operations.
-__device__ __forceinline__ float coordf(uint32_t i, uint32_t n, float range, float mn) {
-  return (float)i / (float)n * range + mn;
-}
-
-// Coordinates of the point indices of a volume, per axis (X, then Y, then Z
-// entries): the same three float operations per entry as the reference.
-struct CoordArgs {
-  float *out;
-  uint32_t X, Y, Z, cx, cy, cz;
-  float mnx, mny, mnz, rgx, rgy, rgz;
-};
-
-__device__ __forceinline__ void coords_fill(const CoordArgs &c, uint32_t i0, uint32_t stride) {
-  const uint32_t n = c.X + c.Y + c.Z;
-  for (uint32_t i = i0; i < n; i += stride) {
-    float v;
-    if (i < c.X) v = coordf(i, c.cx, c.rgx, c.mnx);
-    else if (i < c.X + c.Y) v = coordf(i - c.X, c.cy, c.rgy, c.mny);
-    else v = coordf(i - c.X - c.Y, c.cz, c.rgz, c.mnz);
-    c.out[i] = v;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_coords(CoordArgs c) {
-  coords_fill(c, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
 
 // One active cell (segment s, rank within it) of volume vol: its vertices and
 // the quads of its owned edges.

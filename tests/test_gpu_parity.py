@@ -152,6 +152,39 @@ def test_world_box(ctx, torch_cuda):
     check_against_oracle(ctx, _dev(torch_cuda, f), 0.0, xyz_min=(-5.0, -2.5, 1.0), xyz_max=(5.0, 7.5, 3.0))
 
 
+def test_world_box_through_both_coordinate_paths(ctx, torch_cuda):
+    """The combined extract fills the coordinate tables during its count; a
+    separate count + emit fills them in the emit.  Alternating boxes and paths
+    on one context gives the oracle's vertices every time."""
+    torch = torch_cuda
+    f = oracle.field_sphere((37, 29, 44), (18.0, 14.5, 21.0), 11.7)
+    ft = _dev(torch, f)
+    boxes = [((-5.0, -2.5, 1.0), (5.0, 7.5, 3.0)), ((0.0, 0.0, 0.0), (36.0, 28.0, 43.0)),
+             ((2.0, -1.0, -3.0), (2.5, 9.0, 0.0))]
+    for rep in range(2):
+        for mn, mx in boxes:
+            ref = oracle.run(f, 0.0, mn, mx)
+            V, Q = ref["num_vertices"], ref["num_quads"]
+            verts = torch.empty((V, 3), device="cuda")
+            quads = torch.empty((Q, 4), dtype=torch.int32, device="cuda")
+            if rep == 0:
+                ctx.extract_into(ft, 0.0, verts, quads, xyz_min=mn, xyz_max=mx)
+            else:
+                ctx.count(ft, 0.0)
+                ctx.emit(ft, verts, quads, xyz_min=mn, xyz_max=mx)
+            torch.cuda.synchronize()
+            assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), ref)
+    # count with one box (combined call), then emit the same count with another box
+    mn, mx = boxes[0]
+    ref = oracle.run(f, 0.0, *boxes[2])
+    verts = torch.empty((ref["num_vertices"], 3), device="cuda")
+    quads = torch.empty((ref["num_quads"], 4), dtype=torch.int32, device="cuda")
+    ctx.extract_into(ft, 0.0, verts, quads, xyz_min=mn, xyz_max=mx)
+    ctx.emit(ft, verts, quads, xyz_min=boxes[2][0], xyz_max=boxes[2][1])
+    torch.cuda.synchronize()
+    assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), ref)
+
+
 def test_field_generators_match_cpu(ctx, torch_cuda):
     dims = (70, 51, 33)
     gb = ctx.field_blobs(dims, seed=11, nblobs=9)
