@@ -12,7 +12,10 @@ second).  `roofline` is for the dominant kernel (longest average launch),
 timed live with HIP events the library records around each launch on its
 stream.  `cpu_baseline` is the C oracle (single thread) on the same field.
 
-Two more legs go into the same line:
+Three more legs go into the same line:
+  "dense": the surface-dense case, 1024^3 gyroid (period 24) + noise 0.05,
+      iso 0: ms per extraction, Gvoxels/s and the pipeline's algorithmic
+      bytes (field + vertices + quads) against the HBM peak.
   "batched": 256 volumes, dims per axis log-uniform in [32, 256], alternating
       blob sums (8-32 blobs) and gyroid+noise, isovalues U[-0.2, 0.2]; the
       plain per-volume loop (count, read sizes, allocate, emit, per volume)
@@ -51,8 +54,12 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner sweep)")
+    ap.add_argument("--options", default="",
+                    help="fixed single-volume launch options instead of the autotuner, e.g. "
+                         "'pt_planes=3,emit_blocks_per_cu=16' (names as in framework_amd.autotune.OPTION_NAMES)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
+    ap.add_argument("--skip-dense", action="store_true")
     ap.add_argument("--served-requests", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy --steps times (a kernel of known "
@@ -93,9 +100,15 @@ def main() -> int:
     counts = torch.zeros(4, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     tuned = None
-    if not args.no_tune:
-        from framework_amd import autotune
+    from framework_amd import autotune
 
+    if args.options:
+        by_name = {v: k for k, v in autotune.OPTION_NAMES.items()}
+        opts = {by_name[kv.split("=")[0].strip()]: int(kv.split("=")[1]) for kv in args.options.split(",")}
+        autotune.apply(ctx, opts)
+        tuned = {"options": opts}
+        log(f"launch options {opts} (fixed)")
+    elif not args.no_tune:
         tuned = autotune.tune_single(ctx, field, ISO)
         log(f"launch options {tuned['options']} ({tuned['ms']:.4f} ms)")
 
@@ -165,6 +178,7 @@ def main() -> int:
                "sample": f"full 512^3 blob field, {len(times)} run(s), median {tmed:.3f} s "
                          f"(C oracle restating dmc::run_dmc, one thread, host of the GPU box)"}
 
+    dense = None if args.skip_dense else bench_dense(fa, torch)
     batched = None if args.skip_batched else bench_batched(fa, torch, tune=not args.no_tune)
     served = None if args.skip_served else bench_served(fa, torch, args.served_requests)
 
@@ -192,12 +206,58 @@ def main() -> int:
         "pipeline": {"algorithmic_bytes": B, "achieved_gbs": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
                      "kernel_avg_ms": per_kernel},
         "cpu_baseline": cpu,
+        "dense": dense,
         "batched": batched,
         "served": served,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def bench_dense(fa, torch, n: int = 1024, steps: int = 5) -> dict:
+    """The surface-dense case: n^3 gyroid (period 24) + noise 0.05, iso 0."""
+    ctx = fa.Context()
+    log(f"dense: generating {n}^3 gyroid")
+    f = ctx.field_gyroid((n, n, n), period=24.0, seed=0, noise_amp=0.05)
+    torch.cuda.synchronize()
+    ctx.count(f, 0.0)
+    torch.cuda.synchronize()
+    c = ctx.read_counts()
+    A, V, Q = c["num_active"], c["num_vertices"], c["num_quads"]
+    verts = torch.empty((V, 3), dtype=torch.float32, device="cuda")
+    quads = torch.empty((Q, 4), dtype=torch.int32, device="cuda")
+    counts = torch.zeros(4, dtype=torch.int32, device="cuda")
+    ctx.extract_into(f, 0.0, verts, quads, counts=counts)
+    stream = torch.cuda.current_stream()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0.record(stream)
+    for _ in range(steps):
+        ctx.extract_into(f, 0.0, verts, quads, counts=counts)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    ctx.profile(True)
+    ctx.profile_read()
+    ctx.extract_into(f, 0.0, verts, quads, counts=counts)
+    torch.cuda.synchronize()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    st = counts.cpu().numpy()
+    assert st[3] == 0 and st[1] == V and st[2] == Q
+    N = n ** 3
+    B = 4.0 * N + 12.0 * V + 16.0 * Q
+    out = {"workload": f"{n}^3 FP32 gyroid, period 24, noise U(-0.05, 0.05), iso 0", "active_cells": A,
+           "vertices": V, "quads": Q, "ms_per_extraction": ms, "gvoxels_per_s": N / (ms * 1e-3) / 1e9,
+           "algorithmic_bytes": B, "achieved_gbs": B / (ms * 1e-3) / 1e9,
+           "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items() if k != "other"}}
+    log(f"dense: {out}")
+    ctx.close()
+    del f, verts, quads
+    torch.cuda.empty_cache()
+    return out
 
 
 def _log_uniform_int(rng, lo, hi):

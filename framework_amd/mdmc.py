@@ -299,7 +299,9 @@ class Context:
         _check(lib().mdmc_ctx_profile_read(self._h, ms, n), "mdmc_ctx_profile_read")
         return {k: (ms[i], n[i]) for i, k in enumerate(self.KERNELS)}
 
-    def cells(self, num_active: int, stream=None) -> dict:
+    def cells(self, num_active: int, stream=None, host: bool = True) -> dict:
+        """Per active cell of the last count: cell index, configuration, table
+        variant and first vertex (numpy arrays, or CUDA tensors with host=False)."""
         torch = self._torch
         n = max(int(num_active), 1)
         ci = torch.empty(n, dtype=torch.int32, device="cuda")
@@ -310,6 +312,8 @@ class Context:
                                        _VP(vb.data_ptr()), num_active, self._stream(stream)), "mdmc_cells_device")
         (stream or torch.cuda.current_stream()).synchronize()
         k = int(num_active)
+        if not host:
+            return {"cell_index": ci[:k], "cfg": cfg[:k], "lut2": fl[:k], "vertex_begin": vb[:k]}
         return {"cell_index": ci[:k].cpu().numpy().view(np.uint32), "cfg": cfg[:k].cpu().numpy(),
                 "lut2": fl[:k].cpu().numpy(), "vertex_begin": vb[:k].cpu().numpy().view(np.uint32)}
 

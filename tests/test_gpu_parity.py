@@ -239,3 +239,101 @@ def test_blobs_512_full(ctx, torch_cuda):
     torch_cuda.cuda.synchronize()
     ref = check_against_oracle(ctx, f, 0.1, check_cells=True)
     assert ref["num_active"] > 100000
+
+
+def _closed_form_counts(torch, f, iso, chunk=64):
+    """Active cells (corners not all on one side) and quads (sign-changing
+    grid edges whose four-cell ring lies in the grid), counted on the GPU
+    from the field alone, plane chunk by plane chunk."""
+    Z, Y, X = f.shape
+    A = Q = 0
+    for z0 in range(0, Z - 1, chunk):
+        z1 = min(z0 + chunk, Z - 1)
+        b = (f[z0:z1 + 1] < iso).to(torch.int8)
+        s = (b[:-1, :-1, :-1] + b[:-1, :-1, 1:] + b[:-1, 1:, :-1] + b[:-1, 1:, 1:] +
+             b[1:, :-1, :-1] + b[1:, :-1, 1:] + b[1:, 1:, :-1] + b[1:, 1:, 1:])
+        A += int(((s > 0) & (s < 8)).sum())
+        # x- and y-edges in point planes z in [z0, z1) with 1 <= z <= Z - 2;
+        # z-edges from planes z in [z0, z1)
+        lo, hi = max(z0, 1), min(z1 - 1, Z - 2)
+        if hi >= lo:
+            bb = b[lo - z0:hi - z0 + 1]
+            Q += int((bb[:, 1:-1, 1:] != bb[:, 1:-1, :-1]).sum())  # x-edges, y in [1, Y-2]
+            Q += int((bb[:, 1:, 1:-1] != bb[:, :-1, 1:-1]).sum())  # y-edges, x in [1, X-2]
+        Q += int((b[1:, 1:-1, 1:-1] != b[:-1, 1:-1, 1:-1]).sum())  # z-edges, x, y in [1, dim-2]
+        del b, s
+    return A, Q
+
+
+def test_gyroid_1024_surface_dense(ctx, torch_cuda):
+    """1024^3 gyroid (period 24) + noise 0.05, iso 0: the surface-dense case.
+    Too large for the oracle, so: active-cell and quad counts against closed
+    forms from the field, quad indices valid and distinct, vertices in the
+    box, a repeat run bitwise equal, and an 80^3 corner window against the
+    oracle cell by cell (configuration, variant, vertex count and positions;
+    the window's coordinate division differs, hence a 1e-3 voxel tolerance)."""
+    torch = torch_cuda
+    n = 1024
+    f = ctx.field_gyroid((n, n, n), period=24.0, seed=0, noise_amp=0.05)
+    torch.cuda.synchronize()
+    verts, quads, cnt = ctx.extract(f, 0.0)
+    A, Q = _closed_form_counts(torch, f, 0.0)
+    assert cnt["num_active"] == A and cnt["num_quads"] == Q, (cnt, A, Q)
+    assert A > 2e8
+    V = cnt["num_vertices"]
+    assert int(quads.min()) >= 0 and int(quads.max()) < V
+    for a, b in ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)):
+        assert bool((quads[:, a] != quads[:, b]).all())
+    fin = torch.isfinite(verts)
+    assert bool(fin.all(dim=1).any())
+    vv = torch.where(fin, verts, torch.zeros_like(verts))
+    # in the box up to float rounding of i / cells * cells and of the lerp
+    assert float(vv.min()) >= -1e-3 and float(vv.max()) <= float(n - 1) + 1e-3
+    cells = ctx.cells(cnt["num_active"], host=False)
+    # repeat run into fresh buffers: bitwise equal
+    v2 = torch.empty_like(verts)
+    q2 = torch.empty_like(quads)
+    ctx.extract_into(f, 0.0, v2, q2)
+    torch.cuda.synchronize()
+    assert torch.equal(q2, quads) and torch.equal(v2.view(torch.int32), verts.view(torch.int32))
+    del v2, q2
+    # corner window vs the oracle
+    _window_check(torch, cells, verts, V, n, 80, f[:80, :80, :80].contiguous().cpu().numpy())
+
+
+def _window_check(torch, cells, verts, V, n, W, win):
+    """Cells of an n^3 grid's extraction (tensors of ctx.cells(host=False)
+    and the vertex tensor) against the oracle on the grid's W^3 corner
+    window, for cells at least two cells from the window's upper faces."""
+    dev = verts.device
+    ref = oracle.run(win, 0.0)
+    c = n - 1
+    ci = cells["cell_index"].to(torch.int64) & 0xFFFFFFFF
+    x, y, z = ci % c, (ci // c) % c, ci // (c * c)
+    sel = torch.nonzero((x < W - 3) & (y < W - 3) & (z < W - 3)).flatten()
+    vb_all = cells["vertex_begin"].to(torch.int64) & 0xFFFFFFFF
+    vb_next = torch.cat([vb_all[1:], torch.tensor([V], device=dev, dtype=torch.int64)])
+    g_key = (x[sel] + W * (y[sel] + W * z[sel])).cpu().numpy()
+    g_cfg = cells["cfg"][sel].cpu().numpy()
+    g_l2 = cells["lut2"][sel].cpu().numpy()
+    g_vb = vb_all[sel].cpu().numpy()
+    g_nv = (vb_next[sel] - vb_all[sel]).cpu().numpy()
+    wc = W - 1
+    rci = ref["cell_index"].astype(np.int64)
+    rx, ry, rz = rci % wc, (rci // wc) % wc, rci // (wc * wc)
+    rin = (rx < W - 3) & (ry < W - 3) & (rz < W - 3)
+    r_key = (rx + W * (ry + W * rz))[rin]
+    r_vb = ref["vertex_begin"].astype(np.int64)
+    r_nv = np.append(r_vb[1:], ref["num_vertices"]) - r_vb
+    assert len(r_key) > 1000
+    assert np.array_equal(np.sort(g_key), np.sort(r_key))
+    og, orf = np.argsort(g_key), np.argsort(r_key)
+    assert np.array_equal(g_cfg[og], ref["cfg"][rin][orf])
+    assert np.array_equal(g_l2[og], ((ref["info"] >> 7) & 1)[rin][orf])
+    assert np.array_equal(g_nv[og], r_nv[rin][orf])
+    gidx = np.concatenate([np.arange(b, b + k) for b, k in zip(g_vb[og], g_nv[og])])
+    g_pos = verts[torch.from_numpy(gidx).to(dev)].cpu().numpy()
+    r_pos = np.concatenate([ref["vertices"][b:b + k] for b, k in zip(r_vb[rin][orf], r_nv[rin][orf])])
+    assert np.array_equal(np.isfinite(g_pos), np.isfinite(r_pos))
+    both = np.isfinite(g_pos) & np.isfinite(r_pos)
+    assert np.abs(g_pos[both] - r_pos[both]).max() <= 1e-3
