@@ -60,6 +60,7 @@ def main() -> int:
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--skip-dense", action="store_true")
+    ap.add_argument("--dense-only", action="store_true", help="only the surface-dense leg (prints its JSON)")
     ap.add_argument("--served-requests", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy --steps times (a kernel of known "
@@ -85,6 +86,9 @@ def main() -> int:
         log(f"copied {src.numel() * 4} bytes x {args.steps}")
         return 0
     fa.setup()
+    if args.dense_only:
+        print(json.dumps(bench_dense(fa, torch, steps=args.steps)), flush=True)
+        return 0
     ctx = fa.Context()
     N = DIMS[0] * DIMS[1] * DIMS[2]
     log("generating field")

@@ -1156,6 +1156,25 @@ __device__ __forceinline__ bool locate_cell(const EmitArgs &a, const Totals &tot
   return true;
 }
 
+// Chunks of a wave.  Workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup b runs on XCD b % 8); with at least 8 workgroups, XCD x takes
+// the contiguous chunk range [x n / 8, (x + 1) n / 8) and its workgroups
+// stride through it, so that neighbouring chunks, which share field rows and
+// the records of neighbour segments, meet in one L2.
+struct ChunkIter {
+  uint32_t next, end, step;
+};
+
+__device__ __forceinline__ ChunkIter chunk_iter(uint32_t n, uint32_t wave) {
+  constexpr uint32_t wpb = kEmitThreads / kWave;
+  const uint32_t G = gridDim.x;
+  if (G < 8u) return ChunkIter{blockIdx.x * wpb + wave, n, G * wpb};
+  const uint32_t x = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+  const uint32_t gx = (G >> 3) + (x < (G & 7u) ? 1u : 0u);
+  const uint32_t lo = (uint32_t)((uint64_t)n * x / 8u), hi = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
+  return ChunkIter{lo + slot * wpb + wave, hi, gx * wpb};
+}
+
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
@@ -1167,8 +1186,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
   // scratch buffers even if the count pass produced inconsistent data.
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
-  for (uint32_t ch = blockIdx.x * (kEmitThreads / kWave) + wave; ch < num_chunks;
-       ch += gridDim.x * (kEmitThreads / kWave)) {
+  const ChunkIter it = chunk_iter(num_chunks, wave);
+  for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
     uint32_t s, rank;
     if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
     emit_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
@@ -1246,8 +1265,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit_batch(BatchEmitArgs b) {
   const Totals tot = *a.totals;
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
-  for (uint32_t ch = blockIdx.x * (kEmitThreads / kWave) + wave; ch < num_chunks;
-       ch += gridDim.x * (kEmitThreads / kWave)) {
+  const ChunkIter it = chunk_iter(num_chunks, wave);
+  for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
     uint32_t s, rank;
     if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
     // the last volume whose first segment is <= s (empty volumes share it)
