@@ -147,6 +147,8 @@ struct mdmc_ctx {
   mdmc::Plan plan{};
   bool counted = false;
   bool coords_valid = false;  // coordinate tables hold coords_box for the counted plan
+  bool emitted = false;       // emit_args describe the last emit of the counted plan
+  mdmc::EmitArgs emit_args{};
   float coords_box[6] = {};
   const float *counted_field = nullptr;
   float iso = 0.f;
@@ -297,6 +299,7 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
                  const float *box_max = nullptr) {
   const Scratch &s = ctx->s;
   ctx->coords_valid = false;
+  ctx->emitted = false;
   if (p.empty) {
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, st,
                        reinterpret_cast<uint32_t *>(d_counts), s.totals);
@@ -401,6 +404,31 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
     const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
     hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
+  }
+  HIP_TRY(hipGetLastError());
+  ctx->emitted = true;
+  ctx->emit_args = ea;
+  return MDMC_OK;
+}
+
+// Smoothing iterations over the last emit's mesh (the given outputs must be
+// the ones that emit wrote).
+int launch_smooth(mdmc_ctx *ctx, float *d_vertices, uint64_t vcap, int32_t *d_quads, uint64_t qcap,
+                  uint32_t num_smooth, mdmc_counts *d_counts, hipStream_t st) {
+  if (num_smooth == 0 || ctx->plan.empty) return MDMC_OK;
+  EmitArgs ea = ctx->emit_args;
+  ea.v.verts = d_vertices;
+  ea.v.vcap = d_vertices ? vcap : 0;
+  ea.v.quads = d_quads;
+  ea.v.qcap = d_quads ? qcap : 0;
+  ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
+  const uint64_t want = (ctx->plan.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
+  const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
+  ProfScope ps(ctx, 3, st);
+  for (uint32_t it = 0; it < num_smooth; ++it) {
+    hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
+    hipLaunchKernelGGL(k_reaverage, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
   }
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
@@ -912,6 +940,15 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
                           quad_capacity, d_counts, stream);
 }
 
+int mdmc_smooth_device(mdmc_ctx *ctx, float *d_vertices, uint64_t vertex_capacity, const int32_t *d_quads,
+                       uint64_t quad_capacity, uint32_t num_smooth, mdmc_counts *d_counts, void *stream) {
+  if (!ctx || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
+  if (num_smooth == 0) return MDMC_OK;
+  if (!ctx->emitted) return fail(MDMC_ERR_INVALID, "smoothing without a preceding emit on this context");
+  return launch_smooth(ctx, d_vertices, vertex_capacity, const_cast<int32_t *>(d_quads), quad_capacity, num_smooth,
+                       d_counts, static_cast<hipStream_t>(stream));
+}
+
 int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value) {
   if (!ctx) return fail(MDMC_ERR_INVALID, "null ctx");
   if (option < 0 || option >= kNumOptions) return fail(MDMC_ERR_INVALID, "unknown option");
@@ -1003,7 +1040,6 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
              const float xyz_max[3], float iso, uint32_t num_smooth, mdmc_mesh *out) {
   if (!out || !dims || !xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
   std::memset(out, 0, sizeof(*out));
-  if (num_smooth != 0) return fail(MDMC_ERR_INVALID, "smoothing (num_smooth > 0) is not provided");
   int dev = 0;
   int rc = current_device(&dev);
   if (rc != MDMC_OK) return rc;
@@ -1049,6 +1085,8 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
   if (h.num_quads) RUN_TRY(hipMalloc(&d_q, (size_t)h.num_quads * 4 * sizeof(int32_t)));
   rc = mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_v, h.num_vertices, d_q, h.num_quads,
                         d_counts, nullptr);
+  if (rc != MDMC_OK) { cleanup(); return rc; }
+  rc = mdmc_smooth_device(ctx, d_v, h.num_vertices, d_q, h.num_quads, num_smooth, d_counts, nullptr);
   if (rc != MDMC_OK) { cleanup(); return rc; }
   mdmc_counts h2{};
   RUN_TRY(hipMemcpy(&h2, d_counts, sizeof(h2), hipMemcpyDeviceToHost));

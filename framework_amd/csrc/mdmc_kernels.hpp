@@ -1196,6 +1196,288 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
 
 
 // ---------------------------------------------------------------------------
+// Smoothing (num_smooth > 0): smooth_edge_vertices (dmc.cu:1868-1981) then
+// calc_iso_vertices_kern again (dmc.cu:1517-1593, 2157-2169), per iteration.
+// Both run over the emitted mesh, one lane per active cell in the emit's
+// chunks.  k_smooth moves owned edge points (reading iso vertices only);
+// k_reaverage recomputes iso vertices from the stored edge points (reading
+// edge points only), so neither reads what it writes.
+//
+// The split of a convex ring is at the corner of the largest angle, taken as
+// the smallest cosine (a.b) / (sqrt(a.a) sqrt(b.b)) with IEEE sqrt and
+// division, first smallest wins (the reference takes acosf of rsqrtf-
+// normalised vectors, dmc.cu:1818-1860); everything else is the reference's
+// float operation sequence.
+
+// Position of an active cell from its segment.
+struct CellPos {
+  uint32_t xl, cfg, lut2, x, j, k, vb, qb;
+};
+
+__device__ __forceinline__ CellPos cell_pos(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
+                                            uint64_t &rec) {
+  CellPos c;
+  rec = a.slots[(size_t)s * a.cap + rank];
+  c.xl = rec_x(rec);
+  c.cfg = rec_cfg(rec);
+  c.lut2 = rec_lut2(rec);
+  const uint32_t sl = s - vol.seg_off;
+  const uint32_t xt = sl % vol.XT, row = sl / vol.XT;
+  c.j = row % vol.g.cy;
+  c.k = row / vol.g.cy;
+  c.x = xt * a.cap + c.xl;
+  const uint2 vq = a.seg_vq[s];
+  c.vb = vq.x + rec_lvb(rec) - vol.vbase;
+  c.qb = vq.y + rec_lqb(rec) - vol.qbase;
+  return c;
+}
+
+__device__ __forceinline__ uint32_t iso_count(uint32_t cfg, uint32_t lut2) {
+  const uint32_t info = c_cfg_info[cfg];
+  return lut2 ? ((info >> 3) & 7u) : (info & 7u);
+}
+
+// bipolar owned edges 6, 9, 10 of a configuration, as bits 0, 1, 2
+__device__ __forceinline__ uint32_t owned_bipolar(uint32_t cfg) {
+  return (((cfg >> 2) ^ (cfg >> 6)) & 1u) | ((((cfg >> 5) ^ (cfg >> 6)) & 1u) << 1) |
+         ((((cfg >> 7) ^ (cfg >> 6)) & 1u) << 2);
+}
+
+struct P2 {
+  float x, y;
+};
+
+// calc_cross_z_sign (dmc.cu:1741-1747)
+__device__ __forceinline__ bool cross_nonneg(P2 l, P2 m, P2 r) {
+  const float dx1 = r.x - m.x, dy1 = r.y - m.y, dx2 = l.x - m.x, dy2 = l.y - m.y;
+  return dx1 * dy2 - dx2 * dy1 >= 0.0f;
+}
+
+__device__ __forceinline__ float corner_cos(P2 l, P2 m, P2 r) {
+  const float ax = l.x - m.x, ay = l.y - m.y, bx = r.x - m.x, by = r.y - m.y;
+  const float num = ax * bx + ay * by;
+  return num / (sqrtf(ax * ax + ay * ay) * sqrtf(bx * bx + by * by));
+}
+
+// is_inside_triangle (dmc.cu:1894-1912)
+__device__ __forceinline__ bool inside_tri(P2 p0, P2 p1, P2 p2, P2 t, float &al, float &be, float &ga) {
+  const float v0x = p1.x - p0.x, v0y = p1.y - p0.y, v1x = p2.x - p0.x, v1y = p2.y - p0.y;
+  const float v2x = t.x - p0.x, v2y = t.y - p0.y;
+  const float d00 = v0x * v0x + v0y * v0y, d10 = v1x * v0x + v1y * v0y, d11 = v1x * v1x + v1y * v1y;
+  const float d20 = v2x * v0x + v2y * v0y, d21 = v2x * v1x + v2y * v1y;
+  const float inv = 1.0f / (d00 * d11 - d10 * d10);
+  be = (d11 * d20 - d10 * d21) * inv;
+  ga = (d00 * d21 - d10 * d20) * inv;
+  al = 1.0f - be - ga;
+  return (-1e-4 < (double)be) && (-1e-4 < (double)ga) && ((double)(be + ga) < 1.0 + 1e-4);
+}
+
+__device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank) {
+  uint64_t rec;
+  const CellPos c = cell_pos(a, vol, s, rank, rec);
+  const FieldDesc &g = vol.g;
+  if (c.x >= g.cx) { atomicOr(vol.status, kStatusInternal); return; }
+  const uint32_t bip = owned_bipolar(c.cfg);
+  const bool xin = c.x + 1 < g.cx, yin = c.j + 1 < g.cy, zin = c.k + 1 < g.cz;
+  const uint32_t has = ((bip & 1u) && xin && yin ? 1u : 0u) | ((bip & 2u) && xin && zin ? 2u : 0u) |
+                       ((bip & 4u) && yin && zin ? 4u : 0u);
+  if (!has) return;
+  const float x1 = vol.xt_c[c.x + 1], y1 = vol.yt_c[c.j + 1], z1 = vol.zt_c[c.k + 1];
+  uint32_t q = c.qb, ev = c.vb + iso_count(c.cfg, c.lut2);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const uint32_t e_vtx = ev;
+    if ((bip >> l) & 1u) ++ev;
+    if (!((has >> l) & 1u)) continue;
+    if ((uint64_t)q >= vol.qcap || (uint64_t)e_vtx >= vol.vcap) { atomicOr(vol.status, kStatusInternal); return; }
+    const int4 rq = reinterpret_cast<const int4 *>(vol.quads)[q++];
+    const int32_t iv[4] = {rq.x, rq.y, rq.z, rq.w};
+    const int c0 = l == 2 ? 1 : 0, c1 = l == 0 ? 1 : 2;
+    P2 P[4];
+    bool bad = false;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      bad |= iv[p] < 0 || (uint64_t)iv[p] >= vol.vcap;
+      const float *vp = vol.verts + 3ull * (uint32_t)(bad ? 0 : iv[p]);
+      P[p].x = vp[c0];
+      P[p].y = vp[c1];
+    }
+    if (bad) { atomicOr(vol.status, kStatusInternal); continue; }
+    // calc_quadrilateral_signs (dmc.cu:1749-1780)
+    uint32_t npos = 0, ipos = 0, ineg = 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (cross_nonneg(P[(p + 3) & 3], P[p], P[(p + 1) & 3])) { ++npos; ipos = p; }
+      else ineg = p;
+    }
+    if (npos == 2) continue;  // self-crossing ring (dmc.cu:1783-1786)
+    uint32_t s0;
+    if (npos == 0 || npos == 4) {
+      float best = corner_cos(P[3], P[0], P[1]);
+      s0 = 0;
+#pragma unroll
+      for (int p = 1; p < 4; ++p) {
+        const float cs = corner_cos(P[(p + 3) & 3], P[p], P[(p + 1) & 3]);
+        if (cs < best) { best = cs; s0 = p; }
+      }
+    } else {
+      s0 = npos < 2 ? ipos : ineg;  // the concave corner
+    }
+    const uint32_t s1 = (s0 + 2) & 3;
+    const P2 o = l == 0 ? P2{x1, y1} : (l == 1 ? P2{x1, z1} : P2{y1, z1});
+    float al, be, ga;
+    uint32_t ta, tb, tc;
+    if (inside_tri(P[s0], P[(s0 + 1) & 3], P[s1], o, al, be, ga)) {
+      ta = s0; tb = (s0 + 1) & 3; tc = s1;
+    } else if (inside_tri(P[s1], P[(s1 + 1) & 3], P[s0], o, al, be, ga)) {
+      ta = s1; tb = (s1 + 1) & 3; tc = s0;
+    } else {
+      continue;
+    }
+    const float *va = vol.verts + 3ull * (uint32_t)iv[ta], *vb = vol.verts + 3ull * (uint32_t)iv[tb],
+                *vc = vol.verts + 3ull * (uint32_t)iv[tc];
+    float *E = vol.verts + 3ull * e_vtx;
+    const float r0 = al * va[0] + be * vb[0] + ga * vc[0];
+    const float r1 = al * va[1] + be * vb[1] + ga * vc[1];
+    const float r2 = al * va[2] + be * vb[2] + ga * vc[2];
+    E[0] = r0;
+    E[1] = r1;
+    E[2] = r2;
+  }
+}
+
+// Iso vertices from the stored edge points: for each edge of the cell in
+// order 0..11 with a vertex and an in-grid owner, the owner's stored point
+// (owner cells n = 0 (-1,0,0), 1 (0,-1,0), 2 (0,0,-1), 3 (-1,-1,0),
+// 4 (-1,0,-1), 5 (0,-1,-1), found like the quad ring cells), summed per
+// group from -0.0 and divided by the count; groups with no point keep their
+// value.
+__device__ __forceinline__ void reaverage_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
+                                               const uint64_t *s_lab, float *s_acc) {
+  uint64_t rec;
+  const CellPos c = cell_pos(a, vol, s, rank, rec);
+  const FieldDesc &g = vol.g;
+  const uint32_t cap = a.cap;
+  if (c.x >= g.cx || c.xl >= cap) { atomicOr(vol.status, kStatusInternal); return; }
+  const uint64_t labels = s_lab[c.lut2 * 256 + c.cfg];
+  const uint32_t niso = iso_count(c.cfg, c.lut2);
+  // owner cell (7 = this cell) and the owner's edge shift (0: e6, 1: e9, 2: e10)
+  constexpr uint8_t OWN_N[12] = {5, 2, 2, 4, 3, 1, 7, 0, 1, 7, 7, 0};
+  constexpr uint8_t OWN_L[12] = {2, 1, 2, 1, 0, 0, 0, 0, 2, 1, 2, 1};
+  constexpr uint8_t NEED[12] = {6, 4, 4, 5, 3, 2, 0, 1, 2, 0, 0, 1};  // bit 0: x>0, 1: y>0, 2: z>0
+  const uint32_t have = (c.x > 0 ? 1u : 0u) | (c.j > 0 ? 2u : 0u) | (c.k > 0 ? 4u : 0u);
+  uint32_t need = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
+    if (lab != 0xFu && (NEED[e] & have) == NEED[e] && OWN_N[e] < 7) need |= 1u << OWN_N[e];
+  }
+  const bool wrap = c.xl == 0;  // -x lies in the previous x-tile's segment
+  const uint32_t sx = wrap ? s - 1u : s, xlm = wrap ? cap - 1u : c.xl - 1u;
+  const uint32_t dy = vol.XT, dz = vol.XT * g.cy;
+  const uint32_t ns0[6] = {sx, s - dy, s - dz, sx - dy, sx - dz, s - dy - dz};
+  const uint32_t nx0[6] = {xlm, c.xl, c.xl, xlm, xlm, c.xl};
+  uint32_t ns[6], nxl[6], vq0[6];
+  uint4 gm[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const bool use = (need >> n) & 1u;
+    ns[n] = use ? ns0[n] : s;
+    nxl[n] = use ? nx0[n] : c.xl;
+    gm[n] = a.gmap[(size_t)ns[n] * (cap >> 6) + (nxl[n] >> 6)];
+    vq0[n] = a.seg_vq[ns[n]].x;
+  }
+  const uint32_t *slot_lo = reinterpret_cast<const uint32_t *>(a.slots);
+  uint32_t own_base[8], own_bip[8];  // first edge point of each owner, its bipolar bits
+  bool ok = true;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const uint64_t mask = (uint64_t)gm[n].x | ((uint64_t)gm[n].y << 32);
+    const uint32_t bit = nxl[n] & 63u;
+    const uint32_t r = gm[n].z + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
+    const bool act = ((mask >> bit) & 1ull) && r < cap;
+    if (((need >> n) & 1u) && !act) ok = false;
+    const uint32_t m = slot_lo[2 * ((size_t)ns[n] * cap + (act ? r : rank))];
+    own_base[n] = vq0[n] + rec_lvb(m) - vol.vbase + iso_count(rec_cfg(m), rec_lut2(m));
+    own_bip[n] = owned_bipolar(rec_cfg(m));
+  }
+  own_base[6] = 0;
+  own_bip[6] = 0;
+  own_base[7] = c.vb + niso;
+  own_bip[7] = owned_bipolar(c.cfg);
+  constexpr uint32_t T = kEmitThreads;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((uint32_t)q < niso) {
+      s_acc[(3 * q + 0) * T] = -0.0f;
+      s_acc[(3 * q + 1) * T] = -0.0f;
+      s_acc[(3 * q + 2) * T] = -0.0f;
+    }
+  }
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
+    if (lab == 0xFu || (NEED[e] & have) != NEED[e]) continue;
+    const uint32_t on = OWN_N[e], ol = OWN_L[e];
+    if (!((own_bip[on] >> ol) & 1u)) { ok = false; continue; }
+    const uint32_t idx = own_base[on] + (uint32_t)__popc(own_bip[on] & ((1u << ol) - 1u));
+    if ((uint64_t)idx >= vol.vcap) { ok = false; continue; }
+    const float *pnt = vol.verts + 3ull * idx;
+    float *acc = s_acc + 3 * lab * T;
+    acc[0] = acc[0] + pnt[0];
+    acc[T] = acc[T] + pnt[1];
+    acc[2 * T] = acc[2 * T] + pnt[2];
+    cnt += 1u << (8 * lab);
+  }
+  if ((uint64_t)c.vb + niso > vol.vcap) { atomicOr(vol.status, kStatusVertexOverflow); return; }
+  float *out = vol.verts + 3ull * c.vb;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((uint32_t)q < niso) {
+      const uint32_t n = (cnt >> (8 * q)) & 0xFFu;
+      if (n) {
+        const float d = (float)n;
+        out[3 * q + 0] = s_acc[(3 * q + 0) * T] / d;
+        out[3 * q + 1] = s_acc[(3 * q + 1) * T] / d;
+        out[3 * q + 2] = s_acc[(3 * q + 2) * T] / d;
+      }
+    }
+  }
+  if (!ok) atomicOr(vol.status, kStatusLookup);
+}
+
+__global__ __launch_bounds__(kEmitThreads) void k_smooth(EmitArgs a) {
+  const uint32_t lane = lane_id(), wave = wave_id();
+  const Totals tot = *a.totals;
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
+  const ChunkIter it = chunk_iter(num_chunks, wave);
+  for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
+    uint32_t s, rank;
+    if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
+    smooth_cell(a, a.v, s, rank);
+  }
+}
+
+__global__ __launch_bounds__(kEmitThreads) void k_reaverage(EmitArgs a) {
+  __shared__ uint64_t s_lab[2 * 256];
+  __shared__ float s_acc[12 * kEmitThreads];
+  for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
+  __syncthreads();
+  const uint32_t lane = lane_id(), wave = wave_id();
+  const Totals tot = *a.totals;
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
+  const ChunkIter it = chunk_iter(num_chunks, wave);
+  for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
+    uint32_t s, rank;
+    if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
+    reaverage_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batch count and emit.
 
 struct BatchCountArgs {

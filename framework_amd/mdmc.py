@@ -112,6 +112,8 @@ def lib():
     L.mdmc_extract_device.argtypes = [_VP, _VP, _U3, _F3, _F3, ctypes.c_float, _VP, ctypes.c_uint64, _VP,
                                       ctypes.c_uint64, _VP, _VP]
     L.mdmc_extract_device.restype = ctypes.c_int
+    L.mdmc_smooth_device.argtypes = [_VP, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64, ctypes.c_uint32, _VP, _VP]
+    L.mdmc_smooth_device.restype = ctypes.c_int
     L.mdmc_cells_device.argtypes = [_VP, _VP, _VP, _VP, _VP, ctypes.c_uint64, _VP]
     L.mdmc_cells_device.restype = ctypes.c_int
     L.mdmc_field_blobs_device.argtypes = [_VP, _VP, _U3, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double,
@@ -254,8 +256,19 @@ class Context:
         _check(lib().mdmc_emit_device(self._h, _VP(field.data_ptr()), mn, mx, _VP(vptr), vcap, _VP(qptr), qcap,
                                       _VP(c.data_ptr()), self._stream(stream)), "mdmc_emit_device")
 
-    def extract(self, field, iso: float, xyz_min=None, xyz_max=None, stream=None):
-        """Count, size the outputs exactly, emit.  Returns (vertices (V,3), quads (Q,4), counts)."""
+    def smooth(self, vertices, quads, num_smooth: int, counts=None, stream=None) -> None:
+        """num_smooth smoothing iterations over the mesh the last emit of this
+        context wrote into (vertices, quads)."""
+        c = self._counts if counts is None else counts
+        vptr = vertices.data_ptr() if vertices is not None and vertices.numel() else None
+        qptr = quads.data_ptr() if quads is not None and quads.numel() else None
+        _check(lib().mdmc_smooth_device(self._h, _VP(vptr), vertices.numel() // 3 if vptr else 0, _VP(qptr),
+                                        quads.numel() // 4 if qptr else 0, int(num_smooth), _VP(c.data_ptr()),
+                                        self._stream(stream)), "mdmc_smooth_device")
+
+    def extract(self, field, iso: float, xyz_min=None, xyz_max=None, stream=None, num_smooth: int = 0):
+        """Count, size the outputs exactly, emit (then num_smooth smoothing
+        iterations).  Returns (vertices (V,3), quads (Q,4), counts)."""
         torch = self._torch
         self.count(field, iso, stream=stream)
         (stream or torch.cuda.current_stream()).synchronize()
@@ -266,6 +279,8 @@ class Context:
         verts = torch.empty((V, 3), dtype=torch.float32, device=field.device)
         quads = torch.empty((Q, 4), dtype=torch.int32, device=field.device)
         self.emit(field, verts, quads, xyz_min, xyz_max, stream=stream)
+        if num_smooth:
+            self.smooth(verts, quads, num_smooth, stream=stream)
         (stream or torch.cuda.current_stream()).synchronize()
         cnt = self.read_counts()
         if cnt["status"]:

@@ -6,7 +6,9 @@
 //     instead of printing and calling exit(1) (include/cuda_includes.h:13-22);
 //   * an empty surface returns empty vectors (the reference exits,
 //     dmc.cu:2113-2114);
-//   * num_smooth > 0 throws (the smoothing pass is not provided);
+//   * num_smooth > 0 smooths as the reference does, except that a convex
+//     ring's split corner is chosen by comparing cosines (IEEE sqrt and
+//     division) instead of acosf/rsqrtf angles;
 //   * float3/uint3 are this header's own 12-byte PODs (the reference uses the
 //     CUDA vector types of the same layout); they are also declared in the
 //     global namespace, as the reference's callers use them unqualified

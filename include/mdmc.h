@@ -76,9 +76,9 @@ int mdmc_cleanup(void);
 
 /* Replaces dmc::run_dmc(vertices, triangles, grid, xyz_min, xyz_max, iso,
  * num_smooth) (include/dmc.hpp:56-57, dmc.cu:2069-2208).  `field` is host
- * memory, dims[3] are point counts (x fastest).  num_smooth must be 0
- * (smoothing is not provided; non-zero returns MDMC_ERR_INVALID).  The mesh
- * is allocated by the library. */
+ * memory, dims[3] are point counts (x fastest).  num_smooth smoothing
+ * iterations follow the extraction (dmc.cu:2157-2169).  The mesh is
+ * allocated by the library. */
 int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
              const float xyz_max[3], float iso, uint32_t num_smooth, mdmc_mesh *out);
 
@@ -118,6 +118,17 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
                         const float xyz_min[3], const float xyz_max[3], float iso,
                         float *d_vertices, uint64_t vertex_capacity, int32_t *d_quads,
                         uint64_t quad_capacity, mdmc_counts *d_counts, void *stream);
+
+/* Smoothing iterations over the mesh the last mdmc_emit_device /
+ * mdmc_extract_device of this context wrote (pass the same outputs): each
+ * iteration is smooth_edge_vertices (dmc.cu:1868-1981) followed by
+ * calc_iso_vertices_kern (dmc.cu:1517-1593), as in run_dmc's num_smooth loop
+ * (dmc.cu:2157-2169).  Quads are unchanged.  The split corner of a convex
+ * ring is found by comparing cosines with IEEE sqrt and division (the
+ * reference uses acosf and rsqrtf).  Stream-ordered; status bits in
+ * d_counts->status. */
+int mdmc_smooth_device(mdmc_ctx *ctx, float *d_vertices, uint64_t vertex_capacity, const int32_t *d_quads,
+                       uint64_t quad_capacity, uint32_t num_smooth, mdmc_counts *d_counts, void *stream);
 
 /* Per-active-cell records of the last count on `ctx`, in ascending cell
  * order: cell index1D, config, flags (bit 0 = uses the variant-1 table) and

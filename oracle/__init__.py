@@ -14,6 +14,9 @@ Parity status of each oracle function (how it is pinned is in the tests):
                         FP64 twin bound, symmetry
   quads                 pinned: mesh invariants (closed, oriented 2-manifold,
                         Euler characteristic), hand-built rings
+  smoothing             pinned: hand-built rings (convex, concave, crossing,
+                        outside), closed form on planes, no-op when no
+                        triangle holds the edge
 """
 from __future__ import annotations
 
@@ -62,6 +65,9 @@ def lib():
         u3 = ctypes.POINTER(ctypes.c_uint32)
         L.oracle_dmc_run.argtypes = [f3, u3, f3, f3, ctypes.c_float, ctypes.c_int, ctypes.POINTER(_Result)]
         L.oracle_dmc_run.restype = ctypes.c_int
+        L.oracle_dmc_run_smooth.argtypes = [f3, u3, f3, f3, ctypes.c_float, ctypes.c_int, ctypes.c_uint32,
+                                             ctypes.POINTER(_Result)]
+        L.oracle_dmc_run_smooth.restype = ctypes.c_int
         L.oracle_result_free.argtypes = [ctypes.POINTER(_Result)]
         L.oracle_luts_bytes.argtypes = [ctypes.POINTER(ctypes.c_uint8)]
         L.oracle_field_sphere.argtypes = [f3, u3, ctypes.POINTER(ctypes.c_double), ctypes.c_double]
@@ -86,8 +92,9 @@ def luts_bytes() -> bytes:
     return bytes(buf)
 
 
-def run(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, fp64: bool = False) -> dict:
-    """Extract the mesh of ``field`` (shape (Z, Y, X), float32, x fastest).
+def run(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, fp64: bool = False, num_smooth: int = 0) -> dict:
+    """Extract the mesh of ``field`` (shape (Z, Y, X), float32, x fastest),
+    with ``num_smooth`` smoothing iterations.
 
     Defaults put vertices in voxel units (min 0, max = cells per axis)."""
     field = np.ascontiguousarray(field, dtype=np.float32)
@@ -99,8 +106,8 @@ def run(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, fp64: bool = 
     mn = np.asarray(xyz_min, dtype=np.float32)
     mx = np.asarray(xyz_max, dtype=np.float32)
     r = _Result()
-    rc = lib().oracle_dmc_run(_fptr(field), _dims((X, Y, Z)), _fptr(mn), _fptr(mx),
-                              ctypes.c_float(iso), int(fp64), ctypes.byref(r))
+    rc = lib().oracle_dmc_run_smooth(_fptr(field), _dims((X, Y, Z)), _fptr(mn), _fptr(mx),
+                                     ctypes.c_float(iso), int(fp64), int(num_smooth), ctypes.byref(r))
     if rc != 0:
         raise MemoryError("oracle_dmc_run failed")
     try:

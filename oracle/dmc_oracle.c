@@ -27,6 +27,7 @@
  */
 #include "dmc_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -129,8 +130,197 @@ static uint32_t edge_vertex_index(const cellrec *r, const mdmc_luts *L, int e) {
   return r->vertex_begin + L->num_vertex[lut2][r->cfg] + (uint32_t)off;
 }
 
-int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
-                   const float xyz_max[3], float iso, int want_fp64, oracle_result *out) {
+/* Pass 6, calc_iso_vertices_kern (dmc.cu:1517-1593): each iso vertex is the
+ * mean of the edge points of its group, read back from the owners' slots in
+ * the vertex array (the stored points: after smoothing, the moved ones);
+ * edges whose owner lies outside the grid are skipped. */
+static void reaverage(oracle_result *out, const cellrec *rec, const uint32_t *map, uint32_t A, uint32_t cx,
+                      uint32_t cy, uint32_t cz, const mdmc_luts *L, int want_fp64, int first) {
+  uint32_t a;
+  (void)cz;
+  for (a = 0; a < A; ++a) {
+    uint32_t id = rec[a].index1d;
+    int64_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
+    int lut2 = (rec[a].info >> 7) & 1, e, m;
+    uint8_t cnt[4] = {0, 0, 0, 0};
+    int nv = L->num_vertex[lut2][rec[a].cfg];
+    for (e = 0; e < 12; ++e) {
+      uint8_t lab = L->edge_label[lut2][rec[a].cfg][e];
+      int64_t oi, oj, ok;
+      uint32_t om, src, dst;
+      if (lab == MDMC_NO_VERTEX) continue;
+      oi = i + EDGE_OWNER_D[e][0]; oj = j + EDGE_OWNER_D[e][1]; ok = k + EDGE_OWNER_D[e][2];
+      if (oi < 0 || oj < 0 || ok < 0) continue;
+      om = map[((size_t)ok * cy + (size_t)oj) * cx + (size_t)oi];
+      if (om == INVALID_U32 || !((rec[om].info >> edge_shift(EDGE_OWNER_E[e])) & 1)) {
+        if (first) out->num_anomalies++;
+        continue;
+      }
+      src = edge_vertex_index(&rec[om], L, EDGE_OWNER_E[e]);
+      dst = rec[a].vertex_begin + lab;
+      if (cnt[lab] == 0) {
+        out->vertices[3 * (size_t)dst + 0] = out->vertices[3 * (size_t)src + 0];
+        out->vertices[3 * (size_t)dst + 1] = out->vertices[3 * (size_t)src + 1];
+        out->vertices[3 * (size_t)dst + 2] = out->vertices[3 * (size_t)src + 2];
+        if (want_fp64) {
+          out->vertices64[3 * (size_t)dst + 0] = out->vertices64[3 * (size_t)src + 0];
+          out->vertices64[3 * (size_t)dst + 1] = out->vertices64[3 * (size_t)src + 1];
+          out->vertices64[3 * (size_t)dst + 2] = out->vertices64[3 * (size_t)src + 2];
+        }
+      } else {
+        out->vertices[3 * (size_t)dst + 0] += out->vertices[3 * (size_t)src + 0];
+        out->vertices[3 * (size_t)dst + 1] += out->vertices[3 * (size_t)src + 1];
+        out->vertices[3 * (size_t)dst + 2] += out->vertices[3 * (size_t)src + 2];
+        if (want_fp64) {
+          out->vertices64[3 * (size_t)dst + 0] += out->vertices64[3 * (size_t)src + 0];
+          out->vertices64[3 * (size_t)dst + 1] += out->vertices64[3 * (size_t)src + 1];
+          out->vertices64[3 * (size_t)dst + 2] += out->vertices64[3 * (size_t)src + 2];
+        }
+      }
+      ++cnt[lab];
+    }
+    for (m = 0; m < nv; ++m) {
+      uint32_t dst = rec[a].vertex_begin + (uint32_t)m;
+      int c;
+      if (cnt[m]) {
+        for (c = 0; c < 3; ++c) {
+          out->vertices[3 * (size_t)dst + c] /= (float)cnt[m];
+          if (want_fp64) out->vertices64[3 * (size_t)dst + c] /= (double)cnt[m];
+        }
+      } else {
+        union { uint32_t u; float f; } qnan;
+        qnan.u = 0x7FC00000u;
+        if (first) out->num_unset_vertices++;
+        for (c = 0; c < 3; ++c) {
+          out->vertices[3 * (size_t)dst + c] = qnan.f;
+          if (want_fp64) out->vertices64[3 * (size_t)dst + c] = (double)qnan.f;
+        }
+      }
+    }
+  }
+
+}
+
+/* ---- smoothing (num_smooth > 0): smooth_edge_vertices (dmc.cu:1868-1981)
+ * followed by calc_iso_vertices_kern again, num_smooth times
+ * (dmc.cu:2157-2169).  For each owned bipolar edge whose ring lies in the
+ * grid, the ring's four iso vertices (the quad, in its ccw/cw order) are
+ * projected on the plane across the edge (e6: x,y; e9: x,z; e10: y,z); if
+ * the quadrilateral is not self-crossing, it is split at a diagonal and the
+ * edge point moves to the barycentric blend of the triangle that contains
+ * the edge's trace, when one does.
+ *
+ * Defined here where the reference depends on CUDA approximations: the split
+ * of a convex quadrilateral is at the corner of the largest angle, found as
+ * the smallest cosine (a.b) / (sqrt(a.a) sqrt(b.b)) with IEEE sqrt and
+ * division (first smallest wins), instead of acosf of rsqrtf-normalised
+ * vectors (dmc.cu:1818-1860); every other operation is the reference's, one
+ * correctly rounded float operation each, no contraction. */
+
+typedef struct pt2 {
+  float x, y;
+} pt2;
+
+/* calc_cross_z_sign (dmc.cu:1741-1747): +1 when the z of
+ * (right - mid) x (left - mid) is >= 0. */
+static int cross_sign(pt2 l, pt2 m, pt2 r) {
+  float dx1 = r.x - m.x, dy1 = r.y - m.y;
+  float dx2 = l.x - m.x, dy2 = l.y - m.y;
+  float cz = dx1 * dy2 - dx2 * dy1;
+  return cz >= 0.0f ? 1 : -1;
+}
+
+/* cosine of the corner angle at m */
+static float corner_cos(pt2 l, pt2 m, pt2 r) {
+  float ax = l.x - m.x, ay = l.y - m.y, bx = r.x - m.x, by = r.y - m.y;
+  float num = ax * bx + ay * by;
+  float la = sqrtf(ax * ax + ay * ay), lb = sqrtf(bx * bx + by * by);
+  return num / (la * lb);
+}
+
+/* is_inside_triangle (dmc.cu:1894-1912): barycentric (alpha, beta, gamma)
+ * of t in (p0, p1, p2), inside with a 1e-4 margin (double comparisons). */
+static int inside_tri(pt2 p0, pt2 p1, pt2 p2, pt2 t, float *al, float *be, float *ga) {
+  float v0x = p1.x - p0.x, v0y = p1.y - p0.y;
+  float v1x = p2.x - p0.x, v1y = p2.y - p0.y;
+  float v2x = t.x - p0.x, v2y = t.y - p0.y;
+  float d00 = v0x * v0x + v0y * v0y;
+  float d10 = v1x * v0x + v1y * v0y;
+  float d11 = v1x * v1x + v1y * v1y;
+  float d20 = v2x * v0x + v2y * v0y;
+  float d21 = v2x * v1x + v2y * v1y;
+  float inv = d00 * d11 - d10 * d10;
+  float beta, gamma;
+  inv = 1.0f / inv;
+  beta = (d11 * d20 - d10 * d21) * inv;
+  gamma = (d00 * d21 - d10 * d20) * inv;
+  *be = beta;
+  *ga = gamma;
+  *al = 1.0f - beta - gamma;
+  return (-1e-4 < (double)beta) && (-1e-4 < (double)gamma) && ((double)(beta + gamma) < 1.0 + 1e-4);
+}
+
+static void smooth_pass(oracle_result *out, const cellrec *rec, uint32_t A, uint32_t cx, uint32_t cy,
+                        uint32_t cz, const mdmc_luts *L, const float xyz_min[3], const float range[3]) {
+  uint32_t a;
+  float *V = out->vertices;
+  for (a = 0; a < A; ++a) {
+    uint32_t id = rec[a].index1d;
+    uint32_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
+    uint32_t q = out->quad_begin[a];
+    int l;
+    for (l = 0; l < 3; ++l) {
+      int e = LOCAL_EDGES[l], c0 = l == 2 ? 1 : 0, c1 = l == 0 ? 1 : 2, p, npos = 0, nneg = 0, ipos = 0, ineg = 0;
+      int s0, s1;
+      const int32_t *iv;
+      pt2 P[4], o;
+      float al, be, ga;
+      float *E;
+      if (!((rec[a].info >> l) & 1)) continue;
+      if (e == 6 && !(i + 1 < cx && j + 1 < cy)) continue;
+      if (e == 9 && !(i + 1 < cx && k + 1 < cz)) continue;
+      if (e == 10 && !(j + 1 < cy && k + 1 < cz)) continue;
+      iv = out->quads + 4 * (size_t)q++;
+      for (p = 0; p < 4; ++p) {
+        P[p].x = V[3 * (size_t)iv[p] + c0];
+        P[p].y = V[3 * (size_t)iv[p] + c1];
+      }
+      /* calc_quadrilateral_signs (dmc.cu:1749-1780) */
+      for (p = 0; p < 4; ++p) {
+        if (cross_sign(P[(p + 3) % 4], P[p], P[(p + 1) % 4]) > 0) { ++npos; ipos = p; }
+        else { ++nneg; ineg = p; }
+      }
+      if (npos == nneg) continue; /* self-crossing (dmc.cu:1783-1786) */
+      if (npos == 0 || nneg == 0) {
+        float best = corner_cos(P[3], P[0], P[1]);
+        s0 = 0;
+        for (p = 1; p < 4; ++p) {
+          float c = corner_cos(P[(p + 3) % 4], P[p], P[(p + 1) % 4]);
+          if (c < best) { best = c; s0 = p; }
+        }
+      } else {
+        s0 = npos < nneg ? ipos : ineg; /* the concave corner */
+      }
+      s1 = (s0 + 2) % 4;
+      o.x = ijk_to_xyz(l == 2 ? j + 1 : i + 1, l == 2 ? cy : cx, range[c0], xyz_min[c0]);
+      o.y = l == 0 ? ijk_to_xyz(j + 1, cy, range[1], xyz_min[1]) : ijk_to_xyz(k + 1, cz, range[2], xyz_min[2]);
+      E = V + 3 * (size_t)edge_vertex_index(&rec[a], L, e);
+      if (inside_tri(P[s0], P[(s0 + 1) % 4], P[s1], o, &al, &be, &ga)) {
+        const float *va = V + 3 * (size_t)iv[s0], *vb = V + 3 * (size_t)iv[(s0 + 1) % 4], *vc = V + 3 * (size_t)iv[s1];
+        int c;
+        for (c = 0; c < 3; ++c) E[c] = al * va[c] + be * vb[c] + ga * vc[c];
+      } else if (inside_tri(P[s1], P[(s1 + 1) % 4], P[s0], o, &al, &be, &ga)) {
+        const float *va = V + 3 * (size_t)iv[s1], *vb = V + 3 * (size_t)iv[(s1 + 1) % 4], *vc = V + 3 * (size_t)iv[s0];
+        int c;
+        for (c = 0; c < 3; ++c) E[c] = al * va[c] + be * vb[c] + ga * vc[c];
+      }
+    }
+  }
+}
+
+int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const float xyz_min[3],
+                          const float xyz_max[3], float iso, int want_fp64, uint32_t num_smooth,
+                          oracle_result *out) {
   const mdmc_luts *L = luts();
   uint32_t X = dims[0], Y = dims[1], Z = dims[2];
   uint32_t cx, cy, cz;
@@ -298,69 +488,8 @@ int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_m
     }
   }
 
-  /* Pass 6: calc_iso_vertices_kern (dmc.cu:1517-1593): each iso vertex is the
-   * mean of the edge points of its group, read back from the owners' slots in
-   * the vertex array; edges whose owner lies outside the grid are skipped. */
-  for (a = 0; a < A; ++a) {
-    uint32_t id = rec[a].index1d;
-    int64_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
-    int lut2 = (rec[a].info >> 7) & 1, e, m;
-    uint8_t cnt[4] = {0, 0, 0, 0};
-    int nv = L->num_vertex[lut2][rec[a].cfg];
-    for (e = 0; e < 12; ++e) {
-      uint8_t lab = L->edge_label[lut2][rec[a].cfg][e];
-      int64_t oi, oj, ok;
-      uint32_t om, src, dst;
-      if (lab == MDMC_NO_VERTEX) continue;
-      oi = i + EDGE_OWNER_D[e][0]; oj = j + EDGE_OWNER_D[e][1]; ok = k + EDGE_OWNER_D[e][2];
-      if (oi < 0 || oj < 0 || ok < 0) continue;
-      om = map[((size_t)ok * cy + (size_t)oj) * cx + (size_t)oi];
-      if (om == INVALID_U32 || !((rec[om].info >> edge_shift(EDGE_OWNER_E[e])) & 1)) {
-        out->num_anomalies++;
-        continue;
-      }
-      src = edge_vertex_index(&rec[om], L, EDGE_OWNER_E[e]);
-      dst = rec[a].vertex_begin + lab;
-      if (cnt[lab] == 0) {
-        out->vertices[3 * (size_t)dst + 0] = out->vertices[3 * (size_t)src + 0];
-        out->vertices[3 * (size_t)dst + 1] = out->vertices[3 * (size_t)src + 1];
-        out->vertices[3 * (size_t)dst + 2] = out->vertices[3 * (size_t)src + 2];
-        if (want_fp64) {
-          out->vertices64[3 * (size_t)dst + 0] = out->vertices64[3 * (size_t)src + 0];
-          out->vertices64[3 * (size_t)dst + 1] = out->vertices64[3 * (size_t)src + 1];
-          out->vertices64[3 * (size_t)dst + 2] = out->vertices64[3 * (size_t)src + 2];
-        }
-      } else {
-        out->vertices[3 * (size_t)dst + 0] += out->vertices[3 * (size_t)src + 0];
-        out->vertices[3 * (size_t)dst + 1] += out->vertices[3 * (size_t)src + 1];
-        out->vertices[3 * (size_t)dst + 2] += out->vertices[3 * (size_t)src + 2];
-        if (want_fp64) {
-          out->vertices64[3 * (size_t)dst + 0] += out->vertices64[3 * (size_t)src + 0];
-          out->vertices64[3 * (size_t)dst + 1] += out->vertices64[3 * (size_t)src + 1];
-          out->vertices64[3 * (size_t)dst + 2] += out->vertices64[3 * (size_t)src + 2];
-        }
-      }
-      ++cnt[lab];
-    }
-    for (m = 0; m < nv; ++m) {
-      uint32_t dst = rec[a].vertex_begin + (uint32_t)m;
-      int c;
-      if (cnt[m]) {
-        for (c = 0; c < 3; ++c) {
-          out->vertices[3 * (size_t)dst + c] /= (float)cnt[m];
-          if (want_fp64) out->vertices64[3 * (size_t)dst + c] /= (double)cnt[m];
-        }
-      } else {
-        union { uint32_t u; float f; } qnan;
-        qnan.u = 0x7FC00000u;
-        out->num_unset_vertices++;
-        for (c = 0; c < 3; ++c) {
-          out->vertices[3 * (size_t)dst + c] = qnan.f;
-          if (want_fp64) out->vertices64[3 * (size_t)dst + c] = (double)qnan.f;
-        }
-      }
-    }
-  }
+  /* Pass 6 */
+  reaverage(out, rec, map, A, cx, cy, cz, L, want_fp64, 1);
 
   /* Pass 7: calc_num_triangles_per_voxel_kern (dmc.cu:1983-2008) with
    * circular_edge_exceed_boundary (dmc.cu:1696-1708), then the offsets scan
@@ -416,6 +545,15 @@ int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_m
     }
   }
 
+  /* smoothing iterations; the FP64 twin is left unsmoothed */
+  {
+    uint32_t it;
+    for (it = 0; it < num_smooth; ++it) {
+      smooth_pass(out, rec, A, cx, cy, cz, L, xyz_min, range);
+      reaverage(out, rec, map, A, cx, cy, cz, L, 0, 0);
+    }
+  }
+
   for (a = 0; a < A; ++a) {
     out->cell_index[a] = rec[a].index1d;
     out->cfg[a] = rec[a].cfg;
@@ -425,6 +563,11 @@ int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_m
   free(map);
   free(rec);
   return 0;
+}
+
+int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
+                   const float xyz_max[3], float iso, int want_fp64, oracle_result *out) {
+  return oracle_dmc_run_smooth(field, dims, xyz_min, xyz_max, iso, want_fp64, 0, out);
 }
 
 void oracle_result_free(oracle_result *r) {

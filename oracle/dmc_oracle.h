@@ -39,6 +39,11 @@ typedef struct oracle_result {
  * want_fp64 != 0 also fills vertices64.  Returns 0 on success. */
 int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
                    const float xyz_max[3], float iso, int want_fp64, oracle_result *out);
+/* The same with num_smooth smoothing iterations (the reference's num_smooth,
+ * dmc.cu:2157-2169); vertices64 is not smoothed. */
+int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const float xyz_min[3],
+                          const float xyz_max[3], float iso, int want_fp64, uint32_t num_smooth,
+                          oracle_result *out);
 void oracle_result_free(oracle_result *r);
 
 /* The reference tables rebuilt by the shared builder, for hashing in tests:

@@ -33,12 +33,10 @@ def test_status_strings():
     assert L.mdmc_status_string(12345) == b"unknown status"
 
 
-def test_smoothing_is_rejected_before_touching_the_device():
-    import numpy as np
-
-    with pytest.raises(fa.MDMCError) as e:
-        fa.run_host(np.zeros((4, 4, 4), np.float32), 0.0, num_smooth=3)
-    assert e.value.code == -1
+def test_smoothing_needs_a_context():
+    raw = ctypes.CDLL(fa.mdmc.LIB_PATH)
+    null = ctypes.c_void_p(None)
+    assert raw.mdmc_smooth_device(null, null, ctypes.c_uint64(0), null, ctypes.c_uint64(0), 1, null, null) == -1
 
 
 def test_null_arguments_are_rejected():
