@@ -12,7 +12,10 @@ second).  `roofline` is for the dominant kernel (longest average launch),
 timed live with HIP events the library records around each launch on its
 stream.  `cpu_baseline` is the C oracle (single thread) on the same field.
 
-Three more legs go into the same line:
+Four more legs go into the same line:
+  "host_path": the drop-in host entry (mdmc_run, replacing run_dmc) on the
+      same 512^3 field, host field in and host mesh out, so PCIe transfers
+      are included (never the headline value).
   "dense": the surface-dense case, 1024^3 gyroid (period 24) + noise 0.05,
       iso 0: ms per extraction, Gvoxels/s and the pipeline's algorithmic
       bytes (field + vertices + quads) against the HBM peak.
@@ -60,6 +63,7 @@ def main() -> int:
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-served", action="store_true")
     ap.add_argument("--skip-dense", action="store_true")
+    ap.add_argument("--skip-host", action="store_true")
     ap.add_argument("--dense-only", action="store_true", help="only the surface-dense leg (prints its JSON)")
     ap.add_argument("--served-requests", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
@@ -182,6 +186,7 @@ def main() -> int:
                "sample": f"full 512^3 blob field, {len(times)} run(s), median {tmed:.3f} s "
                          f"(C oracle restating dmc::run_dmc, one thread, host of the GPU box)"}
 
+    host = None if args.skip_host else bench_host(fa, field.cpu().numpy(), V, Q)
     dense = None if args.skip_dense else bench_dense(fa, torch)
     batched = None if args.skip_batched else bench_batched(fa, torch, tune=not args.no_tune)
     served = None if args.skip_served else bench_served(fa, torch, args.served_requests)
@@ -210,6 +215,7 @@ def main() -> int:
         "pipeline": {"algorithmic_bytes": B, "achieved_gbs": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
                      "kernel_avg_ms": per_kernel},
         "cpu_baseline": cpu,
+        "host_path": host,
         "dense": dense,
         "batched": batched,
         "served": served,
@@ -217,6 +223,27 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def bench_host(fa, fnp, V: int, Q: int, reps: int = 5) -> dict:
+    """The drop-in host entry (mdmc_run, the run_dmc replacement) on the same
+    512^3 field: host field in, host mesh out, PCIe transfers included."""
+    import numpy as np
+
+    fa.run_host(fnp, ISO)
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        v, q = fa.run_host(fnp, ISO)
+        times.append(time.perf_counter() - t)
+    assert len(v) == V and len(q) == Q
+    ms = float(np.median(times)) * 1e3
+    N = fnp.size
+    out = {"entry": "mdmc_run (host field in, host mesh out)", "ms_per_extraction": ms,
+           "gvoxels_per_s": N / (ms * 1e-3) / 1e9, "h2d_bytes": 4 * N, "d2h_bytes": 12 * V + 16 * Q,
+           "transfer_gbs": (4 * N + 12 * V + 16 * Q) / (ms * 1e-3) / 1e9}
+    log(f"host path: {out}")
+    return out
 
 
 def bench_dense(fa, torch, n: int = 1024, steps: int = 5) -> dict:

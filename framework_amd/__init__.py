@@ -6,7 +6,7 @@ package holds its sources (``csrc/``), the build step (``build.py``) and thin
 Python bindings (``mdmc.py``).
 """
 from .mdmc import (Context, MDMCError, cleanup, declared_functions, lib, run_dmc, run_host, setup,
-                   triangles_from_quads)
+                   triangles_from_quads, write_obj)
 
 __all__ = ["Context", "MDMCError", "cleanup", "declared_functions", "lib", "run_dmc", "run_host", "setup",
-           "triangles_from_quads"]
+           "triangles_from_quads", "write_obj"]

@@ -17,6 +17,7 @@
 #include "mdmc_fields.h"
 #include "mdmc_kernels.hpp"
 #include "mdmc_internal.h"
+#include "mdmc_io.h"
 
 namespace {
 
@@ -148,6 +149,9 @@ struct mdmc_ctx {
   bool counted = false;
   bool coords_valid = false;  // coordinate tables hold coords_box for the counted plan
   bool emitted = false;       // emit_args describe the last emit of the counted plan
+  // device buffers of the host entry (mdmc_run)
+  void *run_field = nullptr, *run_counts = nullptr, *run_v = nullptr, *run_q = nullptr;
+  size_t run_field_bytes = 0, run_counts_bytes = 0, run_v_bytes = 0, run_q_bytes = 0;
   mdmc::EmitArgs emit_args{};
   float coords_box[6] = {};
   const float *counted_field = nullptr;
@@ -837,6 +841,7 @@ int mdmc_cleanup(void) {
     mdmc_ctx_destroy(g_default_ctx);
     g_default_ctx = nullptr;
   }
+  staged_release();
   std::lock_guard<std::mutex> lk(g_setup_mutex);
   if (dev >= 0 && dev < 64) g_setup_mask &= ~(1ull << dev);
   return MDMC_OK;
@@ -883,6 +888,8 @@ void mdmc_ctx_destroy(mdmc_ctx *ctx) {
   if (!ctx) return;
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->scratch && !ctx->scratch_external) (void)hipFree(ctx->scratch);
+  for (void *b : {ctx->run_field, ctx->run_counts, ctx->run_v, ctx->run_q})
+    if (b) (void)hipFree(b);
   if (ctx->bdev) (void)hipFree(ctx->bdev);
   if (ctx->gdev) (void)hipFree(ctx->gdev);
   if (ctx->ghost) (void)hipHostFree(ctx->ghost);
@@ -1055,58 +1062,60 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
   if (p.empty) return MDMC_OK;
   if (!field) return fail(MDMC_ERR_INVALID, "null field");
   const size_t npts = (size_t)p.X * p.Y * p.Z;
-  float *d_field = nullptr;
-  mdmc_counts *d_counts = nullptr;
-  float *d_v = nullptr;
-  int32_t *d_q = nullptr;
-  mdmc_counts h{};
-  auto cleanup = [&]() {
-    if (d_field) (void)hipFree(d_field);
-    if (d_counts) (void)hipFree(d_counts);
-    if (d_v) (void)hipFree(d_v);
-    if (d_q) (void)hipFree(d_q);
+  // device buffers of the host entry stay with the default context and grow
+  // as needed; the field and the mesh cross PCIe through pinned chunks
+  auto grow = [&](void **buf, size_t *have, size_t want) -> int {
+    if (want <= *have) return MDMC_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (hipMalloc(buf, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(MDMC_ERR_NOMEM, "device buffer allocation failed");
+    }
+    *have = want;
+    return MDMC_OK;
   };
-#define RUN_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t _e = (expr);                                                             \
-    if (_e != hipSuccess) {                                                             \
-      cleanup();                                                                        \
-      return fail(MDMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
-    }                                                                                   \
-  } while (0)
-  RUN_TRY(hipMalloc(&d_field, npts * sizeof(float)));
-  RUN_TRY(hipMalloc(&d_counts, sizeof(mdmc_counts)));
-  RUN_TRY(hipMemcpy(d_field, field, npts * sizeof(float), hipMemcpyHostToDevice));
+  rc = grow(&ctx->run_field, &ctx->run_field_bytes, npts * sizeof(float));
+  if (rc == MDMC_OK) rc = grow(&ctx->run_counts, &ctx->run_counts_bytes, sizeof(mdmc_counts));
+  if (rc != MDMC_OK) return rc;
+  float *d_field = static_cast<float *>(ctx->run_field);
+  mdmc_counts *d_counts = static_cast<mdmc_counts *>(ctx->run_counts);
+  rc = staged_copy(const_cast<float *>(field), d_field, npts * sizeof(float), 0);
+  if (rc != MDMC_OK) return fail(rc, "field upload failed");
   rc = mdmc_count_device(ctx, d_field, dims, iso, d_counts, nullptr);
-  if (rc != MDMC_OK) { cleanup(); return rc; }
-  RUN_TRY(hipMemcpy(&h, d_counts, sizeof(h), hipMemcpyDeviceToHost));
-  if (h.status != 0) { cleanup(); return fail(MDMC_ERR_INTERNAL, "device status after count"); }
-  if (h.num_vertices) RUN_TRY(hipMalloc(&d_v, (size_t)h.num_vertices * 3 * sizeof(float)));
-  if (h.num_quads) RUN_TRY(hipMalloc(&d_q, (size_t)h.num_quads * 4 * sizeof(int32_t)));
-  rc = mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_v, h.num_vertices, d_q, h.num_quads,
-                        d_counts, nullptr);
-  if (rc != MDMC_OK) { cleanup(); return rc; }
+  if (rc != MDMC_OK) return rc;
+  mdmc_counts h{};
+  HIP_TRY(hipMemcpy(&h, d_counts, sizeof(h), hipMemcpyDeviceToHost));
+  if (h.status != 0) return fail(MDMC_ERR_INTERNAL, "device status after count");
+  const size_t vbytes = (size_t)h.num_vertices * 3 * sizeof(float), qbytes = (size_t)h.num_quads * 4 * sizeof(int32_t);
+  rc = grow(&ctx->run_v, &ctx->run_v_bytes, std::max<size_t>(vbytes, 256));
+  if (rc == MDMC_OK) rc = grow(&ctx->run_q, &ctx->run_q_bytes, std::max<size_t>(qbytes, 256));
+  if (rc != MDMC_OK) return rc;
+  float *d_v = static_cast<float *>(ctx->run_v);
+  int32_t *d_q = static_cast<int32_t *>(ctx->run_q);
+  rc = mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_v, h.num_vertices, d_q, h.num_quads, d_counts, nullptr);
+  if (rc != MDMC_OK) return rc;
   rc = mdmc_smooth_device(ctx, d_v, h.num_vertices, d_q, h.num_quads, num_smooth, d_counts, nullptr);
-  if (rc != MDMC_OK) { cleanup(); return rc; }
+  if (rc != MDMC_OK) return rc;
   mdmc_counts h2{};
-  RUN_TRY(hipMemcpy(&h2, d_counts, sizeof(h2), hipMemcpyDeviceToHost));
-  if (h2.status != 0) { cleanup(); return fail(MDMC_ERR_INTERNAL, "device status after emit"); }
+  HIP_TRY(hipMemcpy(&h2, d_counts, sizeof(h2), hipMemcpyDeviceToHost));
+  if (h2.status != 0) return fail(MDMC_ERR_INTERNAL, "device status after emit");
   out->num_active = h.num_active;
   out->num_vertices = h.num_vertices;
   out->num_quads = h.num_quads;
-  out->vertices = static_cast<float *>(std::malloc(std::max<size_t>(1, (size_t)h.num_vertices * 3 * sizeof(float))));
-  out->quads = static_cast<int32_t *>(std::malloc(std::max<size_t>(1, (size_t)h.num_quads * 4 * sizeof(int32_t))));
+  out->vertices = static_cast<float *>(std::malloc(std::max<size_t>(1, vbytes)));
+  out->quads = static_cast<int32_t *>(std::malloc(std::max<size_t>(1, qbytes)));
   if (!out->vertices || !out->quads) {
-    cleanup();
     mdmc_mesh_free(out);
     return fail(MDMC_ERR_NOMEM, "host mesh allocation failed");
   }
-  if (h.num_vertices)
-    RUN_TRY(hipMemcpy(out->vertices, d_v, (size_t)h.num_vertices * 3 * sizeof(float), hipMemcpyDeviceToHost));
-  if (h.num_quads)
-    RUN_TRY(hipMemcpy(out->quads, d_q, (size_t)h.num_quads * 4 * sizeof(int32_t), hipMemcpyDeviceToHost));
-#undef RUN_TRY
-  cleanup();
+  rc = staged_copy(out->vertices, d_v, vbytes, 1);
+  if (rc == MDMC_OK) rc = staged_copy(out->quads, d_q, qbytes, 1);
+  if (rc != MDMC_OK) {
+    mdmc_mesh_free(out);
+    return fail(rc, "mesh download failed");
+  }
   return MDMC_OK;
 }
 

@@ -1,0 +1,196 @@
+// Host I/O either side of the extraction: staged host<->device copies for
+// the host entry (mdmc_run) and the OBJ writer.
+//
+// Staged copies: pageable host memory cannot be the source or target of a
+// DMA, so the copy goes through pinned chunks.  A few host threads each own
+// two pinned chunks and a stream; thread t moves chunks t, t + T, t + 2T, ...
+// (memcpy into its free pinned chunk, then an async DMA on its stream; for
+// downloads the other way round), so host copies of one thread overlap the
+// DMAs of the others and of its own previous chunk.  All streams run at the
+// default priority.
+#include <hip/hip_runtime.h>
+
+#include <cinttypes>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/mdmc.h"
+#include "mdmc_io.h"
+
+namespace {
+
+constexpr int kIoThreads = 4;
+constexpr size_t kIoChunk = 8u << 20;  // bytes per pinned chunk
+
+struct IoLane {
+  hipStream_t stream = nullptr;
+  void *pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+struct IoStaging {
+  int device = -1;
+  IoLane lanes[kIoThreads];
+  bool ready = false;
+};
+
+std::mutex g_io_mutex;
+IoStaging g_io;
+
+void io_release(IoStaging &io) {
+  for (IoLane &l : io.lanes) {
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
+    for (int b = 0; b < 2; ++b) {
+      if (l.pin[b]) (void)hipHostFree(l.pin[b]);
+      if (l.ev[b]) (void)hipEventDestroy(l.ev[b]);
+      l.pin[b] = nullptr;
+      l.ev[b] = nullptr;
+    }
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+    l.stream = nullptr;
+  }
+  io.ready = false;
+  io.device = -1;
+}
+
+int io_init(IoStaging &io, int device) {
+  if (io.ready && io.device == device) return MDMC_OK;
+  if (io.ready) io_release(io);
+  for (IoLane &l : io.lanes) {
+    if (hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess) goto fail;
+    for (int b = 0; b < 2; ++b) {
+      if (hipHostMalloc(&l.pin[b], kIoChunk) != hipSuccess) goto fail;
+      if (hipEventCreateWithFlags(&l.ev[b], hipEventDisableTiming) != hipSuccess) goto fail;
+    }
+  }
+  io.ready = true;
+  io.device = device;
+  return MDMC_OK;
+fail:
+  (void)hipGetLastError();
+  io_release(io);
+  return MDMC_ERR_NOMEM;
+}
+
+// dir 0: host -> device, 1: device -> host.  Returns the first HIP error.
+hipError_t io_lane_copy(IoLane &l, int t, int nthreads, char *host, char *dev, size_t bytes, int dir) {
+  const size_t nchunks = (bytes + kIoChunk - 1) / kIoChunk;
+  bool pending[2] = {false, false};
+  size_t pend_off[2] = {0, 0}, pend_len[2] = {0, 0};
+  int b = 0;
+  hipError_t err = hipSuccess;
+  for (size_t c = (size_t)t; c < nchunks; c += (size_t)nthreads) {
+    const size_t off = c * kIoChunk, len = bytes - off < kIoChunk ? bytes - off : kIoChunk;
+    if (pending[b]) {
+      err = hipEventSynchronize(l.ev[b]);
+      if (err != hipSuccess) return err;
+      if (dir == 1) std::memcpy(host + pend_off[b], l.pin[b], pend_len[b]);
+      pending[b] = false;
+    }
+    if (dir == 0) {
+      std::memcpy(l.pin[b], host + off, len);
+      err = hipMemcpyAsync(dev + off, l.pin[b], len, hipMemcpyHostToDevice, l.stream);
+    } else {
+      err = hipMemcpyAsync(l.pin[b], dev + off, len, hipMemcpyDeviceToHost, l.stream);
+    }
+    if (err != hipSuccess) return err;
+    err = hipEventRecord(l.ev[b], l.stream);
+    if (err != hipSuccess) return err;
+    pending[b] = true;
+    pend_off[b] = off;
+    pend_len[b] = len;
+    b ^= 1;
+  }
+  for (int bb = 0; bb < 2; ++bb) {  // drain both chunks
+    if (!pending[bb]) continue;
+    err = hipEventSynchronize(l.ev[bb]);
+    if (err != hipSuccess) return err;
+    if (dir == 1) std::memcpy(host + pend_off[bb], l.pin[bb], pend_len[bb]);
+    pending[bb] = false;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+namespace mdmc {
+
+// Synchronous staged copy between pageable host memory and device memory on
+// the current device (dir 0: to the device, 1: to the host).
+int staged_copy(void *host, void *dev, size_t bytes, int dir) {
+  if (bytes == 0) return MDMC_OK;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) {
+    (void)hipGetLastError();
+    return MDMC_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lk(g_io_mutex);
+  int rc = io_init(g_io, device);
+  if (rc != MDMC_OK) return rc;
+  const size_t nchunks = (bytes + kIoChunk - 1) / kIoChunk;
+  const int nthreads = (int)(nchunks < (size_t)kIoThreads ? nchunks : (size_t)kIoThreads);
+  hipError_t errs[kIoThreads];
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t)
+    pool.emplace_back([&, t] {
+      (void)hipSetDevice(device);
+      errs[t] = io_lane_copy(g_io.lanes[t], t, nthreads, static_cast<char *>(host), static_cast<char *>(dev),
+                             bytes, dir);
+    });
+  errs[0] = io_lane_copy(g_io.lanes[0], 0, nthreads, static_cast<char *>(host), static_cast<char *>(dev), bytes,
+                         dir);
+  for (std::thread &th : pool) th.join();
+  for (int t = 0; t < nthreads; ++t)
+    if (errs[t] != hipSuccess) {
+      (void)hipGetLastError();
+      return MDMC_ERR_HIP;
+    }
+  return MDMC_OK;
+}
+
+void staged_release() {
+  std::lock_guard<std::mutex> lk(g_io_mutex);
+  if (g_io.ready) io_release(g_io);
+}
+
+}  // namespace mdmc
+
+extern "C" int mdmc_write_obj(const char *path, const float *vertices, uint64_t num_vertices, const int32_t *quads,
+                              uint64_t num_quads) {
+  if (!path || (num_vertices && !vertices) || (num_quads && !quads)) return MDMC_ERR_INVALID;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return MDMC_ERR_INVALID;
+  std::vector<char> buf(1u << 20);
+  size_t used = 0;
+  bool ok = true;
+  auto flush = [&]() {
+    if (used && std::fwrite(buf.data(), 1, used, f) != used) ok = false;
+    used = 0;
+  };
+  auto room = [&](size_t need) {
+    if (buf.size() - used < need) flush();
+  };
+  // "v x y z" with six significant digits (the reference writes floats
+  // through std::ostream at its default precision, dmc.cu:2238-2246)
+  for (uint64_t i = 0; i < num_vertices && ok; ++i) {
+    room(128);
+    const int n = std::snprintf(buf.data() + used, 128, "v %g %g %g\n", (double)vertices[3 * i],
+                                (double)vertices[3 * i + 1], (double)vertices[3 * i + 2]);
+    used += (size_t)n;
+  }
+  // two triangles (q0,q1,q2), (q2,q3,q0) per quad, 1-based indices
+  for (uint64_t q = 0; q < num_quads && ok; ++q) {
+    room(128);
+    const uint32_t a = (uint32_t)quads[4 * q] + 1u, b = (uint32_t)quads[4 * q + 1] + 1u,
+                   c = (uint32_t)quads[4 * q + 2] + 1u, d = (uint32_t)quads[4 * q + 3] + 1u;
+    const int n = std::snprintf(buf.data() + used, 128, "f %" PRIu32 " %" PRIu32 " %" PRIu32 "\nf %" PRIu32
+                                " %" PRIu32 " %" PRIu32 "\n", a, b, c, c, d, a);
+    used += (size_t)n;
+  }
+  flush();
+  if (std::fclose(f) != 0) ok = false;
+  return ok ? MDMC_OK : MDMC_ERR_INVALID;
+}

@@ -112,6 +112,8 @@ def lib():
     L.mdmc_extract_device.argtypes = [_VP, _VP, _U3, _F3, _F3, ctypes.c_float, _VP, ctypes.c_uint64, _VP,
                                       ctypes.c_uint64, _VP, _VP]
     L.mdmc_extract_device.restype = ctypes.c_int
+    L.mdmc_write_obj.argtypes = [ctypes.c_char_p, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]
+    L.mdmc_write_obj.restype = ctypes.c_int
     L.mdmc_smooth_device.argtypes = [_VP, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64, ctypes.c_uint32, _VP, _VP]
     L.mdmc_smooth_device.restype = ctypes.c_int
     L.mdmc_cells_device.argtypes = [_VP, _VP, _VP, _VP, _VP, ctypes.c_uint64, _VP]
@@ -184,6 +186,14 @@ def run_host(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, num_smoo
     finally:
         L.mdmc_mesh_free(ctypes.byref(m))
     return v, q
+
+
+def write_obj(path: str, vertices: np.ndarray, quads: np.ndarray) -> None:
+    """mdmc_write_obj: the mesh as Wavefront OBJ (two triangles per quad)."""
+    v = np.ascontiguousarray(vertices, dtype=np.float32)
+    q = np.ascontiguousarray(quads, dtype=np.int32)
+    _check(lib().mdmc_write_obj(os.fsencode(path), v.ctypes.data_as(_VP) if len(v) else None, len(v),
+                                q.ctypes.data_as(_VP) if len(q) else None, len(q)), "mdmc_write_obj")
 
 
 def run_dmc(grid: np.ndarray, xyz_min, xyz_max, iso: float, num_smooth: int = 0):

@@ -17,6 +17,7 @@
 #ifndef MDMC_DMC_HPP
 #define MDMC_DMC_HPP
 
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -130,6 +131,17 @@ inline void run_dmc(std::vector<float3> &vertices, std::vector<uint3> &triangles
     triangles[2 * (size_t)q + 1] = uint3{(unsigned)p[2], (unsigned)p[3], (unsigned)p[0]};
   }
   mdmc_mesh_free(&mesh);
+}
+
+// dump_obj (dmc.cu:2238-2246): Wavefront OBJ through std::ostream at its
+// default precision, 1-based triangle indices; byte-identical to
+// mdmc_write_obj on the quads the triangles come from.
+inline void dump_obj(const char *filename, const std::vector<float3> &vertices,
+                     const std::vector<uint3> &triangles) {
+  std::ofstream of(filename);
+  for (const float3 &v : vertices) of << "v " << v.x << " " << v.y << " " << v.z << "\n";
+  for (const uint3 &t : triangles) of << "f " << t.x + 1 << " " << t.y + 1 << " " << t.z + 1 << "\n";
+  if (!of) throw error(MDMC_ERR_INVALID, "dump_obj");
 }
 
 }  // namespace dmc

@@ -84,6 +84,13 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
 
 void mdmc_mesh_free(mdmc_mesh *mesh);
 
+/* Writes a mesh as Wavefront OBJ, like the reference's dump_obj
+ * (dmc.cu:2238-2246): "v x y z" per vertex (six significant digits, as the
+ * reference's std::ostream default) and two 1-based triangle faces
+ * (q0,q1,q2), (q2,q3,q0) per quad.  Host memory only; no device call. */
+int mdmc_write_obj(const char *path, const float *vertices, uint64_t num_vertices, const int32_t *quads,
+                   uint64_t num_quads);
+
 const char *mdmc_status_string(int status);
 
 /* Message of the last failing call on this thread. */

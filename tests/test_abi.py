@@ -57,3 +57,32 @@ def test_cpp_shim_header_compiles_and_links(tmp_path):
     subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(main), src,
                     "-L", libdir, "-lmdmc", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(tmp_path / "shim")],
                    check=True)
+
+
+def test_obj_writers_agree(tmp_path):
+    """mdmc_write_obj over quads and the shim's dump_obj over the triangles the
+    shim's run_dmc makes from them give the same bytes; faces are 1-based."""
+    import numpy as np
+
+    rng = np.random.default_rng(3)
+    v = (rng.standard_normal((257, 3)) * [1.0, 300.0, 1e-3]).astype(np.float32)
+    v[5] = [0.1, 1e7, -0.0]
+    v[9, 1] = np.float32(np.nan)
+    q = rng.integers(0, 257, (300, 4)).astype(np.int32)
+    (tmp_path / "v.f32").write_bytes(v.tobytes())
+    (tmp_path / "q.i32").write_bytes(q.tobytes())
+    src = os.path.join(ROOT, "tests", "golden", "obj_check.cpp")
+    libdir = os.path.dirname(fa.mdmc.LIB_PATH)
+    exe = tmp_path / "obj_check"
+    subprocess.run(["g++", "-std=c++14", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src, "-L",
+                    libdir, "-lmdmc", "-Wl,-rpath," + libdir, "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(exe)],
+                   check=True)
+    subprocess.run([str(exe), str(tmp_path / "v.f32"), str(tmp_path / "q.i32"), str(tmp_path / "a.obj"),
+                    str(tmp_path / "b.obj")], check=True)
+    a, b = (tmp_path / "a.obj").read_bytes(), (tmp_path / "b.obj").read_bytes()
+    assert a == b
+    lines = a.decode().splitlines()
+    assert len(lines) == 257 + 600
+    assert lines[0].startswith("v ") and lines[257] == "f %d %d %d" % (q[0, 0] + 1, q[0, 1] + 1, q[0, 2] + 1)
+    fa.write_obj(str(tmp_path / "c.obj"), v, q)
+    assert (tmp_path / "c.obj").read_bytes() == a
