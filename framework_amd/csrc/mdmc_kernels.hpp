@@ -1,26 +1,28 @@
 // Dual Marching Cubes kernels for gfx950 (MI355X).
 //
-// Pipeline for one volume (three stream-ordered launches, no host sync):
+// Pipeline for one volume (stream-ordered launches, no host sync):
 //
-//   k_classify  dense pass over the field.  One wave owns a tile of R cell
-//               rows x one x-tile (64*W cells) and marches a z-chunk of planes.
-//               Each point is loaded once per tile (halo: one extra point row
-//               per R rows, one extra plane per chunk).  Lanes keep the sign
-//               bits (v < iso) of their point column for the tile's rows at
-//               planes k and k+1 in one 32-bit register; the neighbour column
-//               comes from lane+1.  A chunk whose 64 columns are uniformly in
-//               or out is skipped with two ballots.  Active cells get their
-//               2B/3B table variant, vertex and quad counts, and a 64-bit
-//               record written into a per-segment slot range (segment = one
-//               cell row of one x-tile in one plane), ranked with ballot +
-//               mbcnt and wave prefix sums.  Per-segment totals are written.
+//   k_classify_pt  dense pass over the field in plane tiles (7 cell rows x
+//               one x-tile x 1-4 cell planes per 256-thread workgroup): point
+//               rows staged in LDS as sign bits, per-cell configuration keys
+//               by multiply-shift, uniform 256-cell chunks skipped with one
+//               ballot; active cells get their 2B/3B table variant, vertex and
+//               quad counts, a DPP-scan rank and a 64-bit record in their
+//               segment's slot range (segment = one cell row of one x-tile in
+//               one plane), plus a per-64-cell group map (active mask, first
+//               rank).  k_classify is the z-marching form for any row length.
 //   k_scan      single-pass decoupled look-back scan (ticketed tiles) over the
-//               segment totals: vertex/quad offsets per segment, the list of
-//               non-empty segments and where each 64-record chunk starts.
-//   k_emit      sparse pass, one lane per active cell: edge crossing points,
-//               dual vertices (FP32, reference operation order) and quads.
-//               Ring neighbours are found by binary search in the neighbour
-//               segment's sorted records.
+//               segment totals: vertex/quad bases per segment, the non-empty
+//               segment list and where each 64-cell chunk starts.
+//   k_emit      one wave per chunk of 64 active cells: edge crossing points
+//               in the owner's orientation, dual vertices (FP32, reference
+//               operation order) and quads; ring cells are found through the
+//               group map in O(1).
+//   k_smooth, k_reaverage   optional smoothing iterations over the mesh.
+//
+// Batched forms (k_classify_batch, k_batch_counts, k_emit_batch) run the
+// same per-tile and per-cell code over many volumes; k_field_* generate the
+// synthetic benchmark fields.
 //
 // Reference semantics: DMC_CUDA/DMC_CUDA/src/dmc.cu:1018-2208 (dmc::run_dmc).
 #pragma once
