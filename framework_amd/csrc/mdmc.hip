@@ -1119,6 +1119,215 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
   return MDMC_OK;
 }
 
+// Point planes [c0-1, c0+S+2] (clipped) of a slab of S cell planes, and the
+// largest S whose field slab and scratch fit the budget.
+static int ooc_slab_planes(const uint32_t dims[3], uint64_t budget, const int *opt, uint32_t *S_out) {
+  const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
+  if (X < 2 || Y < 2 || Z < 2) {
+    *S_out = 0;
+    return MDMC_OK;
+  }
+  const uint32_t cz = Z - 1;
+  auto need = [&](uint32_t S) -> uint64_t {
+    const uint32_t npl = S < cz ? std::min(S + 4, Z) : Z;  // a middle slab is the largest
+    const uint32_t sd[3] = {X, Y, npl};
+    Plan p;
+    if (plan_tiling(sd, &p, opt) != MDMC_OK) return ~0ull;
+    return 4ull * X * Y * npl + scratch_layout(p).total;
+  };
+  if (need(1) > budget) return fail(MDMC_ERR_NOMEM, "budget below one slab of one cell plane");
+  uint32_t lo = 1, hi = cz;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo + 1) / 2;
+    if (need(mid) <= budget) lo = mid; else hi = mid - 1;
+  }
+  *S_out = lo;
+  return MDMC_OK;
+}
+
+int mdmc_out_of_core_slab_planes(const uint32_t dims[3], uint64_t device_budget, uint32_t *slab_planes) {
+  if (!dims || !slab_planes) return fail(MDMC_ERR_INVALID, "null argument");
+  return ooc_slab_planes(dims, device_budget, nullptr, slab_planes);
+}
+
+// Out-of-core: the grid goes through the device in z-slabs of S cell
+// planes.  Slab [c0, c1) is classified with point planes [c0-1, c1+2]
+// (clipped to the grid): one plane below and two above give every cell of
+// the slab, and every ring cell in plane c1, its 2B/3B neighbours, so their
+// configurations, variants and vertex counts are those of the whole grid.
+// In the slab's own numbering, the vertices of planes >= c0 are the whole
+// grid's numbering shifted by a constant (the slab starts where the previous
+// slabs' vertices end), so the emit writes only planes [c0, c1) and adds that
+// shift to every quad entry.  Coordinates use the whole grid's divisors.
+int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float xyz_min[3],
+                         const float xyz_max[3], float iso, uint64_t device_budget, mdmc_mesh *out) {
+  if (!out || !dims || !xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  if (!is_setup_on(dev)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  if (!g_default_ctx) {
+    rc = mdmc_ctx_create(&g_default_ctx);
+    if (rc != MDMC_OK) return rc;
+  }
+  mdmc_ctx *ctx = g_default_ctx;
+  const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
+  if (X < 2 || Y < 2 || Z < 2) return MDMC_OK;
+  if (!field) return fail(MDMC_ERR_INVALID, "null field");
+  const uint32_t cz = Z - 1;
+  const size_t plane = (size_t)X * Y;
+  uint32_t S = 0;
+  rc = ooc_slab_planes(dims, device_budget, ctx->opt, &S);
+  if (rc != MDMC_OK) return rc;
+  // point planes of a slab of cell planes [c0, c0 + n)
+  auto slab_planes = [&](uint32_t c0, uint32_t n, uint32_t *lo) {
+    const uint32_t l = c0 > 0 ? c0 - 1 : 0, hi = std::min(c0 + n + 2, Z - 1);
+    *lo = l;
+    return hi - l + 1;
+  };
+  void *d_field = nullptr, *d_counts_v = nullptr, *d_v = nullptr, *d_q = nullptr;
+  size_t fv_bytes = 0, v_bytes = 0, q_bytes = 0;
+  std::vector<uint2> segs;
+  float *hv = nullptr;
+  int32_t *hq = nullptr;
+  uint64_t hv_cap = 0, hq_cap = 0, G = 0, GQ = 0, A = 0;
+  auto release = [&]() {
+    for (void *b : {d_field, d_counts_v, d_v, d_q})
+      if (b) (void)hipFree(b);
+  };
+  auto bail = [&](int code, const char *msg) {
+    release();
+    std::free(hv);
+    std::free(hq);
+    return fail(code, msg);
+  };
+  auto grow_dev = [&](void **b, size_t *have, size_t want) -> bool {
+    if (want <= *have) return true;
+    if (*b) (void)hipFree(*b);
+    *b = nullptr;
+    *have = 0;
+    if (hipMalloc(b, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    *have = want;
+    return true;
+  };
+  auto grow_host = [&](void **b, uint64_t *cap, uint64_t want, size_t elem) -> bool {
+    if (want <= *cap) return true;
+    const uint64_t n = std::max<uint64_t>(want, *cap * 2);
+    void *nb = std::realloc(*b, std::max<size_t>(1, (size_t)n * elem));
+    if (!nb) return false;
+    *b = nb;
+    *cap = n;
+    return true;
+  };
+  if (hipMalloc(&d_counts_v, sizeof(mdmc_counts)) != hipSuccess) {
+    (void)hipGetLastError();
+    return bail(MDMC_ERR_NOMEM, "counts allocation failed");
+  }
+  mdmc_counts *d_counts = static_cast<mdmc_counts *>(d_counts_v);
+  for (uint32_t c0 = 0; c0 < cz; c0 += S) {
+    const uint32_t c1 = std::min(c0 + S, cz);
+    uint32_t plo;
+    const uint32_t npl = slab_planes(c0, c1 - c0, &plo);
+    const uint32_t h = c0 - plo;
+    if (!grow_dev(&d_field, &fv_bytes, 4 * plane * npl)) return bail(MDMC_ERR_NOMEM, "field slab allocation failed");
+    rc = staged_copy(const_cast<float *>(field) + plane * plo, d_field, 4 * plane * npl, 0);
+    if (rc != MDMC_OK) return bail(rc, "field slab upload failed");
+    const uint32_t sd[3] = {X, Y, npl};
+    Plan p;
+    rc = plan_tiling(sd, &p, ctx->opt);
+    if (rc != MDMC_OK) return bail(rc, "slab plan failed");
+    rc = ensure_scratch(ctx, p);
+    if (rc != MDMC_OK) return bail(rc, "slab scratch failed");
+    const float *df = static_cast<const float *>(d_field);
+    rc = launch_count(ctx, p, df, iso, d_counts, nullptr);
+    if (rc != MDMC_OK) return bail(rc, "slab count failed");
+    ctx->plan = p;
+    ctx->counted = true;
+    ctx->counted_field = df;
+    ctx->iso = iso;
+    mdmc_counts hc{};
+    if (hipMemcpy(&hc, d_counts, sizeof(hc), hipMemcpyDeviceToHost) != hipSuccess || hc.status)
+      return bail(MDMC_ERR_INTERNAL, "slab count status");
+    // vertex / quad bases of the first segment of planes h and h + (c1 - c0)
+    const uint64_t sp = (uint64_t)p.XT * p.cy;
+    const uint64_t s_lo = h * sp, s_hi = (uint64_t)(h + (c1 - c0)) * sp;
+    uint2 b_lo{}, b_hi{};
+    if (hipMemcpy(&b_lo, ctx->s.seg_vq + s_lo, sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&b_hi, ctx->s.seg_vq + s_hi, sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
+      return bail(MDMC_ERR_HIP, "slab bases");
+    segs.resize((size_t)(s_hi - s_lo));
+    if (!segs.empty() && hipMemcpy(segs.data(), ctx->s.seg_counts + s_lo, segs.size() * sizeof(uint2),
+                                   hipMemcpyDeviceToHost) != hipSuccess)
+      return bail(MDMC_ERR_HIP, "slab segment counts");
+    for (const uint2 &c : segs) A += c.x & 0xFFFFu;
+    const uint32_t Vs = b_hi.x - b_lo.x, Qs = b_hi.y - b_lo.y;
+    if (!grow_dev(&d_v, &v_bytes, std::max<size_t>(256, 12ull * Vs)) ||
+        !grow_dev(&d_q, &q_bytes, std::max<size_t>(256, 16ull * Qs)))
+      return bail(MDMC_ERR_NOMEM, "slab output allocation failed");
+    // emit planes [h, h + c1 - c0) with global coordinates and numbering
+    const Scratch &sc = ctx->s;
+    EmitArgs ea{};
+    ea.v.g = FieldDesc{df, p.X, p.Y, p.Z, p.cx, p.cy, p.cz, iso};
+    CoordArgs co = coord_args(p, sc, xyz_min, xyz_max);
+    co.cz = cz;
+    co.z_off = plo;
+    ea.v.xt_c = sc.coords;
+    ea.v.yt_c = sc.coords + p.X;
+    ea.v.zt_c = sc.coords + p.X + p.Y;
+    ea.v.XT = p.XT;
+    ea.v.vbase = b_lo.x;
+    ea.v.qbase = b_lo.y;
+    ea.v.kmin = h;
+    ea.v.kmax = h + (c1 - c0);
+    ea.v.vshift = (uint32_t)G;
+    ea.cap = p.cap;
+    ea.nsegs = (uint32_t)p.nsegs;
+    ea.max_chunks = p.max_chunks;
+    ea.seg_counts = sc.seg_counts;
+    ea.seg_vq = sc.seg_vq;
+    ea.slots = sc.slots;
+    ea.gmap = sc.gmap;
+    ea.ne = sc.ne;
+    ea.chunk_ne = sc.chunk_ne;
+    ea.totals = sc.totals;
+    ea.v.verts = static_cast<float *>(d_v);
+    ea.v.vcap = Vs;
+    ea.v.quads = static_cast<int32_t *>(d_q);
+    ea.v.qcap = Qs;
+    ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
+    hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0,
+                       nullptr, co);
+    const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
+    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
+    hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, nullptr, ea);
+    if (hipGetLastError() != hipSuccess) return bail(MDMC_ERR_HIP, "slab emit launch");
+    ctx->emitted = false;  // the slab mesh is not the context's mesh
+    if (hipMemcpy(&hc, d_counts, sizeof(hc), hipMemcpyDeviceToHost) != hipSuccess || hc.status)
+      return bail(MDMC_ERR_INTERNAL, "slab emit status");
+    if (G + Vs > 0xFFFFFFFFull) return bail(MDMC_ERR_TOO_LARGE, "more than 2^32-1 vertices");
+    if (!grow_host(reinterpret_cast<void **>(&hv), &hv_cap, G + Vs, 12) ||
+        !grow_host(reinterpret_cast<void **>(&hq), &hq_cap, GQ + Qs, 16))
+      return bail(MDMC_ERR_NOMEM, "host mesh allocation failed");
+    rc = staged_copy(hv + 3 * G, d_v, 12ull * Vs, 1);
+    if (rc == MDMC_OK) rc = staged_copy(hq + 4 * GQ, d_q, 16ull * Qs, 1);
+    if (rc != MDMC_OK) return bail(rc, "slab mesh download failed");
+    G += Vs;
+    GQ += Qs;
+  }
+  release();
+  out->vertices = hv ? hv : static_cast<float *>(std::malloc(1));
+  out->quads = hq ? hq : static_cast<int32_t *>(std::malloc(1));
+  out->num_vertices = (uint32_t)G;
+  out->num_quads = (uint32_t)GQ;
+  out->num_active = (uint32_t)A;
+  return MDMC_OK;
+}
+
 void mdmc_mesh_free(mdmc_mesh *mesh) {
   if (!mesh) return;
   std::free(mesh->vertices);

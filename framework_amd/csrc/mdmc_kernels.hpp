@@ -703,6 +703,7 @@ struct CoordArgs {
   float *out;
   uint32_t X, Y, Z, cx, cy, cz;
   float mnx, mny, mnz, rgx, rgy, rgz;
+  uint32_t z_off;  // global index of the first z point (a slab of a larger grid; cz is then the grid's)
 };
 
 __device__ __forceinline__ void coords_fill(const CoordArgs &c, uint32_t i0, uint32_t stride) {
@@ -711,7 +712,7 @@ __device__ __forceinline__ void coords_fill(const CoordArgs &c, uint32_t i0, uin
     float v;
     if (i < c.X) v = coordf(i, c.cx, c.rgx, c.mnx);
     else if (i < c.X + c.Y) v = coordf(i - c.X, c.cy, c.rgy, c.mny);
-    else v = coordf(i - c.X - c.Y, c.cz, c.rgz, c.mnz);
+    else v = coordf(i - c.X - c.Y + c.z_off, c.cz, c.rgz, c.mnz);
     c.out[i] = v;
   }
 }
@@ -932,6 +933,9 @@ struct EmitVol {
   int32_t *quads;
   uint64_t qcap;
   uint32_t *status;
+  // emit only cell planes [kmin, kmax); quad entries are vertex numbers plus
+  // vshift (a slab of a larger grid, numbered globally)
+  uint32_t kmin = 0, kmax = 0xFFFFFFFFu, vshift = 0;
 };
 
 struct EmitArgs {
@@ -962,6 +966,7 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   const uint32_t j = row % g.cy, k = row / g.cy;
   const uint32_t x = xt * cap + xl;
   if (x >= g.cx || xl >= cap) { atomicOr(vol.status, kStatusInternal); return; }
+  if (k < vol.kmin || k >= vol.kmax) return;
   const uint2 vq = a.seg_vq[s];
   const uint32_t vb = vq.x + rec_lvb(rec) - vol.vbase;  // volume-local numbers
   const uint32_t qb = vq.y + rec_lqb(rec) - vol.qbase;
@@ -1107,11 +1112,11 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
     const uint32_t m2 = nrec[n];
     const uint32_t lab = (uint32_t)(s_lab[rec_lut2(m2) * 256 + rec_cfg(m2)] >> (4 * e)) & 0xFu;
     if (used && (!act[n] || lab == 0xFu)) ok = false;
-    return (int32_t)(vq0[n] + rec_lvb(m2) + lab - vol.vbase);
+    return (int32_t)(vq0[n] + rec_lvb(m2) + lab - vol.vbase + vol.vshift);
   };
   int4 *qo = reinterpret_cast<int4 *>(vol.quads) + qb;
   uint32_t qn = 0;
-  const int32_t self = (int32_t)vb;
+  const int32_t self = (int32_t)(vb + vol.vshift);
   if (e6) {
     const int32_t m0 = self + (int32_t)((labels >> 24) & 0xFu);
     const int32_t m1 = vtx(0, 7, true), m2 = vtx(3, 4, true), m3 = vtx(1, 5, true);

@@ -92,6 +92,12 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with framework_amd.build.build()")
+    # torch first: it brings its own HIP runtime, which the library then
+    # shares; the other order leaves two runtimes in the process
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     L.mdmc_status_string.restype = ctypes.c_char_p
     L.mdmc_status_string.argtypes = [ctypes.c_int]
@@ -112,6 +118,10 @@ def lib():
     L.mdmc_extract_device.argtypes = [_VP, _VP, _U3, _F3, _F3, ctypes.c_float, _VP, ctypes.c_uint64, _VP,
                                       ctypes.c_uint64, _VP, _VP]
     L.mdmc_extract_device.restype = ctypes.c_int
+    L.mdmc_run_out_of_core.argtypes = [_VP, _U3, _F3, _F3, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(Mesh)]
+    L.mdmc_run_out_of_core.restype = ctypes.c_int
+    L.mdmc_out_of_core_slab_planes.argtypes = [_U3, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32)]
+    L.mdmc_out_of_core_slab_planes.restype = ctypes.c_int
     L.mdmc_write_obj.argtypes = [ctypes.c_char_p, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64]
     L.mdmc_write_obj.restype = ctypes.c_int
     L.mdmc_smooth_device.argtypes = [_VP, _VP, ctypes.c_uint64, _VP, ctypes.c_uint64, ctypes.c_uint32, _VP, _VP]
@@ -168,6 +178,16 @@ def triangles_from_quads(quads: np.ndarray) -> np.ndarray:
     return t.reshape(-1, 3)
 
 
+def _take_mesh(L, m):
+    try:
+        V, Q = m.num_vertices, m.num_quads
+        v = np.ctypeslib.as_array(m.vertices, shape=(V * 3,)).copy().reshape(V, 3) if V else np.zeros((0, 3), np.float32)
+        q = np.ctypeslib.as_array(m.quads, shape=(Q * 4,)).copy().reshape(Q, 4) if Q else np.zeros((0, 4), np.int32)
+    finally:
+        L.mdmc_mesh_free(ctypes.byref(m))
+    return v, q
+
+
 def run_host(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, num_smooth: int = 0):
     """mdmc_run: host field in, host (vertices (V,3) float32, quads (Q,4) int32) out."""
     field = np.ascontiguousarray(field, dtype=np.float32)
@@ -179,13 +199,30 @@ def run_host(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, num_smoo
     L = lib()
     _check(L.mdmc_run(field.ctypes.data_as(_VP), _U3(X, Y, Z), mn, mx, ctypes.c_float(iso),
                       ctypes.c_uint32(num_smooth), ctypes.byref(m)), "mdmc_run")
-    try:
-        V, Q = m.num_vertices, m.num_quads
-        v = np.ctypeslib.as_array(m.vertices, shape=(V * 3,)).copy().reshape(V, 3) if V else np.zeros((0, 3), np.float32)
-        q = np.ctypeslib.as_array(m.quads, shape=(Q * 4,)).copy().reshape(Q, 4) if Q else np.zeros((0, 4), np.int32)
-    finally:
-        L.mdmc_mesh_free(ctypes.byref(m))
-    return v, q
+    return _take_mesh(L, m)
+
+
+def run_out_of_core(field: np.ndarray, iso: float, device_budget: int, xyz_min=None, xyz_max=None):
+    """mdmc_run_out_of_core: the host field goes through the device in z-slabs
+    within device_budget bytes; same mesh as run_host."""
+    field = np.ascontiguousarray(field, dtype=np.float32)
+    Z, Y, X = field.shape
+    dmin, dmax = _default_box((X, Y, Z))
+    mn = _F3(*(xyz_min if xyz_min is not None else dmin))
+    mx = _F3(*(xyz_max if xyz_max is not None else dmax))
+    m = Mesh()
+    L = lib()
+    _check(L.mdmc_run_out_of_core(field.ctypes.data_as(_VP), _U3(X, Y, Z), mn, mx, ctypes.c_float(iso),
+                                  int(device_budget), ctypes.byref(m)), "mdmc_run_out_of_core")
+    return _take_mesh(L, m)
+
+
+def out_of_core_slab_planes(dims_xyz, device_budget: int) -> int:
+    """Cell planes per slab of run_out_of_core for these dims and budget."""
+    S = ctypes.c_uint32()
+    _check(lib().mdmc_out_of_core_slab_planes(_U3(*dims_xyz), int(device_budget), ctypes.byref(S)),
+           "mdmc_out_of_core_slab_planes")
+    return S.value
 
 
 def write_obj(path: str, vertices: np.ndarray, quads: np.ndarray) -> None:

@@ -82,6 +82,19 @@ int mdmc_cleanup(void);
 int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
              const float xyz_max[3], float iso, uint32_t num_smooth, mdmc_mesh *out);
 
+/* mdmc_run for grids larger than the device memory one wants to give it (no
+ * counterpart in the reference, which copies the whole grid to the device,
+ * dmc.cu:2077-2079): the host field goes through the device in z-slabs
+ * sized so that a slab and its scratch fit `device_budget` bytes (the mesh
+ * buffers of one slab come on top).  The mesh is identical to mdmc_run's
+ * with num_smooth = 0 (smoothing needs the whole mesh and is not offered
+ * here). */
+int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float xyz_min[3],
+                         const float xyz_max[3], float iso, uint64_t device_budget, mdmc_mesh *out);
+/* Cell planes per slab mdmc_run_out_of_core uses for these dims and budget
+ * (MDMC_ERR_NOMEM if not even one plane fits).  Host-only arithmetic. */
+int mdmc_out_of_core_slab_planes(const uint32_t dims[3], uint64_t device_budget, uint32_t *slab_planes);
+
 void mdmc_mesh_free(mdmc_mesh *mesh);
 
 /* Writes a mesh as Wavefront OBJ, like the reference's dump_obj
