@@ -2,7 +2,8 @@
 
 The tunable launch shapes are context options (MDMC_OPT_* in include/mdmc.h):
 
-  single volume  MDMC_OPT_PT_PLANES            cell planes per classify tile
+  single volume  MDMC_OPT_PT_ROWS              cell rows per classify tile (7, 15)
+                 MDMC_OPT_PT_PLANES            cell planes per classify tile
                  MDMC_OPT_PT_BAND              row blocks per XCD band
                  MDMC_OPT_EMIT_BLOCKS_PER_CU   emit grid
   batches        MDMC_OPT_BATCH_PLANES         cell planes per batch classify tile
@@ -26,13 +27,15 @@ import os
 import numpy as np
 
 from .mdmc import (OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS_PER_CU, OPT_PT_BAND,
-                   OPT_PT_PLANES, MDMCError)
+                   OPT_PT_PLANES, OPT_PT_ROWS, MDMCError)
 
-SINGLE_SPACE = {OPT_PT_PLANES: [0, 2, 3], OPT_PT_BAND: [1, 2, 4], OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}
+SINGLE_SPACE = {OPT_PT_ROWS: [0, 15], OPT_PT_PLANES: [0, 1, 2, 3], OPT_PT_BAND: [1, 2, 4],
+                OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}
 BATCH_SPACE = {OPT_BATCH_PLANES: [0, 1, 2, 4], OPT_CLASSIFY_BLOCKS_PER_CU: [0, 4, 8, 12],
                OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}
 OPTION_NAMES = {OPT_BATCH_PLANES: "batch_planes", OPT_CLASSIFY_BLOCKS_PER_CU: "classify_blocks_per_cu",
-                OPT_EMIT_BLOCKS_PER_CU: "emit_blocks_per_cu", OPT_PT_PLANES: "pt_planes", OPT_PT_BAND: "pt_band"}
+                OPT_EMIT_BLOCKS_PER_CU: "emit_blocks_per_cu", OPT_PT_PLANES: "pt_planes", OPT_PT_BAND: "pt_band",
+                OPT_PT_ROWS: "pt_rows"}
 
 
 def default_cache_path() -> str:

@@ -120,7 +120,12 @@ int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt) {
   p->vec4 = (p->X % 4 == 0) && env_int("MDMC_VEC4", 1) != 0;
   p->ptR = pt_rows(W / 4);
   p->ptKP = pt_planes(W / 4);
-  // context options (0: default): cell planes per tile, XCD band
+  // context options (0: default): cell rows and planes per tile, XCD band
+  const int r_opt = opt ? opt[MDMC_OPT_PT_ROWS] : 0;
+  if (r_opt == 15 && pt_shape_supported(W / 4, 15, (int)pt_planes15(W / 4))) {
+    p->ptR = 15;
+    p->ptKP = pt_planes15(W / 4);
+  }
   const int kp_opt = opt ? opt[MDMC_OPT_PT_PLANES] : 0;
   if (kp_opt > 0 && pt_shape_supported(W / 4, (int)p->ptR, kp_opt)) p->ptKP = (uint32_t)kp_opt;
   const uint32_t rows = p->ptR;
@@ -713,13 +718,15 @@ int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts 
   BatchCountArgs bc{};
   bc.vols = batch_table_dev(ctx);
   bc.nvol = n;
-  bc.seg_counts = S.seg_counts;
+  bc.nsegs = (uint32_t)nsegs;
   bc.seg_vq = S.seg_vq;
+  bc.ne = S.ne;
+  bc.totals = S.totals;
   bc.base = batch_base_dev(ctx, n);
   bc.batch_counts = bcounts;
   {
     ProfScope ps(ctx, 3, st);
-    hipLaunchKernelGGL(k_batch_counts, dim3(std::min<uint32_t>(n, (uint32_t)ctx->num_cus * 4)), dim3(256), 0, st, bc);
+    hipLaunchKernelGGL(k_batch_counts, dim3((n + 255) / 256), dim3(256), 0, st, bc);
   }
   HIP_TRY(hipGetLastError());
   agg.nsegs = nsegs;

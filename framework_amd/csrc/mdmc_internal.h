@@ -40,7 +40,7 @@ struct Scratch {
 
 // opt: the context's option values (MDMC_OPT_*), or null for defaults.
 int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt);
-constexpr int kNumOptions = 5;
+constexpr int kNumOptions = 6;
 
 // Tile shapes with a compiled classify kernel.
 inline bool tiling_supported(uint32_t W, uint32_t R) {
@@ -77,18 +77,26 @@ inline void launch_classify_pt_t(bool vec, const PtArgs &a, uint32_t blocks, hip
 
 // Compiled plane-tile shapes besides the defaults (pt_rows, pt_planes).
 inline bool pt_shape_supported(uint32_t W4, int R, int KP) {
-  if (W4 == 2) return R == 7 && (KP == 2 || KP == 3);
+  if (W4 == 1) return (R == 7 && KP == 4) || (R == 15 && KP == 2);
+  if (W4 == 2) return (R == 7 && (KP == 2 || KP == 3)) || (R == 15 && (KP == 1 || KP == 2));
   return (uint32_t)R == pt_rows(W4) && (uint32_t)KP == pt_planes(W4);
 }
+// cell planes of the 15-row shapes by default
+inline uint32_t pt_planes15(uint32_t W4) { return W4 == 1 ? 2u : 1u; }
 
 // Plane-tile kernel (W = 4, 8 or 16); vec: 16-byte row loads.
 inline int launch_classify_pt(uint32_t W, uint32_t R, uint32_t KP, bool vec, const PtArgs &a,
                               uint32_t blocks, hipStream_t st) {
   switch (W) {
-    case 4: launch_classify_pt_t<1, (int)pt_rows(1), (int)pt_planes(1)>(vec, a, blocks, st); return 0;
+    case 4:
+      if (R == 7 && KP == 4) { launch_classify_pt_t<1, 7, 4>(vec, a, blocks, st); return 0; }
+      if (R == 15 && KP == 2) { launch_classify_pt_t<1, 15, 2>(vec, a, blocks, st); return 0; }
+      return -1;
     case 8:
       if (R == 7 && KP == 2) { launch_classify_pt_t<2, 7, 2>(vec, a, blocks, st); return 0; }
       if (R == 7 && KP == 3) { launch_classify_pt_t<2, 7, 3>(vec, a, blocks, st); return 0; }
+      if (R == 15 && KP == 1) { launch_classify_pt_t<2, 15, 1>(vec, a, blocks, st); return 0; }
+      if (R == 15 && KP == 2) { launch_classify_pt_t<2, 15, 2>(vec, a, blocks, st); return 0; }
       return -1;
     case 16: launch_classify_pt_t<4, (int)pt_rows(4), (int)pt_planes(4)>(vec, a, blocks, st); return 0;
     default: return -1;

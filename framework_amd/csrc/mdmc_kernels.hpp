@@ -1489,35 +1489,39 @@ __global__ __launch_bounds__(kEmitThreads) void k_reaverage(EmitArgs a) {
 
 struct BatchCountArgs {
   const BatchVol *vols;
-  uint32_t nvol;
-  const uint2 *seg_counts;
+  uint32_t nvol, nsegs;
   const uint2 *seg_vq;
+  const uint2 *ne;              // non-empty segments {segment, first active}, sentinel last
+  const Totals *totals;
   uint2 *base;                  // per volume: global vertex / quad number of its first
   const uint32_t *batch_counts;  // the batch scan's counts (status bits in [3])
 };
 
-// One workgroup per volume: its totals and the bases of its numbering.
+// Active cells before segment s: the first active of the first non-empty
+// segment at or after s (the list ends with {nsegs, total}).
+__device__ __forceinline__ uint32_t actives_before(const uint2 *ne, uint32_t num_ne, uint32_t s) {
+  uint32_t lo = 0, hi = num_ne;  // ne[num_ne] is the sentinel
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ne[mid].x < s) lo = mid + 1; else hi = mid;
+  }
+  return ne[lo].y;
+}
+
+// One thread per volume: its totals and the bases of its numbering.
 __global__ __launch_bounds__(256) void k_batch_counts(BatchCountArgs b) {
-  __shared__ uint32_t s_part[256 / kWave];
-  for (uint32_t v = blockIdx.x; v < b.nvol; v += gridDim.x) {
-    const BatchVol &bv = b.vols[v];
-    uint32_t na = 0;
-    for (uint32_t i = threadIdx.x; i < bv.nsegs; i += 256) na += b.seg_counts[bv.seg_off + i].x & 0xFFFFu;
-    na = wave_incl_scan_dpp(na);
-    __syncthreads();
-    if (lane_id() == kWave - 1) s_part[wave_id()] = na;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t A = s_part[0] + s_part[1] + s_part[2] + s_part[3];
-      const uint2 v0 = b.seg_vq[bv.seg_off], v1 = b.seg_vq[bv.seg_off + bv.nsegs];
-      b.base[v] = v0;
-      if (bv.counts) {
-        bv.counts[0] = A;
-        bv.counts[1] = v1.x - v0.x;
-        bv.counts[2] = v1.y - v0.y;
-        bv.counts[3] = b.batch_counts[3];
-      }
-    }
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= b.nvol) return;
+  const BatchVol &bv = b.vols[v];
+  const uint32_t num_ne = min(b.totals->num_nonempty, b.nsegs);
+  const uint2 v0 = b.seg_vq[bv.seg_off], v1 = b.seg_vq[bv.seg_off + bv.nsegs];
+  b.base[v] = v0;
+  if (bv.counts) {
+    bv.counts[0] = bv.nsegs ? actives_before(b.ne, num_ne, bv.seg_off + bv.nsegs) - actives_before(b.ne, num_ne, bv.seg_off)
+                            : 0u;
+    bv.counts[1] = v1.x - v0.x;
+    bv.counts[2] = v1.y - v0.y;
+    bv.counts[3] = b.batch_counts[3];
   }
 }
 

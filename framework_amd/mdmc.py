@@ -75,7 +75,8 @@ class GenSpec(ctypes.Structure):
 
 
 MAX_BATCH = 1024
-OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS_PER_CU, OPT_PT_PLANES, OPT_PT_BAND = 0, 1, 2, 3, 4
+OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS_PER_CU, OPT_PT_PLANES, OPT_PT_BAND, OPT_PT_ROWS = \
+    0, 1, 2, 3, 4, 5
 
 
 def declared_functions() -> list[str]:
