@@ -24,6 +24,7 @@
 #include <cstring>
 #include <deque>
 #include <new>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -470,16 +471,14 @@ int mdmc_server_poll(mdmc_server *s, int wait, mdmc_result *out, uint32_t max, u
     while (pump(s)) {
     }
     if (!s->finished.empty() || !wait) break;
-    // block on a phase in flight; with none (empty, or waiting for results
-    // to be released) there is nothing to wait for
-    bool waited = false;
-    for (Lane &l : s->lanes)
-      if (l.batch && (l.batch->phase == 1 || l.batch->phase == 3)) {
-        (void)hipEventSynchronize(l.ev);
-        waited = true;
-        break;
-      }
-    if (!waited) break;
+    // wait for whichever phase in flight ends first: keep querying every
+    // lane (pump) so that a lane whose count finished gets its emit at once;
+    // with no phase in flight (empty, or waiting for results to be released)
+    // there is nothing to wait for
+    bool in_flight = false;
+    for (Lane &l : s->lanes) in_flight |= l.batch && (l.batch->phase == 1 || l.batch->phase == 3);
+    if (!in_flight) break;
+    std::this_thread::yield();
   }
   while (*n < max && !s->finished.empty()) {
     Req *r = s->finished.front();

@@ -176,7 +176,7 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set |
         res = server.poll(wait=False)
         if not res and i < n:
             # idle until the next arrival or a completion, without spinning hard
-            time.sleep(min(max(arrivals[i] - (server.now() - t0), 0.0), 0.0002))
+            time.sleep(min(max(arrivals[i] - (server.now() - t0), 0.0), 0.00005))
             continue
         if not res and i >= n:
             res = server.poll(wait=True)
