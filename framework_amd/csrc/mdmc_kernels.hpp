@@ -822,11 +822,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
         unsigned long long w[4] = {INC, INC, INC, INC};  // before tile 0: inclusive zeros
         if (idx >= 0) {
           const unsigned long long *p = a.state + 4ull * (uint64_t)idx;
+          // the four channel words are read together; only words not yet
+          // published are read again
+#pragma unroll
+          for (int ch = 0; ch < 4; ++ch)
+            if (!((done >> ch) & 1u)) w[ch] = ld_state(p + ch);
           uint32_t spins = 0;
 #pragma unroll
           for (int ch = 0; ch < 4; ++ch) {
             if ((done >> ch) & 1u) continue;
-            do { w[ch] = ld_state(p + ch); } while ((w[ch] >> 32) == 0 && ++spins < kSpinLimit);
+            while ((w[ch] >> 32) == 0 && ++spins < kSpinLimit) w[ch] = ld_state(p + ch);
           }
           if (spins >= kSpinLimit) timed_out = true;
         }
