@@ -23,6 +23,7 @@ namespace {
 
 thread_local std::string g_last_error;
 std::mutex g_setup_mutex;
+std::mutex g_host_entry_mutex;  // the host entries share the default context
 uint64_t g_setup_mask = 0;  // bit d: tables uploaded on device d
 mdmc_ctx *g_default_ctx = nullptr;
 
@@ -844,9 +845,12 @@ int mdmc_cleanup(void) {
   int dev = 0;
   int rc = current_device(&dev);
   if (rc != MDMC_OK) return rc;
-  if (g_default_ctx) {
-    mdmc_ctx_destroy(g_default_ctx);
-    g_default_ctx = nullptr;
+  {
+    std::lock_guard<std::mutex> entry(g_host_entry_mutex);
+    if (g_default_ctx) {
+      mdmc_ctx_destroy(g_default_ctx);
+      g_default_ctx = nullptr;
+    }
   }
   staged_release();
   std::lock_guard<std::mutex> lk(g_setup_mutex);
@@ -1053,6 +1057,7 @@ int mdmc_cells_device(mdmc_ctx *ctx, uint32_t *d_cell_index, uint8_t *d_cfg, uin
 int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
              const float xyz_max[3], float iso, uint32_t num_smooth, mdmc_mesh *out) {
   if (!out || !dims || !xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> entry(g_host_entry_mutex);
   std::memset(out, 0, sizeof(*out));
   int dev = 0;
   int rc = current_device(&dev);
@@ -1169,6 +1174,7 @@ int mdmc_out_of_core_slab_planes(const uint32_t dims[3], uint64_t device_budget,
 int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float xyz_min[3],
                          const float xyz_max[3], float iso, uint64_t device_budget, mdmc_mesh *out) {
   if (!out || !dims || !xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> entry(g_host_entry_mutex);
   std::memset(out, 0, sizeof(*out));
   int dev = 0;
   int rc = current_device(&dev);

@@ -399,33 +399,8 @@ class Context:
                                               float(noise_amp), self._stream(stream)), "mdmc_field_gyroid_device")
         return out
 
+    # ---- batches (many volumes per launch sequence) and launch options
 
-def _volume_table(fields, isos, boxes=None, outputs=None):
-    """ctypes array of Volume for torch fields shaped (Z, Y, X)."""
-    n = len(fields)
-    arr = (Volume * n)()
-    for i, f in enumerate(fields):
-        if f.dtype.is_floating_point is False or f.dim() != 3 or not f.is_contiguous():
-            raise ValueError("batch fields must be contiguous 3-D float32 CUDA tensors")
-        Z, Y, X = f.shape
-        v = arr[i]
-        v.field = f.data_ptr()
-        v.dims[0], v.dims[1], v.dims[2] = X, Y, Z
-        v.iso = float(isos[i])
-        mn, mx = boxes[i] if boxes is not None else _default_box((X, Y, Z))
-        for k in range(3):
-            v.xyz_min[k] = float(mn[k])
-            v.xyz_max[k] = float(mx[k])
-        if outputs is not None and outputs[i] is not None:
-            verts, quads = outputs[i]
-            v.vertices = verts.data_ptr() if verts is not None and verts.numel() else None
-            v.vertex_capacity = verts.numel() // 3 if verts is not None else 0
-            v.quads = quads.data_ptr() if quads is not None and quads.numel() else None
-            v.quad_capacity = quads.numel() // 4 if quads is not None else 0
-    return arr
-
-
-def _batch_methods():
     def set_option(self, option: int, value: int) -> None:
         _check(lib().mdmc_ctx_set_option(self._h, int(option), int(value)), "mdmc_ctx_set_option")
 
@@ -498,9 +473,29 @@ def _batch_methods():
                "mdmc_batch_generate_device")
         return outs
 
-    for f in (set_option, batch_counts_tensor, batch_count, batch_emit, batch_extract_into, batch_extract,
-              batch_generate):
-        setattr(Context, f.__name__, f)
+
+def _volume_table(fields, isos, boxes=None, outputs=None):
+    """ctypes array of Volume for torch fields shaped (Z, Y, X)."""
+    n = len(fields)
+    arr = (Volume * n)()
+    for i, f in enumerate(fields):
+        if f.dtype.is_floating_point is False or f.dim() != 3 or not f.is_contiguous():
+            raise ValueError("batch fields must be contiguous 3-D float32 CUDA tensors")
+        Z, Y, X = f.shape
+        v = arr[i]
+        v.field = f.data_ptr()
+        v.dims[0], v.dims[1], v.dims[2] = X, Y, Z
+        v.iso = float(isos[i])
+        mn, mx = boxes[i] if boxes is not None else _default_box((X, Y, Z))
+        for k in range(3):
+            v.xyz_min[k] = float(mn[k])
+            v.xyz_max[k] = float(mx[k])
+        if outputs is not None and outputs[i] is not None:
+            verts, quads = outputs[i]
+            v.vertices = verts.data_ptr() if verts is not None and verts.numel() else None
+            v.vertex_capacity = verts.numel() // 3 if verts is not None else 0
+            v.quads = quads.data_ptr() if quads is not None and quads.numel() else None
+            v.quad_capacity = quads.numel() // 4 if quads is not None else 0
+    return arr
 
 
-_batch_methods()
