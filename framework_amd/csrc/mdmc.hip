@@ -580,6 +580,7 @@ int fill_batch_table(mdmc_ctx *ctx, int slot, const mdmc_volume *vols, uint32_t 
     b.tile_off = (uint32_t)tile_off;
     b.NRB = NRB;
     b.NKP = NKP;
+    b.vec = (X % 4 == 0 && (reinterpret_cast<uintptr_t>(mv.field) & 15u) == 0) ? 1u : 0u;
     b.seg_off = (uint32_t)seg_off;
     b.nsegs = (uint32_t)nsegs;
     b.mnx = mv.xyz_min[0];

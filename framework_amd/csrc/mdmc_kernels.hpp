@@ -642,6 +642,7 @@ struct BatchVol {
   ClassifyArgs c;         // per-volume field and tiling; record pointers start at seg_off
   uint32_t tile_off;      // first flattened classify tile
   uint32_t NRB, NKP;      // row blocks and plane blocks of the volume
+  uint32_t vec;           // rows are 16-byte aligned multiples of 4 points
   uint32_t seg_off, nsegs;
   float mnx, mny, mnz, rgx, rgy, rgz;
   float *verts;
@@ -674,7 +675,17 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
     if (threadIdx.x < 4) b.zero.counts_out[threadIdx.x] = 0u;  // the batch's scan counts
   }
   __syncthreads();
-  for (uint32_t t = blockIdx.x; t < b.total_tiles; t += gridDim.x) {
+  // XCD x (workgroups b with b % 8 == x) takes a contiguous eighth of the
+  // flattened tiles, so tiles that share halo rows and planes meet in its L2
+  const uint32_t G = gridDim.x;
+  uint32_t t = blockIdx.x, t_end = b.total_tiles, t_step = G;
+  if (G >= 8u) {
+    const uint32_t x = blockIdx.x & 7u;
+    t = (uint32_t)((uint64_t)b.total_tiles * x / 8u) + (blockIdx.x >> 3);
+    t_end = (uint32_t)((uint64_t)b.total_tiles * (x + 1u) / 8u);
+    t_step = (G >> 3) + (x < (G & 7u) ? 1u : 0u);
+  }
+  for (; t < t_end; t += t_step) {
     // volume of tile t: the last v with tile_off[v] <= t
     uint32_t lo = 0, hi = b.nvol;
     while (hi - lo > 1) {
@@ -687,7 +698,10 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
     const uint32_t local = t - s_toff[v];
     const uint32_t xt = local % a.XT, u = local / a.XT;
     __syncthreads();  // the previous tile's LDS words are no longer read
-    classify_pt_tile<W4, R, KP, false>(a, xt, u % bv.NRB, u / bv.NRB, s_w, s_tab);
+    if (bv.vec)
+      classify_pt_tile<W4, R, KP, true>(a, xt, u % bv.NRB, u / bv.NRB, s_w, s_tab);
+    else
+      classify_pt_tile<W4, R, KP, false>(a, xt, u % bv.NRB, u / bv.NRB, s_w, s_tab);
   }
 }
 
