@@ -226,7 +226,8 @@ def main() -> int:
     host = None if args.skip_host else bench_host(fa, field.cpu().numpy(), V, Q)
     dense = None if args.skip_dense else bench_dense(fa, torch)
     batched = None if args.skip_batched else bench_batched(fa, torch, tune=not args.no_tune)
-    served = None if args.skip_served else bench_served(fa, torch, args.served_requests)
+    served = None if args.skip_served else bench_served(
+        fa, torch, args.served_requests, options=(batched or {}).get("launch_option_ids"))
 
     out = {
         "metric": "Gvoxels/s + %HBM peak at 512^3; served volumes/s with p50/p99 latency",
@@ -427,7 +428,8 @@ def bench_batched(fa, torch, nvol: int = 256, reps: int = 5, tune: bool = True) 
            "speedup_graph": t_plain / t_graph, "batched_gvoxels_per_s": points / t_batch / 1e9,
            "graph_gvoxels_per_s": points / t_graph / 1e9,
            "launch_options": None if tuned is None else
-           {autotune.OPTION_NAMES[k]: v for k, v in tuned["options"].items()}}
+           {autotune.OPTION_NAMES[k]: v for k, v in tuned["options"].items()},
+           "launch_option_ids": None if tuned is None else dict(tuned["options"])}
     log(f"batched: {out}")
     return out
 
@@ -450,13 +452,16 @@ def _served_requests(n: int, seed: int):
     return reqs
 
 
-def bench_served(fa, torch, nreq: int, budget: int = 64 << 30) -> dict:
-    """Poisson stream through the request loop at 0.8x its measured capacity."""
+def bench_served(fa, torch, nreq: int, budget: int = 64 << 30, options: dict | None = None) -> dict:
+    """Poisson stream through the request loop at 0.8x its measured capacity
+    (lanes use the batched leg's tuned launch options when given)."""
     import numpy as np
     from framework_amd.server import Server, run_plain_loop, run_stream
 
     reqs = _served_requests(nreq, seed=5)
     srv = Server(budget, max_batch=32, num_streams=4)
+    if options:
+        srv.set_options(options)
     # capacity: a saturating burst (everything arrives at once) on a sample
     ncap = min(1000, nreq)
     log(f"served: capacity burst of {ncap}")

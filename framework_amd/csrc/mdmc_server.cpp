@@ -532,6 +532,15 @@ int mdmc_server_release(mdmc_server *s, uint64_t id) {
   return MDMC_OK;
 }
 
+int mdmc_server_set_option(mdmc_server *s, int option, int value) {
+  if (!s) return MDMC_ERR_INVALID;
+  for (Lane &l : s->lanes) {
+    const int rc = mdmc_ctx_set_option(l.ctx, option, value);
+    if (rc != MDMC_OK) return rc;
+  }
+  return MDMC_OK;
+}
+
 int mdmc_server_get_stats(mdmc_server *s, mdmc_server_stats *st) {
   if (!s || !st) return MDMC_ERR_INVALID;
   s->note_busy();

@@ -70,6 +70,8 @@ def _bind():
     L.mdmc_server_release.restype = ctypes.c_int
     L.mdmc_server_get_stats.argtypes = [_VP, ctypes.POINTER(ServerStats)]
     L.mdmc_server_get_stats.restype = ctypes.c_int
+    L.mdmc_server_set_option.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
+    L.mdmc_server_set_option.restype = ctypes.c_int
     L.mdmc_server_now.argtypes = [_VP]
     L.mdmc_server_now.restype = ctypes.c_double
     L._server_bound = True
@@ -156,6 +158,12 @@ class Server:
 
     def now(self) -> float:
         return lib().mdmc_server_now(self._h)
+
+    def set_options(self, options: dict) -> None:
+        """Launch options ({MDMC_OPT_* id: value}) for every lane, e.g. an
+        autotuner's batch choice."""
+        for opt, val in options.items():
+            _check(lib().mdmc_server_set_option(self._h, int(opt), int(val)), "mdmc_server_set_option")
 
 
 def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set | None = None) -> dict:

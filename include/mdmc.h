@@ -291,6 +291,8 @@ int mdmc_server_copy_result(mdmc_server *srv, uint64_t id, float *h_vertices, in
 /* Frees a finished result's device memory. */
 int mdmc_server_release(mdmc_server *srv, uint64_t id);
 int mdmc_server_get_stats(mdmc_server *srv, mdmc_server_stats *st);
+/* Sets a launch option (MDMC_OPT_*) on every lane's context. */
+int mdmc_server_set_option(mdmc_server *srv, int option, int value);
 /* Seconds on the server clock (the clock of t_submit / t_done). */
 double mdmc_server_now(mdmc_server *srv);
 

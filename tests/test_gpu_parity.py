@@ -140,6 +140,20 @@ def test_nan_corners(ctx, torch_cuda):
     check_against_oracle(ctx, _dev(torch_cuda, f), 0.1)
 
 
+def test_extreme_values(ctx, torch_cuda):
+    """Magnitudes near the float range and infinities: differences overflow to
+    inf, interpolation parameters become 0, nan or inf, the same IEEE results
+    on both sides."""
+    rng = np.random.default_rng(10)
+    f = (rng.standard_normal((26, 30, 34)) * 3e38).astype(np.float32)
+    flat = f.reshape(-1)
+    flat[rng.choice(f.size, 150, replace=False)] = np.inf
+    flat[rng.choice(f.size, 150, replace=False)] = -np.inf
+    flat[rng.choice(f.size, 100, replace=False)] = np.float32(3.4e38)
+    check_against_oracle(ctx, _dev(torch_cuda, f), 0.0)
+    check_against_oracle(ctx, _dev(torch_cuda, f), 1e38)
+
+
 def test_denormal_values(ctx, torch_cuda):
     rng = np.random.default_rng(9)
     f = (rng.standard_normal((20, 24, 28)) * 1e-40).astype(np.float32)
