@@ -83,6 +83,25 @@ def test_result_alone_equals_result_in_mix(fa_mod, torch_cuda):
         assert np.array_equal(va.view(np.uint32), vs.view(np.uint32)), k
 
 
+def test_every_result_in_a_stream_equals_its_single_extraction(fa_mod, torch_cuda):
+    """All results of a continuously batched stream (arrivals while batches
+    run, uneven sizes, both field kinds) against the single-volume path."""
+    from framework_amd.server import Server, run_stream
+
+    reqs = _requests(24, seed=9, dmin=16, dmax=72)
+    srv = Server(1 << 30, max_batch=5, num_streams=3)
+    arrivals = np.cumsum(np.random.default_rng(1).exponential(1e-4, len(reqs)))
+    res = run_stream(srv, reqs, arrivals, keep=set(range(len(reqs))))
+    st = srv.stats()
+    srv.close()
+    assert (res["status"] == 0).all() and st["completed"] == len(reqs) and st["batches"] >= 3
+    for k, r in enumerate(reqs):
+        vs, qs = _single(fa_mod, r)
+        vm, qm = res["meshes"][k]
+        assert np.array_equal(qm, qs), k
+        assert np.array_equal(vm.view(np.uint32), vs.view(np.uint32)), k
+
+
 def test_admission_under_a_tight_budget(fa_mod, torch_cuda):
     """Requests larger than the budget are rejected; everything else finishes
     with the peak memory within the budget."""
