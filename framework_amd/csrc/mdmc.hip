@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -1146,7 +1147,9 @@ static int ooc_slab_planes(const uint32_t dims[3], uint64_t budget, const int *o
     const uint32_t sd[3] = {X, Y, npl};
     Plan p;
     if (plan_tiling(sd, &p, opt) != MDMC_OK) return ~0ull;
-    return 4ull * X * Y * npl + scratch_layout(p).total;
+    // two field slabs when there is more than one (the next one is uploaded
+    // while the current one is extracted)
+    return (S < cz ? 2ull : 1ull) * 4ull * X * Y * npl + scratch_layout(p).total;
   };
   if (need(1) > budget) return fail(MDMC_ERR_NOMEM, "budget below one slab of one cell plane");
   uint32_t lo = 1, hi = cz;
@@ -1200,14 +1203,17 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     *lo = l;
     return hi - l + 1;
   };
-  void *d_field = nullptr, *d_counts_v = nullptr, *d_v = nullptr, *d_q = nullptr;
-  size_t fv_bytes = 0, v_bytes = 0, q_bytes = 0;
+  void *d_fbuf[2] = {nullptr, nullptr}, *d_counts_v = nullptr, *d_v = nullptr, *d_q = nullptr;
+  size_t fv_bytes[2] = {0, 0}, v_bytes = 0, q_bytes = 0;
   std::vector<uint2> segs;
   float *hv = nullptr;
   int32_t *hq = nullptr;
   uint64_t hv_cap = 0, hq_cap = 0, G = 0, GQ = 0, A = 0;
+  std::thread uploader;
+  int upload_rc = MDMC_OK;
   auto release = [&]() {
-    for (void *b : {d_field, d_counts_v, d_v, d_q})
+    if (uploader.joinable()) uploader.join();
+    for (void *b : {d_fbuf[0], d_fbuf[1], d_counts_v, d_v, d_q})
       if (b) (void)hipFree(b);
   };
   auto bail = [&](int code, const char *msg) {
@@ -1242,14 +1248,33 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     return bail(MDMC_ERR_NOMEM, "counts allocation failed");
   }
   mdmc_counts *d_counts = static_cast<mdmc_counts *>(d_counts_v);
-  for (uint32_t c0 = 0; c0 < cz; c0 += S) {
-    const uint32_t c1 = std::min(c0 + S, cz);
+  // slab i computes from field buffer i % 2 while a host thread uploads slab
+  // i + 1 into the other buffer
+  const uint32_t nslabs = (cz + S - 1) / S;
+  const size_t max_npl = std::min<size_t>((size_t)S + 4, Z);
+  for (int b2 = 0; b2 < 2 && b2 < (int)nslabs; ++b2)
+    if (!grow_dev(&d_fbuf[b2], &fv_bytes[b2], 4 * plane * max_npl))
+      return bail(MDMC_ERR_NOMEM, "field slab allocation failed");
+  auto upload = [&](uint32_t i) {
+    uint32_t lo;
+    const uint32_t n = slab_planes(i * S, std::min(S, cz - i * S), &lo);
+    return staged_copy(const_cast<float *>(field) + plane * lo, d_fbuf[i % 2], 4 * plane * n, 0);
+  };
+  rc = upload(0);
+  if (rc != MDMC_OK) return bail(rc, "field slab upload failed");
+  for (uint32_t i = 0; i < nslabs; ++i) {
+    const uint32_t c0 = i * S, c1 = std::min(c0 + S, cz);
     uint32_t plo;
     const uint32_t npl = slab_planes(c0, c1 - c0, &plo);
     const uint32_t h = c0 - plo;
-    if (!grow_dev(&d_field, &fv_bytes, 4 * plane * npl)) return bail(MDMC_ERR_NOMEM, "field slab allocation failed");
-    rc = staged_copy(const_cast<float *>(field) + plane * plo, d_field, 4 * plane * npl, 0);
-    if (rc != MDMC_OK) return bail(rc, "field slab upload failed");
+    if (uploader.joinable()) uploader.join();
+    if (upload_rc != MDMC_OK) return bail(upload_rc, "field slab upload failed");
+    if (i + 1 < nslabs)
+      uploader = std::thread([&, i, dev] {
+        (void)hipSetDevice(dev);
+        upload_rc = upload(i + 1);
+      });
+    void *d_field = d_fbuf[i % 2];
     const uint32_t sd[3] = {X, Y, npl};
     Plan p;
     rc = plan_tiling(sd, &p, ctx->opt);
@@ -1334,6 +1359,11 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     GQ += Qs;
   }
   release();
+  if (upload_rc != MDMC_OK) {
+    std::free(hv);
+    std::free(hq);
+    return fail(upload_rc, "field slab upload failed");
+  }
   out->vertices = hv ? hv : static_cast<float *>(std::malloc(1));
   out->quads = hq ? hq : static_cast<int32_t *>(std::malloc(1));
   out->num_vertices = (uint32_t)G;

@@ -85,7 +85,8 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
 /* mdmc_run for grids larger than the device memory one wants to give it (no
  * counterpart in the reference, which copies the whole grid to the device,
  * dmc.cu:2077-2079): the host field goes through the device in z-slabs
- * sized so that a slab and its scratch fit `device_budget` bytes (the mesh
+ * sized so that two slabs (the next one is uploaded while the current one is
+ * extracted) and the scratch of one fit `device_budget` bytes (the mesh
  * buffers of one slab come on top).  The mesh is identical to mdmc_run's
  * with num_smooth = 0 (smoothing needs the whole mesh and is not offered
  * here). */
