@@ -36,7 +36,7 @@ namespace mdmc {
 
 constexpr int kWave = 64;
 constexpr uint32_t kStatusVertexOverflow = 1, kStatusQuadOverflow = 2, kStatusLookup = 4,
-                   kStatusScanTimeout = 8, kStatusInternal = 16;
+                   kStatusScanTimeout = 8, kStatusInternal = 16, kStatusIndexRange = 32;
 
 // Case tables in constant memory (uploaded by mdmc_setup):
 //   c_label[v][cfg]   12 nibbles, edge e -> iso-vertex label or 0xF
@@ -870,6 +870,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a) {
         st_state(my + 1, INC | (uint32_t)(excl.y + agg.y));
         st_state(my + 2, INC | (uint32_t)(excl.z + agg.z));
         st_state(my + 3, INC | (uint32_t)(excl.w + agg.w));
+        // the tile where a 32-bit total first passes 2^32 - 1 sees its
+        // inclusive sum wrap (vertex and quad numbers are 32-bit, as the
+        // reference's)
+        const bool wraps = excl.x + agg.x < excl.x || excl.y + agg.y < excl.y || excl.z + agg.z < excl.z;
+        if (wraps && a.counts_out) atomicOr(a.counts_out + 3, kStatusIndexRange);
       }
     }
     if (lane == 0) s_prefix = excl;

@@ -25,7 +25,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "mdmc.h")
 
 MDMC_OK = 0
 STATUS_NAMES = {1: "vertex overflow", 2: "quad overflow", 4: "ring lookup failed", 8: "scan timeout",
-                16: "internal inconsistency"}
+                16: "internal inconsistency", 32: "more than 2^32-1 cells, vertices or quads"}
 
 _lib = None
 

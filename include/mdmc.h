@@ -43,7 +43,8 @@ enum {
   MDMC_STATUS_QUAD_OVERFLOW = 2,   /* quads beyond quad_capacity dropped */
   MDMC_STATUS_LOOKUP_FAILED = 4,   /* a ring neighbour was not found */
   MDMC_STATUS_SCAN_TIMEOUT = 8,    /* look-back spin limit hit */
-  MDMC_STATUS_INTERNAL = 16        /* inconsistent count-pass data (bug) */
+  MDMC_STATUS_INTERNAL = 16,       /* inconsistent count-pass data (bug) */
+  MDMC_STATUS_INDEX_RANGE = 32     /* more than 2^32-1 active cells, vertices or quads */
 };
 
 /* Sizes of one extraction.  Written by the device; read after a stream sync. */
