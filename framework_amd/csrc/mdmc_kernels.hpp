@@ -1206,7 +1206,9 @@ __device__ __forceinline__ ChunkIter chunk_iter(uint32_t n, uint32_t wave) {
   return ChunkIter{lo + slot * wpb + wave, hi, gx * wpb};
 }
 
-__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
+// 8 waves per SIMD: the emit is latency-bound, the register cap costs a few
+// scalar spills only
+__global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_emit(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
@@ -1569,7 +1571,7 @@ struct BatchEmitArgs {
 
 // Emit over the active cells of all volumes; a lane finds its volume by
 // binary search over the volumes' first segments.
-__global__ __launch_bounds__(kEmitThreads) void k_emit_batch(BatchEmitArgs b) {
+__global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_emit_batch(BatchEmitArgs b) {
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
   __shared__ uint32_t s_soff[kMaxBatch + 1];
