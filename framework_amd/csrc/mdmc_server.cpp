@@ -7,6 +7,11 @@
 // allocated with their exact sizes once the count pass of their batch has
 // finished (the scan gives the sizes).
 //
+// Progress: a worker thread owned by the server drives the lanes (every
+// phase change is noticed within microseconds, whatever the caller is
+// doing); submit, poll, release and the rest take the server's mutex and
+// wake it.
+//
 // Scheduling: `num_streams` lanes, each with its own stream and extraction
 // context.  An idle lane takes the queued requests in arrival order (up to
 // max_batch, as long as their footprints fit), generates their fields, and
@@ -18,11 +23,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <unordered_map>
@@ -102,6 +110,16 @@ struct mdmc_server {
   uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, peak = 0;
   double busy = 0, busy_since = -1;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  // worker thread and its synchronisation (everything above is guarded by mu)
+  std::mutex mu;
+  std::condition_variable cv_work;  // submit / release / stop
+  std::condition_variable cv_done;  // a result finished or the lanes went idle
+  std::thread worker;
+  std::atomic<uint64_t> epoch{0};   // bumped (under mu) by calls that may unblock work
+  std::atomic<bool> stop{false};
+  bool idle = false;                // the worker waits for a call (nothing it can do)
+  uint64_t idle_epoch = 0;          // the epoch it went idle at
+  int device = 0;
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -369,6 +387,8 @@ bool pump(mdmc_server *s) {
       }
     }
   }
+  // an idle lane takes whatever is queued at once: holding a lane back for a
+  // fuller batch lowered neither the GPU time per request nor the tail
   for (Lane &lane : s->lanes) {
     if (lane.batch || s->queue.empty()) continue;
     if (start_batch(s, lane) != MDMC_OK) break;
@@ -376,6 +396,52 @@ bool pump(mdmc_server *s) {
   }
   s->note_busy();
   return progress;
+}
+
+// The worker: pumps the lanes under the lock; while a phase is on the device
+// it watches the lanes' events WITHOUT the lock (only the worker records or
+// retires them), so callers are never held up by its polling; with nothing in
+// flight it sleeps until a call bumps the epoch.
+void worker_loop(mdmc_server *s) {
+  (void)hipSetDevice(s->device);
+  std::vector<hipEvent_t> watch;
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (!s->stop.load()) {
+    const uint64_t seen = s->epoch.load();
+    s->idle = false;
+    bool moved = false;
+    while (pump(s)) moved = true;
+    watch.clear();
+    for (const Lane &l : s->lanes)
+      if (l.batch && (l.batch->phase == 1 || l.batch->phase == 3)) watch.push_back(l.ev);
+    if (moved) s->cv_done.notify_all();
+    if (watch.empty()) {
+      s->idle = true;
+      s->idle_epoch = seen;
+      s->cv_done.notify_all();
+      s->cv_work.wait(lk, [&] { return s->stop.load() || s->epoch.load() != seen; });
+      continue;
+    }
+    lk.unlock();
+    for (;;) {
+      bool ready = false;
+      for (hipEvent_t e : watch)
+        if (hipEventQuery(e) != hipErrorNotReady) {
+          ready = true;
+          break;
+        }
+      if (ready || s->stop.load() || s->epoch.load() != seen) break;
+      std::this_thread::yield();
+    }
+    lk.lock();
+  }
+  s->idle = true;
+  s->cv_done.notify_all();
+}
+
+void wake(mdmc_server *s) {
+  s->epoch.fetch_add(1);
+  s->cv_work.notify_one();
 }
 
 }  // namespace
@@ -413,12 +479,31 @@ int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out) {
       return MDMC_ERR_NOMEM;
     }
   }
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    (void)hipGetLastError();
+    mdmc_server_destroy(s);
+    return MDMC_ERR_HIP;
+  }
+  try {
+    s->worker = std::thread(worker_loop, s);
+  } catch (...) {
+    mdmc_server_destroy(s);
+    return MDMC_ERR_NOMEM;
+  }
   *out = s;
   return MDMC_OK;
 }
 
 void mdmc_server_destroy(mdmc_server *s) {
   if (!s) return;
+  if (s->worker.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop.store(true);
+    }
+    s->cv_work.notify_all();
+    s->worker.join();
+  }
   for (Lane &l : s->lanes) {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     if (l.batch) delete l.batch;
@@ -443,6 +528,7 @@ int mdmc_server_submit(mdmc_server *s, const mdmc_request *req, uint64_t *id) {
     return MDMC_ERR_INVALID;
   Req *r = new (std::nothrow) Req();
   if (!r) return MDMC_ERR_NOMEM;
+  std::lock_guard<std::mutex> lk(s->mu);
   r->id = s->next_id++;
   TRACE("submit %llu dims %u %u %u kind %d", (unsigned long long)r->id, req->dims[0], req->dims[1], req->dims[2],
         req->field_kind);
@@ -457,28 +543,24 @@ int mdmc_server_submit(mdmc_server *s, const mdmc_request *req, uint64_t *id) {
   *id = r->id;
   if (r->footprint > s->pool.size()) {
     finish(s, r, MDMC_ERR_REJECTED);
+    s->cv_done.notify_all();
     return MDMC_OK;
   }
   s->queue.push_back(r);
-  pump(s);
+  wake(s);
   return MDMC_OK;
 }
 
 int mdmc_server_poll(mdmc_server *s, int wait, mdmc_result *out, uint32_t max, uint32_t *n) {
   if (!s || !n || (max && !out)) return MDMC_ERR_INVALID;
   *n = 0;
-  for (;;) {
-    while (pump(s)) {
-    }
-    if (!s->finished.empty() || !wait) break;
-    // wait for whichever phase in flight ends first: keep querying every
-    // lane (pump) so that a lane whose count finished gets its emit at once;
-    // with no phase in flight (empty, or waiting for results to be released)
-    // there is nothing to wait for
-    bool in_flight = false;
-    for (Lane &l : s->lanes) in_flight |= l.batch && (l.batch->phase == 1 || l.batch->phase == 3);
-    if (!in_flight) break;
-    std::this_thread::yield();
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (wait) {
+    // until a result is there, or the worker has gone idle since the last
+    // call that could start work (then no result comes without another call)
+    s->cv_done.wait(lk, [&] {
+      return !s->finished.empty() || (s->idle && s->idle_epoch == s->epoch.load());
+    });
   }
   while (*n < max && !s->finished.empty()) {
     Req *r = s->finished.front();
@@ -501,21 +583,31 @@ int mdmc_server_poll(mdmc_server *s, int wait, mdmc_result *out, uint32_t max, u
 
 int mdmc_server_copy_result(mdmc_server *s, uint64_t id, float *h_vertices, int32_t *h_quads) {
   if (!s) return MDMC_ERR_INVALID;
-  auto it = s->reqs.find(id);
-  if (it == s->reqs.end() || it->second->state < 2) return MDMC_ERR_INVALID;
-  Req *r = it->second;
-  if (r->status != MDMC_OK) return r->status;
-  if (h_vertices && r->verts &&
-      hipMemcpy(h_vertices, r->verts, 12ull * r->counts.num_vertices, hipMemcpyDeviceToHost) != hipSuccess)
+  const float *dv = nullptr;
+  const int32_t *dq = nullptr;
+  uint64_t V = 0, Q = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end() || it->second->state < 2) return MDMC_ERR_INVALID;
+    Req *r = it->second;
+    if (r->status != MDMC_OK) return r->status;
+    dv = r->verts;
+    dq = r->quads;
+    V = r->counts.num_vertices;
+    Q = r->counts.num_quads;
+  }
+  // the result stays put until the caller releases it
+  if (h_vertices && dv && hipMemcpy(h_vertices, dv, 12ull * V, hipMemcpyDeviceToHost) != hipSuccess)
     return MDMC_ERR_HIP;
-  if (h_quads && r->quads &&
-      hipMemcpy(h_quads, r->quads, 16ull * r->counts.num_quads, hipMemcpyDeviceToHost) != hipSuccess)
+  if (h_quads && dq && hipMemcpy(h_quads, dq, 16ull * Q, hipMemcpyDeviceToHost) != hipSuccess)
     return MDMC_ERR_HIP;
   return MDMC_OK;
 }
 
 int mdmc_server_release(mdmc_server *s, uint64_t id) {
   if (!s) return MDMC_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
   if (it == s->reqs.end() || it->second->state < 2) return MDMC_ERR_INVALID;
   Req *r = it->second;
@@ -529,11 +621,13 @@ int mdmc_server_release(mdmc_server *s, uint64_t id) {
   }
   s->reqs.erase(it);
   delete r;
+  wake(s);  // freed memory may let queued work start
   return MDMC_OK;
 }
 
 int mdmc_server_set_option(mdmc_server *s, int option, int value) {
   if (!s) return MDMC_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
   for (Lane &l : s->lanes) {
     const int rc = mdmc_ctx_set_option(l.ctx, option, value);
     if (rc != MDMC_OK) return rc;
@@ -543,6 +637,7 @@ int mdmc_server_set_option(mdmc_server *s, int option, int value) {
 
 int mdmc_server_get_stats(mdmc_server *s, mdmc_server_stats *st) {
   if (!s || !st) return MDMC_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->mu);
   s->note_busy();
   std::memset(st, 0, sizeof(*st));
   st->submitted = s->submitted;

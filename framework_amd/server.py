@@ -168,7 +168,7 @@ class Server:
 
 def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set | None = None) -> dict:
     """Submits requests[i] at arrivals[i] seconds (server clock, from now) and
-    drives the loop until all have finished.  Results whose index is in `keep`
+    collects results until all have finished.  Results whose index is in `keep`
     are copied to the host before release.  Returns latencies, timings and
     the kept meshes."""
     n = len(requests)

@@ -239,7 +239,8 @@ int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value);
  * extraction jobs by their device-memory footprint, batches the admitted ones
  * (continuous batching over a few streams, so that jobs arriving while
  * others run are started as memory and streams free up) and keeps fields and
- * results resident.  One host thread drives a server. ---------------------- */
+ * results resident.  A worker thread of the server launches work and notices
+ * completions; every function may be called from any host thread. -------- */
 
 typedef struct mdmc_server mdmc_server;
 
@@ -284,9 +285,9 @@ int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out);
 void mdmc_server_destroy(mdmc_server *srv);
 /* Queues a request; *id identifies its result. */
 int mdmc_server_submit(mdmc_server *srv, const mdmc_request *req, uint64_t *id);
-/* Advances the loop (admission, launches, completions) and returns up to
- * `max` finished results.  wait != 0: block until at least one result is
- * ready (or nothing is queued or in flight). */
+/* Returns up to `max` finished results.  wait != 0: block until at least
+ * one result is ready, or until the server has nothing left that it could
+ * finish without another call (nothing in flight, nothing it can admit). */
 int mdmc_server_poll(mdmc_server *srv, int wait, mdmc_result *out, uint32_t max, uint32_t *n);
 /* Copies a finished result's mesh to host memory (sizes from its counts). */
 int mdmc_server_copy_result(mdmc_server *srv, uint64_t id, float *h_vertices, int32_t *h_quads);
