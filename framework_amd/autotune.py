@@ -4,7 +4,7 @@ The tunable launch shapes are context options (MDMC_OPT_* in include/mdmc.h):
 
   single volume  MDMC_OPT_PT_ROWS              cell rows per classify tile (7, 15)
                  MDMC_OPT_PT_PLANES            cell planes per classify tile
-                 MDMC_OPT_PT_BAND              row blocks per XCD band
+                 MDMC_OPT_PT_BAND              columns per XCD group
                  MDMC_OPT_EMIT_BLOCKS_PER_CU   emit grid
   batches        MDMC_OPT_BATCH_PLANES         cell planes per batch classify tile
                  MDMC_OPT_CLASSIFY_BLOCKS_PER_CU  persistent classify grid
@@ -29,7 +29,7 @@ import numpy as np
 from .mdmc import (OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS_PER_CU, OPT_PT_BAND,
                    OPT_PT_PLANES, OPT_PT_ROWS, MDMCError)
 
-SINGLE_SPACE = {OPT_PT_ROWS: [0, 15], OPT_PT_PLANES: [0, 1, 2, 3], OPT_PT_BAND: [1, 2, 4],
+SINGLE_SPACE = {OPT_PT_ROWS: [0, 15], OPT_PT_PLANES: [0, 1, 2, 3], OPT_PT_BAND: [0, 2, 4],
                 OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}
 BATCH_SPACE = {OPT_BATCH_PLANES: [0, 1, 2, 4], OPT_CLASSIFY_BLOCKS_PER_CU: [0, 4, 8, 12],
                OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}

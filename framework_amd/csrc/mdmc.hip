@@ -122,7 +122,7 @@ int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt) {
   p->vec4 = (p->X % 4 == 0) && env_int("MDMC_VEC4", 1) != 0;
   p->ptR = pt_rows(W / 4);
   p->ptKP = pt_planes(W / 4);
-  // context options (0: default): cell rows and planes per tile, XCD band
+  // context options (0: default): cell rows and planes per tile, columns per group
   const int r_opt = opt ? opt[MDMC_OPT_PT_ROWS] : 0;
   if (r_opt == 15 && pt_shape_supported(W / 4, 15, (int)pt_planes15(W / 4))) {
     p->ptR = 15;
@@ -134,9 +134,11 @@ int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt) {
   const uint32_t nrb = (p->cy + rows - 1) / rows;
   const int band_opt = opt ? opt[MDMC_OPT_PT_BAND] : 0;
   p->band = band_opt > 0 && band_opt <= 64 ? (uint32_t)band_opt : 1u;
-  p->NS = (nrb + 8 * p->band - 1) / (8 * p->band);
-  const uint32_t kp = p->ptKP;
-  p->pt_blocks = 8ull * p->band * p->NS * p->XT * ((p->cz + kp - 1) / kp);
+  p->NRBp = nrb;
+  p->NKBp = (p->cz + p->ptKP - 1) / p->ptKP;
+  // 8 XCDs, each at most ceil(NKB * groups / 8) group tiles of `band` columns
+  const uint64_t ng = ((uint64_t)p->XT * nrb + p->band - 1) / p->band;
+  p->pt_blocks = 8ull * p->band * (((uint64_t)p->NKBp * ng + 7) / 8);
   return MDMC_OK;
 }
 
@@ -337,7 +339,7 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   {
     ProfScope ps(ctx, 0, st);
     if (!p.zmarch) {
-      PtArgs pa{ca, p.band, p.NS};
+      PtArgs pa{ca, p.band, p.NKBp, p.NRBp};
       const bool vec = p.vec4 && ((uintptr_t)d_field % 16 == 0);
       rc = launch_classify_pt(p.W, p.ptR, p.ptKP, vec, pa, (uint32_t)p.pt_blocks, st);
     } else {

@@ -21,7 +21,7 @@ struct Plan {
   int vec4;    // rows are a multiple of 4 points
   int zmarch;  // use the z-marching classify kernel
   uint32_t ptR, ptKP;        // plane tiles: cell rows, cell planes
-  uint32_t band, NS;         // plane tiles: row blocks per XCD band, bands per plane block
+  uint32_t band, NKBp, NRBp; // plane tiles: columns per group, plane blocks, row blocks
   uint64_t pt_blocks;        // plane-tile grid
 };
 

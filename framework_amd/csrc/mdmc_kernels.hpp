@@ -337,12 +337,16 @@ __global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
 // Plane-tile classify: the main kernel for rows that are a multiple of 4
 // points.  A 256-thread workgroup owns one tile = R cell rows x one x-tile
 // (256*W4 cells) x KP cell planes.  Workgroups are dispatched round-robin over
-// the 8 XCDs (block b runs on XCD b % 8), each with its own L2; XCD x is given
-// the x-th band of row blocks of every plane block, walked in memory order
-// (x-tile, then row block, then plane block).  The point row shared by two
-// neighbouring row blocks and the point plane shared by two plane blocks are
-// then read twice by the same XCD a few tiles apart and come from its L2, so
-// HBM sees (almost) every field value once.
+// the 8 XCDs (block b runs on XCD b % 8), each with its own L2.  A column is
+// one x-tile of one row block; columns form groups of B (the band option).
+// In every plane block, XCD x takes groups x, x+8, x+16, ... and walks them
+// in memory order, plane block after plane block, so the point plane shared
+// by two plane blocks is read twice by the same XCD a few tiles apart and
+// comes from its L2, and all XCDs stream through the same few planes at a
+// time (a small working set for the address translation caches).  The groups
+// left over when their count is not a multiple of 8 are dealt out over the
+// XCDs eight plane blocks at a time, so every XCD gets the same number of
+// tiles (within one group) and none of them finishes late.
 //
 // Phase 1: wave w converts point rows r = w, w+4, ... (R+1 of them, so every
 // wave has the same share) of all KP+1 point planes.  A lane owns 4
@@ -370,9 +374,44 @@ __host__ __device__ constexpr uint32_t key_to_cfg(uint32_t key) {
 
 struct PtArgs {
   ClassifyArgs c;
-  uint32_t B;   // row blocks per XCD band
-  uint32_t NS;  // bands of 8*B row blocks per plane block
+  uint32_t B;    // columns per group
+  uint32_t NKB;  // plane blocks
+  uint32_t NRB;  // row blocks
 };
+
+// Tile of block b (XCD b % 8, b / 8-th of that XCD's walk); blocks past the
+// XCD's share get plane block NKB, a tile outside the grid.
+struct PtTile {
+  uint32_t xt, rb, kb;
+};
+__host__ __device__ inline PtTile pt_tile_of(uint32_t b, uint32_t XT, uint32_t NRB, uint32_t NKB, uint32_t B) {
+  const uint32_t x = b & 7u, s = b >> 3;
+  const uint32_t cols = XT * NRB;
+  const uint32_t ng = (cols + B - 1u) / B;  // column groups
+  const uint32_t q = ng >> 3, r = ng & 7u;  // groups per XCD per plane block; left over
+  const uint32_t gs = s / B, w = s - gs * B;  // group tile of the walk, column in the group
+  const uint32_t per = 8u * q + r;          // group tiles per XCD per 8 plane blocks
+  const uint32_t S = gs / per, o = gs - S * per;
+  PtTile t{0u, 0u, NKB};
+  if ((uint64_t)S * 8u >= NKB) return t;
+  const uint32_t nk = NKB - 8u * S < 8u ? NKB - 8u * S : 8u;
+  uint32_t kb, g;
+  if (o < nk * q) {  // the XCD's own groups, plane block by plane block
+    kb = 8u * S + o / q;
+    g = (o % q) * 8u + x;
+  } else {  // left-over group tiles e = x, x + 8, ... of these plane blocks
+    const uint32_t e = x + 8u * (o - nk * q);
+    if (e >= nk * r) return t;
+    kb = 8u * S + e / r;
+    g = 8u * q + e % r;
+  }
+  const uint32_t col = g * B + w;
+  if (col >= cols) return t;
+  t.xt = col % XT;
+  t.rb = col / XT;
+  t.kb = kb;
+  return t;
+}
 
 // Cell rows per plane tile (R + 1 point rows, 2 per wave) and cell planes per
 // tile (KP + 1 point planes), chosen so that all loads of a wave fit in
@@ -622,12 +661,8 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
     zero_scan_state(a, threadIdx.x, kPtThreads);
     if (a.counts_out != nullptr && threadIdx.x < 4) a.counts_out[threadIdx.x] = 0u;
   }
-  const uint32_t s_idx = blockIdx.x >> 3;  // position in this XCD's sequence
-  const uint32_t xt = s_idx % a.XT;
-  const uint32_t u = s_idx / a.XT;
-  const uint32_t u2 = u / pa.B;
-  const uint32_t rb = ((u2 % pa.NS) * 8u + (blockIdx.x & 7u)) * pa.B + u % pa.B;
-  classify_pt_tile<W4, R, KP, VEC>(a, xt, rb, u2 / pa.NS, s_w, s_tab);
+  const PtTile t = pt_tile_of(blockIdx.x, a.XT, pa.NRB, pa.NKB, pa.B);
+  classify_pt_tile<W4, R, KP, VEC>(a, t.xt, t.rb, t.kb, s_w, s_tab);
 }
 
 // ---------------------------------------------------------------------------

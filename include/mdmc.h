@@ -230,7 +230,7 @@ enum {
   MDMC_OPT_CLASSIFY_BLOCKS_PER_CU = 1, /* persistent batch classify grid */
   MDMC_OPT_EMIT_BLOCKS_PER_CU = 2,     /* emit grid (single and batch) */
   MDMC_OPT_PT_PLANES = 3,              /* cell planes per single-volume classify tile */
-  MDMC_OPT_PT_BAND = 4,                /* single-volume classify: row blocks per XCD band */
+  MDMC_OPT_PT_BAND = 4,                /* single-volume classify: columns per XCD group */
   MDMC_OPT_PT_ROWS = 5                 /* cell rows per single-volume classify tile: 7 or 15 */
 };
 int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value);
