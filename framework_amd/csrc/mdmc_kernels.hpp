@@ -430,10 +430,13 @@ struct PtShape {
 };
 
 // Table indexed by key: c_cfg_count of the key's configuration | cfg << 16.
-__device__ __forceinline__ void pt_init_table(uint32_t *s_tab) {
+// This thread's entry (a global load) ...
+__device__ __forceinline__ uint32_t pt_table_entry() {
   const uint32_t cfg = key_to_cfg(threadIdx.x);
-  s_tab[threadIdx.x] = (uint32_t)c_cfg_count[cfg] | (cfg << 16);
+  return (uint32_t)c_cfg_count[cfg] | (cfg << 16);
 }
+// ... and the whole table in LDS (the caller synchronises).
+__device__ __forceinline__ void pt_init_table(uint32_t *s_tab) { s_tab[threadIdx.x] = pt_table_entry(); }
 
 // Zeroes the state the scan expects to start from (first workgroup only).
 __device__ __forceinline__ void zero_scan_state(const ClassifyArgs &a, uint32_t tid, uint32_t nthreads) {
@@ -448,7 +451,8 @@ __device__ __forceinline__ void zero_scan_state(const ClassifyArgs &a, uint32_t 
 // lane's 4 points are one 16-byte load; otherwise 4 separate loads.
 template <int W4, int R, int KP, bool VEC>
 __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t xt, uint32_t rb,
-                                                 uint32_t kb, uint32_t *s_w, const uint32_t *s_tab) {
+                                                 uint32_t kb, uint32_t *s_w, uint32_t *s_tab,
+                                                 uint32_t tab_entry = 0u, bool store_tab = false) {
   using S = PtShape<W4, R, KP>;
   constexpr int kRowsPerWave = (R + 1) / 4;
   constexpr uint32_t CAP = 256u * W4;
@@ -538,6 +542,9 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
       }
     }
   }
+  // the table entry was loaded before the field rows; storing it here keeps
+  // its wait behind theirs
+  if (store_tab) s_tab[threadIdx.x] = tab_entry;
   __syncthreads();
   if (!tile_ok) return;  // uniform over the workgroup
 
@@ -656,13 +663,13 @@ __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
   __shared__ uint32_t s_tab[256];
   const ClassifyArgs &a = pa.c;
-  pt_init_table(s_tab);
+  const uint32_t tab = pt_table_entry();
   if (blockIdx.x == 0) {
     zero_scan_state(a, threadIdx.x, kPtThreads);
     if (a.counts_out != nullptr && threadIdx.x < 4) a.counts_out[threadIdx.x] = 0u;
   }
   const PtTile t = pt_tile_of(blockIdx.x, a.XT, pa.NRB, pa.NKB, pa.B);
-  classify_pt_tile<W4, R, KP, VEC>(a, t.xt, t.rb, t.kb, s_w, s_tab);
+  classify_pt_tile<W4, R, KP, VEC>(a, t.xt, t.rb, t.kb, s_w, s_tab, tab, true);
 }
 
 // ---------------------------------------------------------------------------
