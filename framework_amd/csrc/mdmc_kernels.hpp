@@ -429,7 +429,10 @@ struct PtShape {
   static constexpr uint32_t kLdsWords = KP * kPlaneWords;
 };
 
-// Table indexed by key: c_cfg_count of the key's configuration | cfg << 16.
+// Table indexed by key: c_cfg_count of the key's configuration | cfg << 16,
+// followed by the staging of a tile's row counts (kPtRowsMax uint2).
+constexpr uint32_t kPtRowsMax = 64;
+constexpr uint32_t kPtTabWords = 256 + 2 * kPtRowsMax;
 // This thread's entry (a global load) ...
 __device__ __forceinline__ uint32_t pt_table_entry() {
   const uint32_t cfg = key_to_cfg(threadIdx.x);
@@ -454,6 +457,8 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
                                                  uint32_t kb, uint32_t *s_w, uint32_t *s_tab,
                                                  uint32_t tab_entry = 0u, bool store_tab = false) {
   using S = PtShape<W4, R, KP>;
+  static_assert(KP * R <= kPtRowsMax, "row counts are staged in kPtRowsMax LDS entries");
+  uint2 *s_cnt = reinterpret_cast<uint2 *>(s_tab + 256);
   constexpr int kRowsPerWave = (R + 1) / 4;
   constexpr uint32_t CAP = 256u * W4;
   constexpr uint32_t kRowWords = S::kRowWords;
@@ -653,7 +658,14 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
       cnt_v += (tot >> 9) & 0x7FFu;
       cnt_q += tot >> 20;
     }
-    if (lane == 0) a.seg_counts[seg] = make_uint2(cnt_a | (cnt_q << 16), cnt_v);
+    if (lane == 0) s_cnt[m] = make_uint2(cnt_a | (cnt_q << 16), cnt_v);
+  }
+  // the tile's row counts leave in one store (a store per row costs more)
+  __syncthreads();
+  if (threadIdx.x < (uint32_t)(KP * R)) {
+    const uint32_t m = threadIdx.x, cp = m / R, r = m - cp * R;
+    const uint32_t y = j0 + r, z = k + cp;
+    if (y < g.cy && z < g.cz) a.seg_counts[xt + a.XT * (y + g.cy * z)] = s_cnt[m];
   }
 }
 
@@ -661,7 +673,7 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
 template <int W4, int R, int KP, bool VEC>
 __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
-  __shared__ uint32_t s_tab[256];
+  __shared__ __align__(16) uint32_t s_tab[kPtTabWords];
   const ClassifyArgs &a = pa.c;
   const uint32_t tab = pt_table_entry();
   if (blockIdx.x == 0) {
@@ -707,7 +719,7 @@ struct BatchArgs {
 template <int W4, int R, int KP>
 __global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
-  __shared__ uint32_t s_tab[256];
+  __shared__ __align__(16) uint32_t s_tab[kPtTabWords];
   __shared__ uint32_t s_toff[kMaxBatch + 1];
   pt_init_table(s_tab);
   for (uint32_t i = threadIdx.x; i <= b.nvol; i += kPtThreads)
