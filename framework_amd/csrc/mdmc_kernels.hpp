@@ -584,39 +584,49 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
       }
       lact &= xvalid[c];
       if (__ballot(lact != 0u) == 0ull) continue;
-      // per cell: variant, vertex count, quad count (0 for inactive cells)
-      uint32_t pk = 0, info4 = 0, cfgs = 0;
-      if (lact) {
+      // per cell: table entry (0 for inactive cells, so they count nothing)
+      uint32_t e4[4];
+      uint32_t amb = 0;  // bit q: cell q is a 2B/3B configuration
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t ent = s_tab[(keys >> (8 * q)) & 0xFFu];
+        e4[q] = ((lact >> q) & 1u) ? ent : 0u;
+        amb |= ((e4[q] >> 11) & 1u) << q;
+      }
+      // 2B/3B cells (rare): the variant depends on the neighbour across the
+      // listed face, read from the field
+      uint32_t l2m = 0;
+      if (__ballot(amb != 0u) != 0ull) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (!((lact >> q) & 1u)) continue;
-          const uint32_t e32 = s_tab[(keys >> (8 * q)) & 0xFFu];
-          const uint32_t e = e32 & 0xFFFFu;
-          cfgs |= (e32 >> 16) << (8 * q);
-          uint32_t l2 = 0;
-          if (e & 0x800u) {  // 2B/3B: check the neighbour across the listed face
-            const uint32_t x = xb + q;
-            int nx = (int)x, ny = (int)y, nz = (int)z;
-            switch ((e >> 12) & 7u) {
-              case 0: nx += 1; break;
-              case 1: nx -= 1; break;
-              case 2: ny += 1; break;
-              case 3: ny -= 1; break;
-              case 4: nz += 1; break;
-              default: nz -= 1; break;
-            }
-            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.cx && ny < (int)g.cy && nz < (int)g.cz)
-              l2 = (c_cfg_count[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)] >> 11) & 1u;
+          if (!((amb >> q) & 1u)) continue;
+          int nx = (int)(xb + q), ny = (int)y, nz = (int)z;
+          switch ((e4[q] >> 12) & 7u) {
+            case 0: nx += 1; break;
+            case 1: nx -= 1; break;
+            case 2: ny += 1; break;
+            case 3: ny -= 1; break;
+            case 4: nz += 1; break;
+            default: nz -= 1; break;
           }
-          const uint32_t nv = (e >> (4 * l2)) & 0xFu;
-          const bool xin = xb + q + 1 < g.cx;
-          const uint32_t qmask = (uint32_t)(xin && yin) | ((uint32_t)(xin && zin) << 1) |
-                                 ((uint32_t)(yin && zin) << 2);
-          const uint32_t nq = (uint32_t)__popc((e >> 8) & 7u & qmask);
-          // per-cell byte: nv (3 bits) | nq (2 bits) << 3 | variant << 5
-          info4 |= (nv | (nq << 3) | (l2 << 5)) << (8 * q);
-          pk += 1u | (nv << 9) | (nq << 20);
+          if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.cx && ny < (int)g.cy && nz < (int)g.cz)
+            l2m |= ((c_cfg_count[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)] >> 11) & 1u) << q;
         }
+      }
+      // per cell: variant, vertex count, quad count
+      uint32_t pk = 0, info4 = 0, cfgs = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t e = e4[q] & 0xFFFFu, l2 = (l2m >> q) & 1u;
+        cfgs |= (e4[q] >> 16) << (8 * q);
+        const uint32_t nv = (e >> (4 * l2)) & 0xFu;
+        const bool xin = xb + q + 1 < g.cx;
+        const uint32_t qmask = (uint32_t)(xin && yin) | ((uint32_t)(xin && zin) << 1) |
+                               ((uint32_t)(yin && zin) << 2);
+        const uint32_t nq = (uint32_t)__popc((e >> 8) & 7u & qmask);
+        // per-cell byte: nv (3 bits) | nq (2 bits) << 3 | variant << 5
+        info4 |= (nv | (nq << 3) | (l2 << 5)) << (8 * q);
+        pk += ((lact >> q) & 1u) | (nv << 9) | (nq << 20);
       }
       const uint32_t incl = wave_incl_scan_dpp(pk);
       const uint32_t excl = incl - pk;
@@ -669,7 +679,7 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
   }
 }
 
-// Single volume: block b -> tile through the XCD band mapping.
+// Single volume: block b -> tile through the XCD mapping (pt_tile_of).
 template <int W4, int R, int KP, bool VEC>
 __global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
