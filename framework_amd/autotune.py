@@ -7,7 +7,7 @@ The tunable launch shapes are context options (MDMC_OPT_* in include/mdmc.h):
                  MDMC_OPT_PT_BAND              columns per XCD group
                  MDMC_OPT_EMIT_BLOCKS_PER_CU   emit grid
   batches        MDMC_OPT_BATCH_PLANES         cell planes per batch classify tile
-                 MDMC_OPT_CLASSIFY_BLOCKS_PER_CU  persistent classify grid
+                 MDMC_OPT_CLASSIFY_BLOCKS_PER_CU  persistent classify grid (0: a workgroup per tile)
                  MDMC_OPT_EMIT_BLOCKS_PER_CU   emit grid
 
 None of them changes a result, only how the work is laid over the chip, so

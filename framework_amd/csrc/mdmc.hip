@@ -614,15 +614,30 @@ int fill_batch_table(mdmc_ctx *ctx, int slot, const mdmc_volume *vols, uint32_t 
 // see) of a batch.
 int batch_layout(const mdmc_volume *vols, uint32_t n, bool check_fields, Plan *agg, uint32_t *W4_out,
                  uint32_t *KP_out) {
-  uint32_t W4 = 1;
   for (uint32_t v = 0; v < n; ++v) {
     const mdmc_volume &mv = vols[v];
     if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
     if (check_fields && !mv.field) return fail(MDMC_ERR_INVALID, "null field in batch");
     if ((uint64_t)mv.dims[0] * mv.dims[1] * mv.dims[2] > 0xFFFFFFFFull)
       return fail(MDMC_ERR_TOO_LARGE, "grid has more than 2^32-1 points");
-    const uint32_t cx = mv.dims[0] - 1;
-    while (W4 < 4 && 256u * W4 < cx) W4 <<= 1;
+  }
+  // x-tile width: every volume may span several x-tiles, so take the width
+  // with the fewest padded cells over the batch (lanes past a row's end do
+  // no work), the wider on a tie (fewer tiles)
+  uint32_t W4 = 1;
+  uint64_t best = ~0ull;
+  for (uint32_t w : {4u, 2u, 1u}) {
+    uint64_t padded = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      const mdmc_volume &mv = vols[v];
+      if (mv.dims[0] < 2 || mv.dims[1] < 2 || mv.dims[2] < 2) continue;
+      const uint64_t c = 256ull * w;
+      padded += (mv.dims[0] - 1 + c - 1) / c * c * (mv.dims[1] - 1) * (uint64_t)(mv.dims[2] - 1);
+    }
+    if (padded < best) {
+      best = padded;
+      W4 = w;
+    }
   }
   const uint32_t KP = pt_planes(W4);
   const uint32_t cap = 256u * W4;
@@ -698,8 +713,11 @@ int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts 
   ba.zero.counts_out = bcounts;
   {
     ProfScope ps(ctx, 0, st);
-    const int bpc = ctx->opt[MDMC_OPT_CLASSIFY_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_CLASSIFY_BLOCKS_PER_CU] : 6;
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * bpc));
+    // default: a workgroup per tile (each XCD's workgroups take its share of
+    // the tiles in order); the option makes the grid persistent
+    const int bpc = ctx->opt[MDMC_OPT_CLASSIFY_BLOCKS_PER_CU];
+    const uint64_t want = bpc > 0 ? (uint64_t)ctx->num_cus * bpc : ((uint64_t)tiles + 7) / 8 * 8;
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, ((uint64_t)tiles + 7) / 8 * 8));
     if (launch_classify_batch(W4, KP, ba, blocks, st) != 0) return fail(MDMC_ERR_INVALID, "no batch classify kernel");
   }
   HIP_TRY(hipGetLastError());

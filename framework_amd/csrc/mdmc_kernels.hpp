@@ -724,8 +724,9 @@ struct BatchArgs {
   ClassifyArgs zero;      // scan state of the batch (zeroed by block 0)
 };
 
-// Persistent classify over the flattened tiles of all volumes (block b takes
-// tiles b, b + G, ...; tiles of a volume are in memory order).
+// Classify over the flattened tiles of all volumes (tiles of a volume in
+// memory order): with a grid of at least the tile count a workgroup per tile,
+// with a smaller grid each workgroup walks several (persistent).
 template <int W4, int R, int KP>
 __global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];

@@ -75,6 +75,31 @@ def test_batch_matches_single_volume(fa_mod, torch_cuda):
     sctx.close()
 
 
+def test_batch_with_volumes_spanning_several_x_tiles(fa_mod, torch_cuda):
+    """Narrow volumes next to wide ones: the batch takes 256-cell x-tiles and
+    the wide volumes span 2-3 of them, while single extractions of the same
+    fields use one 512- or 1024-cell x-tile.  Results must not differ."""
+    gen = fa_mod.Context()
+    dims_list = [(600, 20, 15), (40, 40, 40), (300, 30, 20), (520, 9, 9), (257, 12, 11), (33, 60, 7)]
+    fields = [gen.field_gyroid(d, period=9.0, seed=70 + i, noise_amp=0.3) if i % 2 else
+              gen.field_blobs(d, seed=80 + i, nblobs=10, sigma=(2.0, 8.0)) for i, d in enumerate(dims_list)]
+    torch_cuda.cuda.synchronize()
+    isos = [0.05, -0.1, 0.0, 0.1, 0.02, -0.05]
+    bctx, sctx = fa_mod.Context(), fa_mod.Context()
+    res = bctx.batch_extract(fields, isos)
+    for i, (f, iso) in enumerate(zip(fields, isos)):
+        v, q, c = _single(sctx, f, iso)
+        bv, bq, bc = res[i]
+        assert (bc["num_active"], bc["num_vertices"], bc["num_quads"]) == \
+            (c["num_active"], c["num_vertices"], c["num_quads"]), dims_list[i]
+        assert c["num_quads"] > 0, dims_list[i]
+        assert np.array_equal(bq.cpu().numpy(), q), dims_list[i]
+        assert np.array_equal(bv.cpu().numpy().view(np.uint32), v.view(np.uint32)), dims_list[i]
+    bctx.close()
+    sctx.close()
+    gen.close()
+
+
 def test_batch_against_oracle(fa_mod, torch_cuda):
     fields, isos = _mixed_batch(fa_mod, torch_cuda, 6, seed=11, max_dim=48)
     ctx = fa_mod.Context()
