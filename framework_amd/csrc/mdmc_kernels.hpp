@@ -594,23 +594,46 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
         amb |= ((e4[q] >> 11) & 1u) << q;
       }
       // 2B/3B cells (rare): the variant depends on the neighbour across the
-      // listed face, read from the field
+      // listed face.  Inside the tile the neighbour's corner bits are in LDS
+      // (the word one cell or lane over, or of the next row or plane); only a
+      // neighbour outside the tile is read from the field.
       uint32_t l2m = 0;
       if (__ballot(amb != 0u) != 0ull) {
+        const uint32_t widx = cp * kPlaneWords + (r + 1u) * kRowWords + c * kWave + lane;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           if (!((amb >> q) & 1u)) continue;
+          const uint32_t face = (e4[q] >> 12) & 7u;
           int nx = (int)(xb + q), ny = (int)y, nz = (int)z;
-          switch ((e4[q] >> 12) & 7u) {
-            case 0: nx += 1; break;
-            case 1: nx -= 1; break;
-            case 2: ny += 1; break;
-            case 3: ny -= 1; break;
-            case 4: nz += 1; break;
-            default: nz -= 1; break;
+          bool in_tile;
+          uint32_t nidx = widx, nq = (uint32_t)q;
+          switch (face) {
+            case 0:
+              nx += 1;
+              in_tile = q < 3 || lane + kWave * c + 1u < kWave * W4;
+              nidx = widx + (q == 3 ? 1u : 0u);
+              nq = (uint32_t)(q + 1) & 3u;
+              break;
+            case 1:
+              nx -= 1;
+              in_tile = q > 0 || lane + kWave * c > 0u;
+              nidx = widx - (q == 0 ? 1u : 0u);
+              nq = (uint32_t)(q + 3) & 3u;
+              break;
+            case 2: ny += 1; in_tile = r + 1u < (uint32_t)R; nidx = widx + kRowWords; break;
+            case 3: ny -= 1; in_tile = r > 0u; nidx = widx - kRowWords; break;
+            case 4: nz += 1; in_tile = cp + 1u < (uint32_t)KP; nidx = widx + kPlaneWords; break;
+            default: nz -= 1; in_tile = cp > 0u; nidx = widx - kPlaneWords; break;
           }
-          if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (int)g.cx && ny < (int)g.cy && nz < (int)g.cz)
-            l2m |= ((c_cfg_count[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)] >> 11) & 1u) << q;
+          if (nx < 0 || ny < 0 || nz < 0 || nx >= (int)g.cx || ny >= (int)g.cy || nz >= (int)g.cz) continue;
+          uint32_t ent;
+          if (in_tile) {
+            const uint32_t tq = (s_w[nidx] >> nq) & 0x03030303u;
+            ent = s_tab[((tq * 0x41041u) >> 18) & 0xFFu];
+          } else {
+            ent = c_cfg_count[cell_cfg_global(g, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz)];
+          }
+          l2m |= ((ent >> 11) & 1u) << q;
         }
       }
       // per cell: variant, vertex count, quad count
