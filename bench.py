@@ -13,7 +13,8 @@ streams (--streams), so that one extraction's short scan and emit overlap
 the next one's classify; the one-stream time per extraction is reported
 beside it (config.ms_per_extraction_one_stream).  `roofline` is for the dominant kernel (longest average launch),
 timed live with HIP events the library records around each launch on its
-stream.  `cpu_baseline` is the C oracle (single thread) on the same field.
+stream.  `cpu_baseline` is the C oracle on the same field, on up to 16 threads
+(and on one thread, `cpu_baseline.single_thread`).
 
 Four more legs go into the same line:
   "host_path": the drop-in host entry (mdmc_run, replacing run_dmc) on the
@@ -207,21 +208,38 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
 
-        log("CPU baseline (oracle, 1 thread)")
         fnp = field.cpu().numpy()
-        times = []
-        tstart = time.perf_counter()
-        while True:
-            t = time.perf_counter()
-            r = oracle.run(fnp, ISO)
-            times.append(time.perf_counter() - t)
-            if time.perf_counter() - tstart > args.cpu_seconds or len(times) >= 5:
-                break
-        assert r["num_quads"] == Q and r["num_vertices"] == V
-        tmed = sorted(times)[len(times) // 2]
-        cpu = {"value": N / tmed / 1e9, "unit": "Gvoxels/s", "cores": 1, "kind": "port",
-               "sample": f"full 512^3 blob field, {len(times)} run(s), median {tmed:.3f} s "
-                         f"(C oracle restating dmc::run_dmc, one thread, host of the GPU box)"}
+
+        def time_oracle(threads: int, budget: float) -> tuple:
+            oracle.set_threads(threads)
+            times = []
+            tstart = time.perf_counter()
+            while True:
+                t = time.perf_counter()
+                r = oracle.run(fnp, ISO)
+                times.append(time.perf_counter() - t)
+                if time.perf_counter() - tstart > budget or len(times) >= 5:
+                    break
+            assert r["num_quads"] == Q and r["num_vertices"] == V
+            return sorted(times)[len(times) // 2], len(times)
+
+        # the per-cell loops on every core of this process's CPU share (at
+        # most 16), and one thread
+        try:
+            ncpu = len(os.sched_getaffinity(0))
+        except AttributeError:
+            ncpu = os.cpu_count() or 1
+        nthr = max(1, min(16, ncpu))
+        log(f"CPU baseline (oracle, {nthr} threads and 1 thread)")
+        tmul, nmul = time_oracle(nthr, args.cpu_seconds / 2)
+        tone, none = time_oracle(1, args.cpu_seconds / 2)
+        oracle.set_threads(1)
+        cpu = {"value": N / tmul / 1e9, "unit": "Gvoxels/s", "cores": nthr, "kind": "port",
+               "sample": f"full 512^3 blob field, {nmul} run(s), median {tmul:.3f} s (C oracle restating "
+                         f"dmc::run_dmc, OpenMP over cell planes and active cells, {nthr} threads, host of "
+                         f"the GPU box)",
+               "single_thread": {"value": N / tone / 1e9, "cores": 1,
+                                 "sample": f"same field, {none} run(s), median {tone:.3f} s"}}
 
     host = None if args.skip_host else bench_host(fa, field.cpu().numpy(), V, Q)
     dense = None if args.skip_dense else bench_dense(fa, torch)

@@ -74,6 +74,8 @@ def lib():
         L.oracle_field_blobs.argtypes = [f3, u3, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.oracle_field_gyroid.argtypes = [f3, u3, ctypes.c_double, ctypes.c_uint64, ctypes.c_double]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -84,6 +86,12 @@ def _fptr(a):
 
 def _dims(dims):
     return (ctypes.c_uint32 * 3)(*[int(d) for d in dims])
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads for the oracle's independent per-cell loops (results do
+    not depend on the count)."""
+    lib().oracle_set_threads(int(n))
 
 
 def luts_bytes() -> bytes:

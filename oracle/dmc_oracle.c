@@ -38,6 +38,9 @@
 
 static mdmc_luts g_luts;
 static int g_luts_ready = 0;
+static int g_threads = 1; /* OpenMP threads of the independent per-cell loops */
+
+void oracle_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 
 static const mdmc_luts *luts(void) {
   if (!g_luts_ready) {
@@ -136,9 +139,13 @@ static uint32_t edge_vertex_index(const cellrec *r, const mdmc_luts *L, int e) {
  * edges whose owner lies outside the grid are skipped. */
 static void reaverage(oracle_result *out, const cellrec *rec, const uint32_t *map, uint32_t A, uint32_t cx,
                       uint32_t cy, uint32_t cz, const mdmc_luts *L, int want_fp64, int first) {
-  uint32_t a;
+  int64_t a;
+  uint32_t anomalies = 0, unset = 0;
   (void)cz;
-  for (a = 0; a < A; ++a) {
+  /* independent per cell: reads owners' edge-point slots, writes its own
+   * iso-vertex slots */
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : anomalies, unset)
+  for (a = 0; a < (int64_t)A; ++a) {
     uint32_t id = rec[a].index1d;
     int64_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
     int lut2 = (rec[a].info >> 7) & 1, e, m;
@@ -153,7 +160,7 @@ static void reaverage(oracle_result *out, const cellrec *rec, const uint32_t *ma
       if (oi < 0 || oj < 0 || ok < 0) continue;
       om = map[((size_t)ok * cy + (size_t)oj) * cx + (size_t)oi];
       if (om == INVALID_U32 || !((rec[om].info >> edge_shift(EDGE_OWNER_E[e])) & 1)) {
-        if (first) out->num_anomalies++;
+        if (first) anomalies++;
         continue;
       }
       src = edge_vertex_index(&rec[om], L, EDGE_OWNER_E[e]);
@@ -190,7 +197,7 @@ static void reaverage(oracle_result *out, const cellrec *rec, const uint32_t *ma
       } else {
         union { uint32_t u; float f; } qnan;
         qnan.u = 0x7FC00000u;
-        if (first) out->num_unset_vertices++;
+        if (first) unset++;
         for (c = 0; c < 3; ++c) {
           out->vertices[3 * (size_t)dst + c] = qnan.f;
           if (want_fp64) out->vertices64[3 * (size_t)dst + c] = (double)qnan.f;
@@ -198,7 +205,8 @@ static void reaverage(oracle_result *out, const cellrec *rec, const uint32_t *ma
       }
     }
   }
-
+  out->num_anomalies += anomalies;
+  out->num_unset_vertices += unset;
 }
 
 /* ---- smoothing (num_smooth > 0): smooth_edge_vertices (dmc.cu:1868-1981)
@@ -325,9 +333,10 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
   uint32_t X = dims[0], Y = dims[1], Z = dims[2];
   uint32_t cx, cy, cz;
   size_t C, idx;
-  uint32_t *map = NULL;
+  uint32_t *map = NULL, *plane_base = NULL;
   cellrec *rec = NULL;
-  uint32_t A = 0, V = 0, Q = 0, a;
+  uint32_t A = 0, V = 0, Q = 0, a, ties = 0, anomalies = 0;
+  int64_t kk, aa;
   float range[3];
   double range64[3];
 
@@ -347,16 +356,39 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
    * exclusive scan (dmc.cu:2092-2098) and compact_voxel_flags_kern
    * (dmc.cu:1134-1158): a dense cell -> compact index map. */
   map = (uint32_t *)malloc(C * sizeof(uint32_t));
-  if (!map) return -1;
-  for (idx = 0; idx < C; ++idx) {
-    uint32_t i = (uint32_t)(idx % cx), j = (uint32_t)((idx / cx) % cy), k = (uint32_t)(idx / ((size_t)cx * cy));
-    float v[8];
-    uint8_t cfg;
-    cell_values(field, X, Y, i, j, k, v);
-    cfg = config_of(v, iso);
-    if (cfg != 0 && cfg != 0xFF) map[idx] = A++;
-    else map[idx] = INVALID_U32;
+  plane_base = (uint32_t *)malloc(((size_t)cz + 1) * sizeof(uint32_t));
+  if (!map || !plane_base) { free(map); free(plane_base); return -1; }
+  /* flags per cell and active counts per cell plane (planes in parallel),
+   * then the planes' exclusive prefix, then the compact numbers in cell
+   * order: the same map as one sequential count */
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (kk = 0; kk < (int64_t)cz; ++kk) {
+    uint32_t n = 0, i, j, k = (uint32_t)kk;
+    for (j = 0; j < cy; ++j)
+      for (i = 0; i < cx; ++i) {
+        float v[8];
+        uint8_t cfg;
+        size_t id = ((size_t)k * cy + j) * cx + i;
+        cell_values(field, X, Y, i, j, k, v);
+        cfg = config_of(v, iso);
+        if (cfg != 0 && cfg != 0xFF) { map[id] = 0; ++n; }
+        else map[id] = INVALID_U32;
+      }
+    plane_base[k] = n;
   }
+  for (idx = 0; idx < cz; ++idx) {
+    uint32_t n = plane_base[idx];
+    plane_base[idx] = A;
+    A += n;
+  }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (kk = 0; kk < (int64_t)cz; ++kk) {
+    uint32_t next = plane_base[kk];
+    size_t id = (size_t)kk * cy * cx, end = id + (size_t)cy * cx;
+    for (; id < end; ++id)
+      if (map[id] != INVALID_U32) map[id] = next++;
+  }
+  free(plane_base);
   out->num_active = A;
   if (A == 0) { free(map); return 0; }
   rec = (cellrec *)calloc(A, sizeof(cellrec));
@@ -368,12 +400,15 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
   if (!rec || !out->cell_index || !out->cfg || !out->info || !out->vertex_begin || !out->quad_begin) {
     free(map); free(rec); oracle_result_free(out); return -1;
   }
-  for (idx = 0; idx < C; ++idx)
-    if (map[idx] != INVALID_U32) rec[map[idx]].index1d = (uint32_t)idx;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (kk = 0; kk < (int64_t)C; ++kk)
+    if (map[kk] != INVALID_U32) rec[map[kk]].index1d = (uint32_t)kk;
 
   /* Pass 2: init_voxels_info_kern (dmc.cu:1173-1221): config plus bipolar and
    * ccw bits of the owned edges 6 (p2,p6), 9 (p5,p6), 10 (p7,p6). */
-  for (a = 0; a < A; ++a) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : ties)
+  for (aa = 0; aa < (int64_t)A; ++aa) {
+    uint32_t a = (uint32_t)aa;
     uint32_t id = rec[a].index1d;
     uint32_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
     float v[8];
@@ -385,19 +420,22 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
     for (l = 0; l < 3; ++l) {
       float v0 = v[first[l]], v1 = v[6];
       int bip = (v0 < iso) != (v1 < iso);
-      if (ref_bipolar(v0, v1, iso) && !bip) out->num_tie_edges++;
+      if (ref_bipolar(v0, v1, iso) && !bip) ties++;
       if (bip) {
         rec[a].info |= (uint8_t)(1u << l);
         if (v0 <= iso) rec[a].info |= (uint8_t)(1u << (l + 3));
       }
     }
   }
+  out->num_tie_edges += ties;
 
   /* Pass 3: correct_voxels_info_kern (dmc.cu:1345-1371) with
    * is_adjacent_ambiguous_config (dmc.cu:1301-1340): a 2B/3B cell whose
    * in-grid neighbour across its ambiguous face is also 2B/3B uses the
    * variant-1 table.  Only this cell is marked (dmc.cu:1368-1369). */
-  for (a = 0; a < A; ++a) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : anomalies)
+  for (aa = 0; aa < (int64_t)A; ++aa) {
+    uint32_t a = (uint32_t)aa;
     uint8_t face = L->amb_face[rec[a].cfg];
     uint32_t id = rec[a].index1d;
     int64_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
@@ -416,7 +454,7 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
     }
     if (i < 0 || j < 0 || k < 0 || i >= cx || j >= cy || k >= cz) continue;
     nmap = map[((size_t)k * cy + (size_t)j) * cx + (size_t)i];
-    if (nmap == INVALID_U32) { out->num_anomalies++; continue; }
+    if (nmap == INVALID_U32) { anomalies++; continue; }
     if (L->amb_face[rec[nmap].cfg] != MDMC_NOT_AMBIGUOUS) rec[a].info |= 0x80;
   }
 
@@ -437,7 +475,9 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
 
   /* Pass 5: sample_edge_intersection_vertices_kern (dmc.cu:1468-1514).  e6
    * runs p2 -> p6, e9 p5 -> p6, e10 p6 -> p7. */
-  for (a = 0; a < A; ++a) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (aa = 0; aa < (int64_t)A; ++aa) {
+    uint32_t a = (uint32_t)aa;
     uint32_t id = rec[a].index1d;
     uint32_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
     uint32_t w = rec[a].vertex_begin + L->num_vertex[(rec[a].info >> 7) & 1][rec[a].cfg];
@@ -511,7 +551,9 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
   /* Pass 8: generate_triangles_kern (dmc.cu:2011-2047) with
    * get_circular_vertices_by_edge (dmc.cu:1868-1892): the four ring cells'
    * iso vertices, ccw order 0,1,2,3 or cw order 0,3,2,1. */
-  for (a = 0; a < A; ++a) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : anomalies)
+  for (aa = 0; aa < (int64_t)A; ++aa) {
+    uint32_t a = (uint32_t)aa;
     uint32_t id = rec[a].index1d;
     int64_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
     uint32_t q = out->quad_begin[a];
@@ -531,9 +573,9 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
         uint32_t rm = map[((size_t)rk * cy + (size_t)rj) * cx + (size_t)ri];
         uint8_t lab;
         members[p] = rm;
-        if (rm == INVALID_U32) { out->num_anomalies++; vidx[p] = -1; continue; }
+        if (rm == INVALID_U32) { anomalies++; vidx[p] = -1; continue; }
         lab = L->edge_label[(rec[rm].info >> 7) & 1][rec[rm].cfg][re];
-        if (lab == MDMC_NO_VERTEX) { out->num_anomalies++; vidx[p] = -1; continue; }
+        if (lab == MDMC_NO_VERTEX) { anomalies++; vidx[p] = -1; continue; }
         vidx[p] = (int32_t)(rec[rm].vertex_begin + lab);
       }
       (void)members;
@@ -544,6 +586,7 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
       ++q;
     }
   }
+  out->num_anomalies += anomalies;
 
   /* smoothing iterations; the FP64 twin is left unsmoothed */
   {

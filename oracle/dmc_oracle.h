@@ -45,6 +45,10 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
                           const float xyz_max[3], float iso, int want_fp64, uint32_t num_smooth,
                           oracle_result *out);
 void oracle_result_free(oracle_result *r);
+/* Threads for the per-cell loops that are independent (the compaction and
+ * prefix sums stay in cell order, so results do not depend on it); 1 by
+ * default. */
+void oracle_set_threads(int n);
 
 /* The reference tables rebuilt by the shared builder, for hashing in tests:
  * edge_label[2][256][12], num_vertex[2][256], amb_face[256] (7168 bytes). */

@@ -53,9 +53,12 @@ def test_batch_scratch_query_without_gpu():
         vols[i].dims[0], vols[i].dims[1], vols[i].dims[2] = d
     b = ctypes.c_uint64()
     assert L.mdmc_batch_scratch_bytes(vols, 3, ctypes.byref(b)) == 0
-    # the record slots dominate: 8 bytes per slot, 512-cell x-tiles here (cx = 299)
-    segs = 63 * 63 + 19 * 9
-    assert b.value >= segs * 512 * 8
+    # the record slots dominate: 8 bytes per slot.  The x-tile width with the
+    # fewest padded cells is 256 here (63 -> 256, 299 -> 2 x 256 beats 512 and
+    # 512), so the 300-wide volume spans two x-tiles
+    segs = 63 * 63 + 2 * 19 * 9
+    assert b.value >= segs * 256 * 8
+    assert b.value < (63 * 63 + 19 * 9) * 512 * 8
     one = (mdmc.Volume * 1)()
     one[0].dims[0], one[0].dims[1], one[0].dims[2] = 1, 1, 1
     assert L.mdmc_batch_scratch_bytes(one, 1, ctypes.byref(b)) == 0 and b.value == 0

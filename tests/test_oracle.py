@@ -278,3 +278,27 @@ def test_unset_iso_vertex_is_nan():
     assert r["num_unset_vertices"] == 1
     assert np.isnan(r["vertices"][0]).all() and r["vertices"][0].view(np.uint32)[0] == 0x7FC00000
     assert r["num_quads"] == 0 or not (r["quads"] == 0).any()
+
+
+def test_threads_do_not_change_results():
+    """The per-cell loops run on several threads; the compact numbering and
+    the prefix sums stay in cell order, so every output array is the same
+    bitwise as with one thread (ties, 2B/3B cells, smoothing included)."""
+    rng = np.random.default_rng(4)
+    noise = rng.standard_normal((23, 31, 37)).astype(np.float32)
+    sphere = oracle.field_sphere((64, 64, 64), (32.0, 32.0, 32.0), 20.0)
+    gyro = oracle.field_gyroid((40, 36, 44), period=9.0, seed=2, noise_amp=0.3)
+    cases = [(noise, 0.1, 0), (sphere, 0.0, 0), (gyro, 0.0, 2)]
+    try:
+        for field, iso, ns in cases:
+            oracle.set_threads(1)
+            one = oracle.run(field, iso, fp64=True, num_smooth=ns)
+            oracle.set_threads(8)
+            many = oracle.run(field, iso, fp64=True, num_smooth=ns)
+            for key, val in one.items():
+                if isinstance(val, np.ndarray):
+                    assert np.array_equal(val.view(np.uint8), many[key].view(np.uint8)), key
+                else:
+                    assert val == many[key], key
+    finally:
+        oracle.set_threads(1)
