@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mdmc_tilemap.h"
+
 #pragma clang fp contract(off)
 
 namespace mdmc {
@@ -378,40 +380,6 @@ struct PtArgs {
   uint32_t NKB;  // plane blocks
   uint32_t NRB;  // row blocks
 };
-
-// Tile of block b (XCD b % 8, b / 8-th of that XCD's walk); blocks past the
-// XCD's share get plane block NKB, a tile outside the grid.
-struct PtTile {
-  uint32_t xt, rb, kb;
-};
-__host__ __device__ inline PtTile pt_tile_of(uint32_t b, uint32_t XT, uint32_t NRB, uint32_t NKB, uint32_t B) {
-  const uint32_t x = b & 7u, s = b >> 3;
-  const uint32_t cols = XT * NRB;
-  const uint32_t ng = (cols + B - 1u) / B;  // column groups
-  const uint32_t q = ng >> 3, r = ng & 7u;  // groups per XCD per plane block; left over
-  const uint32_t gs = s / B, w = s - gs * B;  // group tile of the walk, column in the group
-  const uint32_t per = 8u * q + r;          // group tiles per XCD per 8 plane blocks
-  const uint32_t S = gs / per, o = gs - S * per;
-  PtTile t{0u, 0u, NKB};
-  if ((uint64_t)S * 8u >= NKB) return t;
-  const uint32_t nk = NKB - 8u * S < 8u ? NKB - 8u * S : 8u;
-  uint32_t kb, g;
-  if (o < nk * q) {  // the XCD's own groups, plane block by plane block
-    kb = 8u * S + o / q;
-    g = (o % q) * 8u + x;
-  } else {  // left-over group tiles e = x, x + 8, ... of these plane blocks
-    const uint32_t e = x + 8u * (o - nk * q);
-    if (e >= nk * r) return t;
-    kb = 8u * S + e / r;
-    g = 8u * q + e % r;
-  }
-  const uint32_t col = g * B + w;
-  if (col >= cols) return t;
-  t.xt = col % XT;
-  t.rb = col / XT;
-  t.kb = kb;
-  return t;
-}
 
 // Cell rows per plane tile (R + 1 point rows, 2 per wave) and cell planes per
 // tile (KP + 1 point planes), chosen so that all loads of a wave fit in

@@ -62,3 +62,14 @@ def test_batch_scratch_query_without_gpu():
     one = (mdmc.Volume * 1)()
     one[0].dims[0], one[0].dims[1], one[0].dims[2] = 1, 1, 1
     assert L.mdmc_batch_scratch_bytes(one, 1, ctypes.byref(b)) == 0 and b.value == 0
+
+
+def test_classify_tile_map_covers_every_tile_once(tmp_path):
+    """The plane-tile classify's workgroup -> tile map (mdmc_tilemap.h), over
+    3000 random grid shapes: every tile exactly once, XCD shares within one
+    tile of each other (tests/golden/tile_map_check.cpp)."""
+    exe = tmp_path / "tile_map_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "framework_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "golden", "tile_map_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
+    assert out.strip() == "OK", out
