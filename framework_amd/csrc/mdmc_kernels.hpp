@@ -397,17 +397,32 @@ struct PtShape {
   static constexpr uint32_t kLdsWords = KP * kPlaneWords;
 };
 
-// Table indexed by key: c_cfg_count of the key's configuration | cfg << 16,
-// followed by the staging of a tile's row counts (kPtRowsMax uint2).
+// Two tables indexed by key, then the staging of a tile's row counts
+// (kPtRowsMax uint2):
+//   s_tab[key]        c_cfg_count of the key's configuration | cfg << 16
+//   s_tab[256 + key]  an interior cell's contribution, variant 0, all three
+//                     owned edges' rings in the grid: 1 | nv << 9 | nq << 20 (the
+//                     packed counts the wave scan adds up) | (nv | nq << 3) << 24
+//                     (its record byte); 0 for the inactive keys 0 and 255
 constexpr uint32_t kPtRowsMax = 64;
-constexpr uint32_t kPtTabWords = 256 + 2 * kPtRowsMax;
-// This thread's entry (a global load) ...
+constexpr uint32_t kPtTabWords = 512 + 2 * kPtRowsMax;
+// This thread's entry of the first table (a global load) ...
 __device__ __forceinline__ uint32_t pt_table_entry() {
   const uint32_t cfg = key_to_cfg(threadIdx.x);
   return (uint32_t)c_cfg_count[cfg] | (cfg << 16);
 }
-// ... and the whole table in LDS (the caller synchronises).
-__device__ __forceinline__ void pt_init_table(uint32_t *s_tab) { s_tab[threadIdx.x] = pt_table_entry(); }
+// ... its entry of the second, from it ...
+__device__ __forceinline__ uint32_t pt_table2_entry(uint32_t tab) {
+  const uint32_t cfg = tab >> 16, nv = tab & 0xFu, nq = (uint32_t)__popc((tab >> 8) & 7u);
+  if (cfg == 0u || cfg == 0xFFu) return 0u;
+  return 1u | (nv << 9) | (nq << 20) | ((nv | (nq << 3)) << 24);
+}
+// ... and both tables in LDS (the caller synchronises).
+__device__ __forceinline__ void pt_store_tables(uint32_t *s_tab, uint32_t tab) {
+  s_tab[threadIdx.x] = tab;
+  s_tab[256 + threadIdx.x] = pt_table2_entry(tab);
+}
+__device__ __forceinline__ void pt_init_table(uint32_t *s_tab) { pt_store_tables(s_tab, pt_table_entry()); }
 
 // Zeroes the state the scan expects to start from (first workgroup only).
 __device__ __forceinline__ void zero_scan_state(const ClassifyArgs &a, uint32_t tid, uint32_t nthreads) {
@@ -426,7 +441,7 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
                                                  uint32_t tab_entry = 0u, bool store_tab = false) {
   using S = PtShape<W4, R, KP>;
   static_assert(KP * R <= kPtRowsMax, "row counts are staged in kPtRowsMax LDS entries");
-  uint2 *s_cnt = reinterpret_cast<uint2 *>(s_tab + 256);
+  uint2 *s_cnt = reinterpret_cast<uint2 *>(s_tab + 512);
   constexpr int kRowsPerWave = (R + 1) / 4;
   constexpr uint32_t CAP = 256u * W4;
   constexpr uint32_t kRowWords = S::kRowWords;
@@ -517,7 +532,7 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
   }
   // the table entry was loaded before the field rows; storing it here keeps
   // its wait behind theirs
-  if (store_tab) s_tab[threadIdx.x] = tab_entry;
+  if (store_tab) pt_store_tables(s_tab, tab_entry);
   __syncthreads();
   if (!tile_ok) return;  // uniform over the workgroup
 
@@ -542,17 +557,19 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
       // all 20 corner bits equal -> none of the lane's 4 cells is active
       if (__ballot(w != 0u && w != 0x1F1F1F1Fu) == 0ull) continue;
       const uint32_t xb = x0 + 256u * c + 4u * lane;
+      // per cell: key, and active (bit 0 of the second table entry)
       uint32_t keys = 0, lact = 0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const uint32_t tq = (w >> q) & 0x03030303u;
         const uint32_t key = ((tq * 0x41041u) >> 18) & 0xFFu;
         keys |= key << (8 * q);
-        lact |= (uint32_t)(key != 0u && key != 0xFFu) << q;
+        lact |= (s_tab[256u + key] & 1u) << q;
       }
       lact &= xvalid[c];
       if (__ballot(lact != 0u) == 0ull) continue;
-      // per cell: table entry (0 for inactive cells, so they count nothing)
+      // first table entry of each active cell (0 for inactive cells, so they
+      // count nothing)
       uint32_t e4[4];
       uint32_t amb = 0;  // bit q: cell q is a 2B/3B configuration
 #pragma unroll
@@ -561,12 +578,13 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
         e4[q] = ((lact >> q) & 1u) ? ent : 0u;
         amb |= ((e4[q] >> 11) & 1u) << q;
       }
+      const bool any_amb = __ballot(amb != 0u) != 0ull;
       // 2B/3B cells (rare): the variant depends on the neighbour across the
       // listed face.  Inside the tile the neighbour's corner bits are in LDS
       // (the word one cell or lane over, or of the next row or plane); only a
       // neighbour outside the tile is read from the field.
       uint32_t l2m = 0;
-      if (__ballot(amb != 0u) != 0ull) {
+      if (any_amb) {
         const uint32_t widx = cp * kPlaneWords + (r + 1u) * kRowWords + c * kWave + lane;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -604,20 +622,32 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
           l2m |= ((ent >> 11) & 1u) << q;
         }
       }
-      // per cell: variant, vertex count, quad count
+      // per cell: variant, vertex count, quad count; per-cell byte: nv (3
+      // bits) | nq (2 bits) << 3 | variant << 5
       uint32_t pk = 0, info4 = 0, cfgs = 0;
+      if (yin && zin && x0 + 256u * (c + 1u) < g.cx && !any_amb) {
+        // interior chunk (every cell and its owned edges' rings in the grid)
+        // without 2B/3B cells: the second table has it all
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const uint32_t e = e4[q] & 0xFFFFu, l2 = (l2m >> q) & 1u;
-        cfgs |= (e4[q] >> 16) << (8 * q);
-        const uint32_t nv = (e >> (4 * l2)) & 0xFu;
-        const bool xin = xb + q + 1 < g.cx;
-        const uint32_t qmask = (uint32_t)(xin && yin) | ((uint32_t)(xin && zin) << 1) |
-                               ((uint32_t)(yin && zin) << 2);
-        const uint32_t nq = (uint32_t)__popc((e >> 8) & 7u & qmask);
-        // per-cell byte: nv (3 bits) | nq (2 bits) << 3 | variant << 5
-        info4 |= (nv | (nq << 3) | (l2 << 5)) << (8 * q);
-        pk += ((lact >> q) & 1u) | (nv << 9) | (nq << 20);
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t e2 = s_tab[256u + ((keys >> (8 * q)) & 0xFFu)];
+          cfgs |= (e4[q] >> 16) << (8 * q);
+          pk += e2 & 0x00FFFFFFu;
+          info4 |= (e2 >> 24) << (8 * q);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t e = e4[q] & 0xFFFFu, l2 = (l2m >> q) & 1u;
+          cfgs |= (e4[q] >> 16) << (8 * q);
+          const uint32_t nv = (e >> (4 * l2)) & 0xFu;
+          const bool xin = xb + q + 1 < g.cx;
+          const uint32_t qmask = (uint32_t)(xin && yin) | ((uint32_t)(xin && zin) << 1) |
+                                 ((uint32_t)(yin && zin) << 2);
+          const uint32_t nq = (uint32_t)__popc((e >> 8) & 7u & qmask);
+          info4 |= (nv | (nq << 3) | (l2 << 5)) << (8 * q);
+          pk += ((lact >> q) & 1u) | (nv << 9) | (nq << 20);
+        }
       }
       const uint32_t incl = wave_incl_scan_dpp(pk);
       const uint32_t excl = incl - pk;
@@ -672,7 +702,7 @@ __device__ __forceinline__ void classify_pt_tile(const ClassifyArgs &a, uint32_t
 
 // Single volume: block b -> tile through the XCD mapping (pt_tile_of).
 template <int W4, int R, int KP, bool VEC>
-__global__ __launch_bounds__(kPtThreads) void k_classify_pt(PtArgs pa) {
+__global__ __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(W4 >= 4 ? 4 : 8, W4 >= 4 ? 4 : 8))) void k_classify_pt(PtArgs pa) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
   __shared__ __align__(16) uint32_t s_tab[kPtTabWords];
   const ClassifyArgs &a = pa.c;
