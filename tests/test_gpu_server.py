@@ -138,3 +138,45 @@ def test_device_field_requests(fa_mod, torch_cuda):
         assert np.array_equal(q.cpu().numpy(), qm)
         assert np.array_equal(v.cpu().numpy().view(np.uint32), vm.view(np.uint32))
     ctx.close()
+
+
+def test_clients_on_several_threads(fa_mod, torch_cuda):
+    """Four host threads submit while the caller's thread waits on poll: the
+    server's worker drives the lanes, every request comes back once, and the
+    results equal their single extractions; a wait with nothing in flight
+    returns at once."""
+    import threading
+
+    from framework_amd.server import Server
+
+    reqs = _requests(32, seed=13, dmin=16, dmax=64)
+    srv = Server(1 << 30, max_batch=4, num_streams=3)
+    ids = [None] * len(reqs)
+
+    def client(k0):
+        for k in range(k0, len(reqs), 4):
+            ids[k] = srv.submit(reqs[k])
+
+    threads = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+    for t in threads:
+        t.start()
+    got, meshes = {}, {}
+    while len(got) < len(reqs):
+        for r in srv.poll(wait=True):
+            assert r["id"] not in got
+            got[r["id"]] = r
+            if r["status"] == 0 and r["tag"] % 5 == 0:
+                meshes[r["tag"]] = srv.copy_result(r)
+            srv.release(r["id"])
+    for t in threads:
+        t.join()
+    assert sorted(got) == sorted(ids)
+    assert all(r["status"] == 0 for r in got.values())
+    assert srv.poll(wait=True) == []
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["queued"] == 0 and st["in_flight"] == 0
+    for k, (vm, qm) in meshes.items():
+        vs, qs = _single(fa_mod, reqs[k])
+        assert np.array_equal(qm, qs), k
+        assert np.array_equal(vm.view(np.uint32), vs.view(np.uint32)), k
