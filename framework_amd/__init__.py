@@ -6,8 +6,8 @@ package holds its sources (``csrc/``), the build step (``build.py``) and thin
 Python bindings (``mdmc.py``).
 """
 from .mdmc import (Context, MDMCError, cleanup, declared_functions, lib, run_dmc, run_host, setup,
-                   triangles_from_quads, write_obj, run_out_of_core, out_of_core_slab_planes)
+                   triangles_from_quads, write_obj, run_out_of_core, out_of_core_slab_planes, run_device)
 
 __all__ = ["Context", "MDMCError", "cleanup", "declared_functions", "lib", "run_dmc", "run_host", "setup",
            "triangles_from_quads", "write_obj", "run_out_of_core",
-           "out_of_core_slab_planes"]
+           "out_of_core_slab_planes", "run_device"]

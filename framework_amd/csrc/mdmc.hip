@@ -1153,6 +1153,83 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
   return MDMC_OK;
 }
 
+int mdmc_run_device(const float *d_field, const uint32_t dims[3], const float xyz_min[3],
+                    const float xyz_max[3], float iso, uint32_t num_smooth, void *stream,
+                    mdmc_device_mesh *out) {
+  if (!out || !dims || !xyz_min || !xyz_max) return fail(MDMC_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> entry(g_host_entry_mutex);
+  std::memset(out, 0, sizeof(*out));
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  if (!is_setup_on(dev)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  if (!g_default_ctx) {
+    rc = mdmc_ctx_create(&g_default_ctx);
+    if (rc != MDMC_OK) return rc;
+  }
+  mdmc_ctx *ctx = g_default_ctx;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!ctx->run_counts) {
+    if (hipMalloc(&ctx->run_counts, sizeof(mdmc_counts)) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->run_counts = nullptr;
+      return fail(MDMC_ERR_NOMEM, "device buffer allocation failed");
+    }
+    ctx->run_counts_bytes = sizeof(mdmc_counts);
+  }
+  mdmc_counts *d_counts = static_cast<mdmc_counts *>(ctx->run_counts);
+  rc = mdmc_count_device(ctx, d_field, dims, iso, d_counts, stream);
+  if (rc != MDMC_OK) return rc;
+  if (ctx->plan.empty) return MDMC_OK;
+  mdmc_counts h{};
+  HIP_TRY(hipMemcpyAsync(&h, d_counts, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (h.status != 0) return fail(MDMC_ERR_INTERNAL, "device status after count");
+  if (h.num_vertices == 0 && h.num_quads == 0) {
+    out->num_active = h.num_active;
+    return MDMC_OK;
+  }
+  const size_t vbytes = (size_t)h.num_vertices * 3 * sizeof(float), qbytes = (size_t)h.num_quads * 4 * sizeof(int32_t);
+  if (vbytes && hipMalloc(reinterpret_cast<void **>(&out->vertices), vbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    out->vertices = nullptr;
+    return fail(MDMC_ERR_NOMEM, "device mesh allocation failed");
+  }
+  if (qbytes && hipMalloc(reinterpret_cast<void **>(&out->quads), qbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    out->quads = nullptr;
+    mdmc_device_mesh_free(out);
+    return fail(MDMC_ERR_NOMEM, "device mesh allocation failed");
+  }
+  auto bail = [&](int code) {
+    mdmc_device_mesh_free(out);
+    return code;
+  };
+  rc = mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, out->vertices, h.num_vertices, out->quads, h.num_quads,
+                        d_counts, stream);
+  if (rc != MDMC_OK) return bail(rc);
+  rc = mdmc_smooth_device(ctx, out->vertices, h.num_vertices, out->quads, h.num_quads, num_smooth, d_counts, stream);
+  if (rc != MDMC_OK) return bail(rc);
+  mdmc_counts h2{};
+  if (hipMemcpyAsync(&h2, d_counts, sizeof(h2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    return bail(fail(MDMC_ERR_HIP, "stream error after emit"));
+  }
+  if (h2.status != 0) return bail(fail(MDMC_ERR_INTERNAL, "device status after emit"));
+  out->num_active = h.num_active;
+  out->num_vertices = h.num_vertices;
+  out->num_quads = h.num_quads;
+  return MDMC_OK;
+}
+
+void mdmc_device_mesh_free(mdmc_device_mesh *mesh) {
+  if (!mesh) return;
+  if (mesh->vertices) (void)hipFree(mesh->vertices);
+  if (mesh->quads) (void)hipFree(mesh->quads);
+  std::memset(mesh, 0, sizeof(*mesh));
+}
+
 // Point planes [c0-1, c0+S+2] (clipped) of a slab of S cell planes, and the
 // largest S whose field slab and scratch fit the budget.
 static int ooc_slab_planes(const uint32_t dims[3], uint64_t budget, const int *opt, uint32_t *S_out) {

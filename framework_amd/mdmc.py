@@ -53,6 +53,12 @@ class Mesh(ctypes.Structure):
                 ("num_active", ctypes.c_uint32)]
 
 
+class DeviceMesh(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("quads", ctypes.c_void_p),
+                ("num_vertices", ctypes.c_uint32), ("num_quads", ctypes.c_uint32),
+                ("num_active", ctypes.c_uint32)]
+
+
 _U3 = ctypes.c_uint32 * 3
 _F3 = ctypes.c_float * 3
 _VP = ctypes.c_void_p
@@ -108,6 +114,11 @@ def lib():
     L.mdmc_run.argtypes = [_VP, _U3, _F3, _F3, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(Mesh)]
     L.mdmc_run.restype = ctypes.c_int
     L.mdmc_mesh_free.argtypes = [ctypes.POINTER(Mesh)]
+    L.mdmc_run_device.argtypes = [_VP, _U3, _F3, _F3, ctypes.c_float, ctypes.c_uint32, _VP,
+                                  ctypes.POINTER(DeviceMesh)]
+    L.mdmc_run_device.restype = ctypes.c_int
+    L.mdmc_device_mesh_free.argtypes = [ctypes.POINTER(DeviceMesh)]
+    L.mdmc_device_mesh_free.restype = None
     L.mdmc_ctx_create.argtypes = [ctypes.POINTER(_VP)]
     L.mdmc_ctx_create.restype = ctypes.c_int
     L.mdmc_ctx_destroy.argtypes = [_VP]
@@ -201,6 +212,44 @@ def run_host(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, num_smoo
     _check(L.mdmc_run(field.ctypes.data_as(_VP), _U3(X, Y, Z), mn, mx, ctypes.c_float(iso),
                       ctypes.c_uint32(num_smooth), ctypes.byref(m)), "mdmc_run")
     return _take_mesh(L, m)
+
+
+class _DeviceArray:
+    """A library device buffer seen by torch through __cuda_array_interface__."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (int(ptr), False),
+                                         "version": 2, "strides": None}
+
+
+def run_device(field, iso: float, xyz_min=None, xyz_max=None, num_smooth: int = 0, stream=None):
+    """mdmc_run_device: a (Z, Y, X) float32 CUDA tensor in, the library's
+    device mesh out, returned as (vertices (V,3), quads (Q,4), counts) torch
+    tensors on the device (copies; the library mesh is freed here)."""
+    import torch
+
+    if not field.is_cuda or field.dtype != torch.float32 or not field.is_contiguous():
+        raise ValueError("field must be a contiguous float32 CUDA tensor")
+    Z, Y, X = field.shape
+    dmin, dmax = _default_box((X, Y, Z))
+    mn = _F3(*(xyz_min if xyz_min is not None else dmin))
+    mx = _F3(*(xyz_max if xyz_max is not None else dmax))
+    st = stream if stream is not None else torch.cuda.current_stream()
+    m = DeviceMesh()
+    L = lib()
+    _check(L.mdmc_run_device(_VP(field.data_ptr()), _U3(X, Y, Z), mn, mx, ctypes.c_float(iso),
+                             ctypes.c_uint32(num_smooth), _VP(st.cuda_stream), ctypes.byref(m)), "mdmc_run_device")
+    V, Q, A = m.num_vertices, m.num_quads, m.num_active
+    try:
+        dev = field.device
+        v = (torch.as_tensor(_DeviceArray(m.vertices, (V, 3), "<f4"), device=dev).clone() if V
+             else torch.zeros((0, 3), dtype=torch.float32, device=dev))
+        q = (torch.as_tensor(_DeviceArray(m.quads, (Q, 4), "<i4"), device=dev).clone() if Q
+             else torch.zeros((0, 4), dtype=torch.int32, device=dev))
+        torch.cuda.synchronize(dev)
+    finally:
+        L.mdmc_device_mesh_free(ctypes.byref(m))
+    return v, q, {"num_active": A, "num_vertices": V, "num_quads": Q}
 
 
 def run_out_of_core(field: np.ndarray, iso: float, device_budget: int, xyz_min=None, xyz_max=None):

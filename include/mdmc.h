@@ -99,6 +99,27 @@ int mdmc_out_of_core_slab_planes(const uint32_t dims[3], uint64_t device_budget,
 
 void mdmc_mesh_free(mdmc_mesh *mesh);
 
+/* A mesh in device memory, allocated by the library (mdmc_run_device). */
+typedef struct mdmc_device_mesh {
+  float *vertices;   /* device, 3 * num_vertices */
+  int32_t *quads;    /* device, 4 * num_quads */
+  uint32_t num_vertices;
+  uint32_t num_quads;
+  uint32_t num_active;
+} mdmc_device_mesh;
+
+/* The device-resident form of run_dmc (no counterpart in the reference, which
+ * takes a host grid, dmc.cu:2077-2079): `d_field` is device memory on the
+ * current device; count, exact output allocation (hipMalloc), emit and
+ * num_smooth smoothing iterations run on `stream` (a hipStream_t, NULL for
+ * the default stream), and the call returns once the mesh is complete.
+ * Shares the library's default context with mdmc_run (calls serialise).
+ * An empty mesh has null pointers. */
+int mdmc_run_device(const float *d_field, const uint32_t dims[3], const float xyz_min[3],
+                    const float xyz_max[3], float iso, uint32_t num_smooth, void *stream,
+                    mdmc_device_mesh *out);
+void mdmc_device_mesh_free(mdmc_device_mesh *mesh);
+
 /* Writes a mesh as Wavefront OBJ, like the reference's dump_obj
  * (dmc.cu:2238-2246): "v x y z" per vertex (six significant digits, as the
  * reference's std::ostream default) and two 1-based triangle faces

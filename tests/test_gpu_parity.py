@@ -222,6 +222,26 @@ def test_host_entry_matches_device(ctx, torch_cuda):
     assert tris.shape == (2 * ref["num_quads"], 3)
 
 
+def test_device_entry_with_library_mesh(ctx, torch_cuda):
+    """mdmc_run_device (device field in, library-allocated device mesh out,
+    on a caller stream) == the context path == the oracle; smoothing and an
+    empty surface too."""
+    import framework_amd as fa
+
+    f = oracle.field_gyroid((70, 53, 61), period=11.0, seed=4, noise_amp=0.2)
+    fd = _dev(torch_cuda, f)
+    st = torch_cuda.cuda.Stream()
+    v, q, c = fa.run_device(fd, 0.0, stream=st)
+    ref = oracle.run(f, 0.0)
+    assert c["num_vertices"] == ref["num_vertices"] and c["num_active"] == ref["num_active"]
+    assert_same_mesh(v.cpu().numpy(), q.cpu().numpy(), ref)
+    vs, qs, _ = fa.run_device(fd, 0.0, num_smooth=2)
+    vc, qc, _ = ctx.extract(fd, 0.0, num_smooth=2)
+    assert torch_cuda.equal(qs, qc) and torch_cuda.equal(vs.view(torch_cuda.int32), vc.view(torch_cuda.int32))
+    ve, qe, ce = fa.run_device(torch_cuda.zeros((9, 8, 7), device="cuda"), 0.5)
+    assert ce["num_vertices"] == 0 and ve.shape == (0, 3) and qe.shape == (0, 4)
+
+
 def test_deterministic(ctx, torch_cuda):
     f = ctx.field_gyroid((90, 90, 90), period=15.0, seed=3, noise_amp=0.1)
     torch_cuda.cuda.synchronize()
