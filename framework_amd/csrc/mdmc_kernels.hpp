@@ -1660,10 +1660,11 @@ struct BatchEmitArgs {
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_emit_batch(BatchEmitArgs b) {
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
-  __shared__ uint32_t s_soff[kMaxBatch + 1];
+  // first segment of each volume (the search below reads entries 1 .. nvol-1;
+  // kMaxBatch entries keep the workgroup's LDS at 20 KB, 8 workgroups per CU)
+  __shared__ uint32_t s_soff[kMaxBatch];
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
-  for (uint32_t i = threadIdx.x; i <= b.nvol; i += kEmitThreads)
-    s_soff[i] = i < b.nvol ? b.vols[i].seg_off : b.a.nsegs;
+  for (uint32_t i = threadIdx.x; i < b.nvol; i += kEmitThreads) s_soff[i] = b.vols[i].seg_off;
   __syncthreads();
   const EmitArgs &a = b.a;
   const uint32_t lane = lane_id(), wave = wave_id();
@@ -1680,23 +1681,40 @@ __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8,
       const uint32_t mid = (lo + hi) >> 1;
       if (s_soff[mid] <= s) lo = mid; else hi = mid;
     }
-    const BatchVol &bv = b.vols[lo];
-    EmitVol e;
-    e.g = bv.c.g;
-    e.xt_c = bv.coords;
-    e.yt_c = bv.coords + bv.c.g.X;
-    e.zt_c = bv.coords + bv.c.g.X + bv.c.g.Y;
-    e.XT = bv.c.XT;
-    e.seg_off = bv.seg_off;
-    const uint2 bs = b.base[lo];
-    e.vbase = bs.x;
-    e.qbase = bs.y;
-    e.verts = bv.verts;
-    e.vcap = bv.verts ? bv.vcap : 0;
-    e.quads = bv.quads;
-    e.qcap = bv.quads ? bv.qcap : 0;
-    e.status = bv.counts ? bv.counts + 3 : a.v.status;
-    emit_cell(a, e, s, rank, s_lab, s_acc + threadIdx.x);
+    // the lanes of one volume at a time (64 consecutive active cells rarely
+    // span two volumes), so that the volume's descriptor is wave-uniform and
+    // comes in through scalar loads instead of occupying vector registers
+    for (;;) {
+      const uint32_t v = __builtin_amdgcn_readfirstlane(lo);
+      if (lo != v) continue;
+      const __attribute__((address_space(4))) BatchVol *bv = (const __attribute__((address_space(4))) BatchVol *)b.vols + v;
+      const __attribute__((address_space(4))) uint2 *bb = (const __attribute__((address_space(4))) uint2 *)b.base + v;
+      EmitVol e;
+      e.g.f = bv->c.g.f;
+      e.g.X = bv->c.g.X;
+      e.g.Y = bv->c.g.Y;
+      e.g.Z = bv->c.g.Z;
+      e.g.cx = bv->c.g.cx;
+      e.g.cy = bv->c.g.cy;
+      e.g.cz = bv->c.g.cz;
+      e.g.iso = bv->c.g.iso;
+      float *const co = bv->coords;
+      e.xt_c = co;
+      e.yt_c = co + e.g.X;
+      e.zt_c = co + e.g.X + e.g.Y;
+      e.XT = bv->c.XT;
+      e.seg_off = bv->seg_off;
+      e.vbase = bb->x;
+      e.qbase = bb->y;
+      e.verts = bv->verts;
+      e.vcap = e.verts ? bv->vcap : 0;
+      e.quads = bv->quads;
+      e.qcap = e.quads ? bv->qcap : 0;
+      uint32_t *const cnt = bv->counts;
+      e.status = cnt ? cnt + 3 : a.v.status;
+      emit_cell(a, e, s, rank, s_lab, s_acc + threadIdx.x);
+      break;
+    }
   }
 }
 
