@@ -1302,7 +1302,6 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
   };
   void *d_fbuf[2] = {nullptr, nullptr}, *d_counts_v = nullptr, *d_v = nullptr, *d_q = nullptr;
   size_t fv_bytes[2] = {0, 0}, v_bytes = 0, q_bytes = 0;
-  std::vector<uint2> segs;
   float *hv = nullptr;
   int32_t *hq = nullptr;
   uint64_t hv_cap = 0, hq_cap = 0, G = 0, GQ = 0, A = 0;
@@ -1340,11 +1339,13 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     *cap = n;
     return true;
   };
-  if (hipMalloc(&d_counts_v, sizeof(mdmc_counts)) != hipSuccess) {
+  // counts (4 words) then the slab record of k_slab_info (9 words)
+  if (hipMalloc(&d_counts_v, 16 * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
     return bail(MDMC_ERR_NOMEM, "counts allocation failed");
   }
   mdmc_counts *d_counts = static_cast<mdmc_counts *>(d_counts_v);
+  uint32_t *d_info = static_cast<uint32_t *>(d_counts_v) + 4;
   // slab i computes from field buffer i % 2 while a host thread uploads slab
   // i + 1 into the other buffer
   const uint32_t nslabs = (cz + S - 1) / S;
@@ -1385,21 +1386,20 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     ctx->counted = true;
     ctx->counted_field = df;
     ctx->iso = iso;
-    mdmc_counts hc{};
-    if (hipMemcpy(&hc, d_counts, sizeof(hc), hipMemcpyDeviceToHost) != hipSuccess || hc.status)
-      return bail(MDMC_ERR_INTERNAL, "slab count status");
-    // vertex / quad bases of the first segment of planes h and h + (c1 - c0)
+    // the count status, the vertex / quad bases of the first segment of
+    // planes h and h + (c1 - c0), and the slab's active cells: one copy
     const uint64_t sp = (uint64_t)p.XT * p.cy;
     const uint64_t s_lo = h * sp, s_hi = (uint64_t)(h + (c1 - c0)) * sp;
-    uint2 b_lo{}, b_hi{};
-    if (hipMemcpy(&b_lo, ctx->s.seg_vq + s_lo, sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&b_hi, ctx->s.seg_vq + s_hi, sizeof(uint2), hipMemcpyDeviceToHost) != hipSuccess)
-      return bail(MDMC_ERR_HIP, "slab bases");
-    segs.resize((size_t)(s_hi - s_lo));
-    if (!segs.empty() && hipMemcpy(segs.data(), ctx->s.seg_counts + s_lo, segs.size() * sizeof(uint2),
-                                   hipMemcpyDeviceToHost) != hipSuccess)
-      return bail(MDMC_ERR_HIP, "slab segment counts");
-    for (const uint2 &c : segs) A += c.x & 0xFFFFu;
+    hipLaunchKernelGGL(k_slab_info, dim3(1), dim3(64), 0, nullptr, ctx->s.seg_vq, ctx->s.ne, ctx->s.totals,
+                       (uint32_t)p.nsegs, reinterpret_cast<const uint32_t *>(d_counts), (uint32_t)s_lo,
+                       (uint32_t)s_hi, d_info);
+    uint32_t info[9] = {};
+    if (hipMemcpy(info, d_info, sizeof(info), hipMemcpyDeviceToHost) != hipSuccess)
+      return bail(MDMC_ERR_HIP, "slab record");
+    if (info[3]) return bail(MDMC_ERR_INTERNAL, "slab count status");
+    const uint2 b_lo = make_uint2(info[4], info[5]), b_hi = make_uint2(info[6], info[7]);
+    A += info[8];
+    mdmc_counts hc{};
     const uint32_t Vs = b_hi.x - b_lo.x, Qs = b_hi.y - b_lo.y;
     if (!grow_dev(&d_v, &v_bytes, std::max<size_t>(256, 12ull * Vs)) ||
         !grow_dev(&d_q, &q_bytes, std::max<size_t>(256, 16ull * Qs)))

@@ -1620,6 +1620,23 @@ __device__ __forceinline__ uint32_t actives_before(const uint2 *ne, uint32_t num
   return ne[lo].y;
 }
 
+// One slab of an out-of-core extraction, after its count: the count words,
+// the vertex / quad bases of segments s_lo and s_hi (the slab's own cell
+// planes) and its active cells, in one small record for one host copy:
+// out[0..3] counts, out[4..7] base(s_lo).xy base(s_hi).xy, out[8] actives.
+__global__ void k_slab_info(const uint2 *seg_vq, const uint2 *ne, const Totals *totals, uint32_t nsegs,
+                            const uint32_t *counts, uint32_t s_lo, uint32_t s_hi, uint32_t *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t num_ne = min(totals->num_nonempty, nsegs);
+  for (int i = 0; i < 4; ++i) out[i] = counts[i];
+  const uint2 lo = seg_vq[s_lo], hi = seg_vq[s_hi];
+  out[4] = lo.x;
+  out[5] = lo.y;
+  out[6] = hi.x;
+  out[7] = hi.y;
+  out[8] = actives_before(ne, num_ne, s_hi) - actives_before(ne, num_ne, s_lo);
+}
+
 // One thread per volume: its totals and the bases of its numbering.
 __global__ __launch_bounds__(256) void k_batch_counts(BatchCountArgs b) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
