@@ -23,7 +23,7 @@
 namespace {
 
 constexpr int kIoThreads = 4;
-constexpr size_t kIoChunk = 8u << 20;  // bytes per pinned chunk
+constexpr size_t kIoChunk = 4u << 20;  // bytes per pinned chunk
 
 struct IoLane {
   hipStream_t stream = nullptr;
@@ -37,8 +37,10 @@ struct IoStaging {
   bool ready = false;
 };
 
-std::mutex g_io_mutex;
-IoStaging g_io;
+// one staging set per direction: an upload and a download (the next slab in
+// and the last slab's mesh out, say) run at the same time over full-duplex PCIe
+std::mutex g_io_mutex[2];
+IoStaging g_io[2];
 
 void io_release(IoStaging &io) {
   for (IoLane &l : io.lanes) {
@@ -127,8 +129,10 @@ int staged_copy(void *host, void *dev, size_t bytes, int dir) {
     (void)hipGetLastError();
     return MDMC_ERR_HIP;
   }
-  std::lock_guard<std::mutex> lk(g_io_mutex);
-  int rc = io_init(g_io, device);
+  if (dir != 0 && dir != 1) return MDMC_ERR_INVALID;
+  IoStaging &io = g_io[dir];
+  std::lock_guard<std::mutex> lk(g_io_mutex[dir]);
+  int rc = io_init(io, device);
   if (rc != MDMC_OK) return rc;
   const size_t nchunks = (bytes + kIoChunk - 1) / kIoChunk;
   const int nthreads = (int)(nchunks < (size_t)kIoThreads ? nchunks : (size_t)kIoThreads);
@@ -137,10 +141,10 @@ int staged_copy(void *host, void *dev, size_t bytes, int dir) {
   for (int t = 1; t < nthreads; ++t)
     pool.emplace_back([&, t] {
       (void)hipSetDevice(device);
-      errs[t] = io_lane_copy(g_io.lanes[t], t, nthreads, static_cast<char *>(host), static_cast<char *>(dev),
+      errs[t] = io_lane_copy(io.lanes[t], t, nthreads, static_cast<char *>(host), static_cast<char *>(dev),
                              bytes, dir);
     });
-  errs[0] = io_lane_copy(g_io.lanes[0], 0, nthreads, static_cast<char *>(host), static_cast<char *>(dev), bytes,
+  errs[0] = io_lane_copy(io.lanes[0], 0, nthreads, static_cast<char *>(host), static_cast<char *>(dev), bytes,
                          dir);
   for (std::thread &th : pool) th.join();
   for (int t = 0; t < nthreads; ++t)
@@ -152,8 +156,10 @@ int staged_copy(void *host, void *dev, size_t bytes, int dir) {
 }
 
 void staged_release() {
-  std::lock_guard<std::mutex> lk(g_io_mutex);
-  if (g_io.ready) io_release(g_io);
+  for (int d = 0; d < 2; ++d) {
+    std::lock_guard<std::mutex> lk(g_io_mutex[d]);
+    if (g_io[d].ready) io_release(g_io[d]);
+  }
 }
 
 }  // namespace mdmc
