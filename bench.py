@@ -236,8 +236,8 @@ def main() -> int:
         oracle.set_threads(1)
         cpu = {"value": N / tmul / 1e9, "unit": "Gvoxels/s", "cores": nthr, "kind": "port",
                "sample": f"full 512^3 blob field, {nmul} run(s), median {tmul:.3f} s (C oracle restating "
-                         f"dmc::run_dmc, OpenMP over cell planes and active cells, {nthr} threads, host of "
-                         f"the GPU box)",
+                         f"dmc::run_dmc, OpenMP over cell planes and active cells, {nthr} threads, on the host "
+                         f"of the GPU that ran the steps)",
                "single_thread": {"value": N / tone / 1e9, "cores": 1,
                                  "sample": f"same field, {none} run(s), median {tone:.3f} s"}}
 
