@@ -101,6 +101,19 @@ def _descend(torch, ctx, space: dict, run, check, reps: int) -> tuple[dict, list
             log.append(({OPTION_NAMES[o]: v for o, v in cand.items()}, ms))
             if ms < best_ms:
                 best, best_ms = cand, ms
+    # confirm against the defaults, alternating the two so that drifting
+    # clocks hit both alike: a choice must win by more than 1% to stand
+    default = {opt: 0 for opt in space}
+    if best != default:
+        t_best, t_def = [], []
+        for _ in range(3):
+            apply(ctx, default)
+            t_def.append(_time(torch, run, reps))
+            apply(ctx, best)
+            t_best.append(_time(torch, run, reps))
+        if float(np.median(t_best)) > 0.99 * float(np.median(t_def)):
+            best = default
+        log.append(({"confirm": "default" if best == default else "tuned"}, float(np.median(t_best))))
     apply(ctx, best)
     return best, log
 
