@@ -494,6 +494,9 @@ def bench_served(fa, torch, nreq: int, budget: int = 64 << 30, options: dict | N
     srv.close()
     ok = res["status"] == 0
     lat = np.sort(res["latency"][ok])
+    # latency histogram (ms): counts per bin [edges[i], edges[i+1]), the last open
+    hist_edges = [0.0, 0.1, 0.2, 0.5, 1.0, 2.0, 5.0, 10.0, 20.0, 50.0]
+    hist_counts = np.histogram(lat * 1e3, bins=hist_edges + [float("inf")])[0].tolist()
     # plain loop on a sample of the same requests
     nplain = min(300, nreq)
     ctx = fa.Context()
@@ -504,7 +507,8 @@ def bench_served(fa, torch, nreq: int, budget: int = 64 << 30, options: dict | N
     out = {"requests": nreq, "completed_ok": int(ok.sum()), "rejected": int((res["status"] == -8).sum()),
            "offered_rate": rate, "capacity_volumes_per_s": capacity,
            "volumes_per_s": nreq / res["wall"], "p50_ms": float(np.percentile(lat, 50) * 1e3),
-           "p99_ms": float(np.percentile(lat, 99) * 1e3), "batches": st["batches"],
+           "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "latency_hist_ms": {"edges": hist_edges, "counts": hist_counts}, "batches": st["batches"],
            "memory_peak_gb": st["memory_peak"] / 1e9, "budget_gb": budget / 1e9,
            "plain_loop_volumes_per_s": plain_vps, "plain_loop_sample": nplain,
            "plain_loop_p50_ms": float(np.percentile(plain["latency"], 50) * 1e3),
