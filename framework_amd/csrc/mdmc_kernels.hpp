@@ -10,7 +10,8 @@
 //               quad counts, a DPP-scan rank and a 64-bit record in their
 //               segment's slot range (segment = one cell row of one x-tile in
 //               one plane), plus a per-64-cell group map (active mask, first
-//               rank).  k_classify is the z-marching form for any row length.
+//               rank).  Workgroups map to tiles XCD by XCD (mdmc_tilemap.h).
+//               k_classify is the z-marching form for any row length.
 //   k_scan      single-pass decoupled look-back scan (ticketed tiles) over the
 //               segment totals: vertex/quad bases per segment, the non-empty
 //               segment list and where each 64-cell chunk starts.
@@ -21,8 +22,9 @@
 //   k_smooth, k_reaverage   optional smoothing iterations over the mesh.
 //
 // Batched forms (k_classify_batch, k_batch_counts, k_emit_batch) run the
-// same per-tile and per-cell code over many volumes; k_field_* generate the
-// synthetic benchmark fields.
+// same per-tile and per-cell code over many volumes; k_slab_info reports an
+// out-of-core slab in one record; k_field_* generate the synthetic benchmark
+// fields.
 //
 // Reference semantics: DMC_CUDA/DMC_CUDA/src/dmc.cu:1018-2208 (dmc::run_dmc).
 #pragma once
