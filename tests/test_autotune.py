@@ -58,7 +58,7 @@ def test_every_single_volume_option_keeps_the_mesh():
     gen = fa.Context()
     fields = [gen.field_gyroid((300, 45, 29), period=11.0, seed=3, noise_amp=0.2),   # 512-cell x-tiles
               gen.field_blobs((131, 70, 40), seed=5, nblobs=12, sigma=(4.0, 12.0)),  # 256-cell x-tiles
-              gen.field_blobs((700, 20, 12), seed=6, nblobs=9, sigma=(4.0, 12.0))]   # 1024-cell x-tiles
+              gen.field_blobs((700, 20, 12), seed=6, nblobs=9, sigma=(4.0, 12.0))]   # two 512-cell x-tiles
     torch.cuda.synchronize()
     for f in fields:
         ref_ctx = fa.Context()
