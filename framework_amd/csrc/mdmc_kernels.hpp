@@ -751,7 +751,7 @@ struct BatchArgs {
 // memory order): with a grid of at least the tile count a workgroup per tile,
 // with a smaller grid each workgroup walks several (persistent).
 template <int W4, int R, int KP>
-__global__ __launch_bounds__(kPtThreads) void k_classify_batch(BatchArgs b) {
+__global__ __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(W4 == 1 ? 8 : (W4 == 2 ? 6 : 4), W4 == 1 ? 8 : (W4 == 2 ? 6 : 4)))) void k_classify_batch(BatchArgs b) {
   __shared__ uint32_t s_w[PtShape<W4, R, KP>::kLdsWords];
   __shared__ __align__(16) uint32_t s_tab[kPtTabWords];
   __shared__ uint32_t s_toff[kMaxBatch + 1];
