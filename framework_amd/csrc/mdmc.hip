@@ -164,6 +164,10 @@ struct mdmc_ctx {
   // device buffers of the host entry (mdmc_run)
   void *run_field = nullptr, *run_counts = nullptr, *run_v = nullptr, *run_q = nullptr;
   size_t run_field_bytes = 0, run_counts_bytes = 0, run_v_bytes = 0, run_q_bytes = 0;
+  // device buffers of the out-of-core entry: two field slabs (sized exactly
+  // for the last call's slabs), its counts and slab record, a slab's mesh
+  void *ooc_field[2] = {nullptr, nullptr}, *ooc_counts = nullptr, *ooc_v = nullptr, *ooc_q = nullptr;
+  size_t ooc_field_bytes[2] = {0, 0}, ooc_v_bytes = 0, ooc_q_bytes = 0;
   mdmc::EmitArgs emit_args{};
   float coords_box[6] = {};
   const float *counted_field = nullptr;
@@ -924,7 +928,8 @@ void mdmc_ctx_destroy(mdmc_ctx *ctx) {
   if (!ctx) return;
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->scratch && !ctx->scratch_external) (void)hipFree(ctx->scratch);
-  for (void *b : {ctx->run_field, ctx->run_counts, ctx->run_v, ctx->run_q})
+  for (void *b : {ctx->run_field, ctx->run_counts, ctx->run_v, ctx->run_q, ctx->ooc_field[0], ctx->ooc_field[1],
+                  ctx->ooc_counts, ctx->ooc_v, ctx->ooc_q})
     if (b) (void)hipFree(b);
   if (ctx->bdev) (void)hipFree(ctx->bdev);
   if (ctx->gdev) (void)hipFree(ctx->gdev);
@@ -1303,8 +1308,10 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     *lo = l;
     return hi - l + 1;
   };
-  void *d_fbuf[2] = {nullptr, nullptr}, *d_counts_v = nullptr, *d_v = nullptr, *d_q = nullptr;
-  size_t fv_bytes[2] = {0, 0}, v_bytes = 0, q_bytes = 0;
+  // the device buffers stay with the default context, so that repeated calls
+  // on grids of one size allocate nothing
+  void **const d_fbuf = ctx->ooc_field;
+  void *&d_v = ctx->ooc_v, *&d_q = ctx->ooc_q;
   float *hv = nullptr;
   int32_t *hq = nullptr;
   uint64_t hv_cap = 0, hq_cap = 0, G = 0, GQ = 0, A = 0;
@@ -1312,8 +1319,6 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
   int upload_rc = MDMC_OK;
   auto release = [&]() {
     if (uploader.joinable()) uploader.join();
-    for (void *b : {d_fbuf[0], d_fbuf[1], d_counts_v, d_v, d_q})
-      if (b) (void)hipFree(b);
   };
   auto bail = [&](int code, const char *msg) {
     release();
@@ -1321,8 +1326,10 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     std::free(hq);
     return fail(code, msg);
   };
-  auto grow_dev = [&](void **b, size_t *have, size_t want) -> bool {
-    if (want <= *have) return true;
+  // exact: reallocate unless the buffer has exactly the wanted size (a field
+  // slab never holds more than the budget allowed for this call)
+  auto grow_dev = [&](void **b, size_t *have, size_t want, bool exact) -> bool {
+    if (exact ? want == *have : want <= *have) return true;
     if (*b) (void)hipFree(*b);
     *b = nullptr;
     *have = 0;
@@ -1343,19 +1350,27 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     return true;
   };
   // counts (4 words) then the slab record of k_slab_info (9 words)
-  if (hipMalloc(&d_counts_v, 16 * sizeof(uint32_t)) != hipSuccess) {
+  if (!ctx->ooc_counts && hipMalloc(&ctx->ooc_counts, 16 * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
+    ctx->ooc_counts = nullptr;
     return bail(MDMC_ERR_NOMEM, "counts allocation failed");
   }
-  mdmc_counts *d_counts = static_cast<mdmc_counts *>(d_counts_v);
-  uint32_t *d_info = static_cast<uint32_t *>(d_counts_v) + 4;
+  mdmc_counts *d_counts = static_cast<mdmc_counts *>(ctx->ooc_counts);
+  uint32_t *d_info = static_cast<uint32_t *>(ctx->ooc_counts) + 4;
   // slab i computes from field buffer i % 2 while a host thread uploads slab
   // i + 1 into the other buffer
   const uint32_t nslabs = (cz + S - 1) / S;
   const size_t max_npl = std::min<size_t>((size_t)S + 4, Z);
-  for (int b2 = 0; b2 < 2 && b2 < (int)nslabs; ++b2)
-    if (!grow_dev(&d_fbuf[b2], &fv_bytes[b2], 4 * plane * max_npl))
-      return bail(MDMC_ERR_NOMEM, "field slab allocation failed");
+  for (int b2 = 0; b2 < 2; ++b2) {
+    if (b2 < (int)nslabs) {
+      if (!grow_dev(&d_fbuf[b2], &ctx->ooc_field_bytes[b2], 4 * plane * max_npl, true))
+        return bail(MDMC_ERR_NOMEM, "field slab allocation failed");
+    } else if (d_fbuf[b2]) {  // a single slab: no second buffer
+      (void)hipFree(d_fbuf[b2]);
+      d_fbuf[b2] = nullptr;
+      ctx->ooc_field_bytes[b2] = 0;
+    }
+  }
   auto upload = [&](uint32_t i) {
     uint32_t lo;
     const uint32_t n = slab_planes(i * S, std::min(S, cz - i * S), &lo);
@@ -1404,8 +1419,8 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     A += info[8];
     mdmc_counts hc{};
     const uint32_t Vs = b_hi.x - b_lo.x, Qs = b_hi.y - b_lo.y;
-    if (!grow_dev(&d_v, &v_bytes, std::max<size_t>(256, 12ull * Vs)) ||
-        !grow_dev(&d_q, &q_bytes, std::max<size_t>(256, 16ull * Qs)))
+    if (!grow_dev(&d_v, &ctx->ooc_v_bytes, std::max<size_t>(256, 12ull * Vs), false) ||
+        !grow_dev(&d_q, &ctx->ooc_q_bytes, std::max<size_t>(256, 16ull * Qs), false))
       return bail(MDMC_ERR_NOMEM, "slab output allocation failed");
     // emit planes [h, h + c1 - c0) with global coordinates and numbering
     const Scratch &sc = ctx->s;

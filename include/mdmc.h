@@ -90,7 +90,9 @@ int mdmc_run(const float *field, const uint32_t dims[3], const float xyz_min[3],
  * extracted) and the scratch of one fit `device_budget` bytes (the mesh
  * buffers of one slab come on top).  The mesh is identical to mdmc_run's
  * with num_smooth = 0 (smoothing needs the whole mesh and is not offered
- * here). */
+ * here).  Like mdmc_run's, these device buffers stay allocated between calls
+ * (the field slabs at exactly this call's size) until mdmc_cleanup, so
+ * repeated calls on one grid size allocate nothing. */
 int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float xyz_min[3],
                          const float xyz_max[3], float iso, uint64_t device_budget, mdmc_mesh *out);
 /* Cell planes per slab mdmc_run_out_of_core uses for these dims and budget

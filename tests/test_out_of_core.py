@@ -69,3 +69,25 @@ def test_out_of_core_equals_in_core():
     ref = oracle.run(f, iso)
     v, q = fa.run_out_of_core(f, iso, _budget_for(fa, f.shape[::-1], 3))
     assert np.array_equal(q, ref["quads"]) and np.array_equal(v.view(np.uint32), ref["vertices"].view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_out_of_core_repeated_calls_reuse_buffers():
+    """The slab buffers stay with the library between calls: the same call
+    twice, a single-slab call (no second field slab), a larger grid, then the
+    first call again, each identical to the in-core mesh."""
+    import torch
+
+    import framework_amd as fa
+
+    torch.cuda.set_device(0)
+    fa.setup()
+    small = oracle.field_blobs((64, 48, 100), 2, 12, sigma=(3.0, 12.0))
+    big = oracle.field_gyroid((90, 70, 120), 13.0, 7, 0.3)
+    ref = {id(f): fa.run_host(f, 0.1) for f in (small, big)}
+    for f, S in ((small, 4), (small, 4), (small, small.shape[0] - 1), (big, 6), (small, 4)):
+        Z, Y, X = f.shape
+        v, q = fa.run_out_of_core(f, 0.1, _budget_for(fa, (X, Y, Z), S))
+        v0, q0 = ref[id(f)]
+        assert np.array_equal(q, q0), (f.shape, S)
+        assert np.array_equal(v.view(np.uint32), v0.view(np.uint32)), (f.shape, S)
