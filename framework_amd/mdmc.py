@@ -72,6 +72,15 @@ class Volume(ctypes.Structure):
                 ("quads", _VP), ("quad_capacity", ctypes.c_uint64)]
 
 
+# the same layout as a numpy record, so that a batch table is filled with a
+# few array assignments instead of per-field ctypes writes
+_VOLUME_DTYPE = np.dtype({
+    "names": ["field", "dims", "iso", "xyz_min", "xyz_max", "vertices", "vertex_capacity", "quads", "quad_capacity"],
+    "formats": ["<u8", ("<u4", (3,)), "<f4", ("<f4", (3,)), ("<f4", (3,)), "<u8", "<u8", "<u8", "<u8"],
+    "offsets": [getattr(Volume, n).offset for n, _ in Volume._fields_],
+    "itemsize": ctypes.sizeof(Volume)})
+
+
 class GenSpec(ctypes.Structure):
     """mdmc_gen_spec: one synthetic field of a batched generation."""
     _fields_ = [("out", _VP), ("dims", ctypes.c_uint32 * 3), ("kind", ctypes.c_int), ("seed", ctypes.c_uint64),
@@ -456,47 +465,54 @@ class Context:
     def batch_counts_tensor(self, n: int):
         return self._torch.zeros((n, 4), dtype=self._torch.int32, device="cuda")
 
-    def batch_count(self, fields, isos, counts, boxes=None, stream=None) -> None:
+    def batch_count(self, fields, isos, counts, boxes=None, stream=None, _table=None) -> None:
         """Classify + scan many volumes in one launch sequence (counts: (n, 4) int32 CUDA tensor)."""
-        arr = _volume_table(fields, isos, boxes)
-        _check(lib().mdmc_batch_count_device(self._h, arr, len(fields), _VP(counts.data_ptr()),
+        arr = _table if _table is not None else _volume_table(fields, isos, boxes)
+        _check(lib().mdmc_batch_count_device(self._h, _table_ptr(arr), len(fields), _VP(counts.data_ptr()),
                                              self._stream(stream)), "mdmc_batch_count_device")
 
-    def batch_emit(self, fields, isos, outputs, counts, boxes=None, stream=None) -> None:
-        arr = _volume_table(fields, isos, boxes, outputs)
-        _check(lib().mdmc_batch_emit_device(self._h, arr, len(fields), _VP(counts.data_ptr()),
+    def batch_emit(self, fields, isos, outputs, counts, boxes=None, stream=None, _table=None) -> None:
+        arr = _table if _table is not None else _volume_table(fields, isos, boxes, outputs)
+        _check(lib().mdmc_batch_emit_device(self._h, _table_ptr(arr), len(fields), _VP(counts.data_ptr()),
                                             self._stream(stream)), "mdmc_batch_emit_device")
 
     def batch_extract_into(self, fields, isos, outputs, counts, boxes=None, stream=None) -> None:
         """Count + emit into preallocated outputs [(verts, quads), ...] without a host sync."""
         arr = _volume_table(fields, isos, boxes, outputs)
-        _check(lib().mdmc_batch_extract_device(self._h, arr, len(fields), _VP(counts.data_ptr()),
+        _check(lib().mdmc_batch_extract_device(self._h, _table_ptr(arr), len(fields), _VP(counts.data_ptr()),
                                                self._stream(stream)), "mdmc_batch_extract_device")
 
     def batch_extract(self, fields, isos, boxes=None, stream=None):
-        """Count, size every output exactly, emit.  Returns [(verts, quads, counts dict), ...]."""
+        """Count, size every output exactly, emit.  Returns [(verts, quads, counts dict), ...].
+        The vertex (quad) arrays of a batch are consecutive views of one allocation."""
         torch = self._torch
         n = len(fields)
         counts = self.batch_counts_tensor(n)
-        self.batch_count(fields, isos, counts, boxes, stream)
+        table = _volume_table(fields, isos, boxes)
+        self.batch_count(fields, isos, counts, stream=stream, _table=table)
         (stream or torch.cuda.current_stream()).synchronize()
         c = counts.cpu().numpy().view(np.uint32)
-        outs = []
-        for i in range(n):
-            V, Q = int(c[i, 1]), int(c[i, 2])
-            outs.append((torch.empty((V, 3), dtype=torch.float32, device="cuda"),
-                         torch.empty((Q, 4), dtype=torch.int32, device="cuda")))
-        self.batch_emit(fields, isos, outs, counts, boxes, stream)
+        nv = c[:, 1].astype(np.int64)
+        nq = c[:, 2].astype(np.int64)
+        all_v = torch.empty((int(nv.sum()), 3), dtype=torch.float32, device="cuda")
+        all_q = torch.empty((int(nq.sum()), 4), dtype=torch.int32, device="cuda")
+        voff = np.concatenate(([0], np.cumsum(nv)[:-1])).astype(np.uint64)
+        qoff = np.concatenate(([0], np.cumsum(nq)[:-1])).astype(np.uint64)
+        table["vertices"] = np.where(nv > 0, np.uint64(all_v.data_ptr()) + np.uint64(12) * voff, 0)
+        table["vertex_capacity"] = nv
+        table["quads"] = np.where(nq > 0, np.uint64(all_q.data_ptr()) + np.uint64(16) * qoff, 0)
+        table["quad_capacity"] = nq
+        self.batch_emit(fields, isos, None, counts, stream=stream, _table=table)
         (stream or torch.cuda.current_stream()).synchronize()
         c = counts.cpu().numpy().view(np.uint32)
-        res = []
-        for i in range(n):
-            st = int(c[i, 3])
-            if st:
-                raise MDMCError(-7, f"volume {i}: status {st}")
-            res.append((outs[i][0], outs[i][1], {"num_active": int(c[i, 0]), "num_vertices": int(c[i, 1]),
-                                                 "num_quads": int(c[i, 2]), "status": st}))
-        return res
+        bad = np.nonzero(c[:, 3])[0]
+        if len(bad):
+            raise MDMCError(-7, f"volume {int(bad[0])}: status {int(c[bad[0], 3])}")
+        vs = torch.split(all_v, nv.tolist())
+        qs = torch.split(all_q, nq.tolist())
+        cl = c[:, :3].tolist()
+        return [(vs[i], qs[i], {"num_active": cl[i][0], "num_vertices": cl[i][1], "num_quads": cl[i][2],
+                                "status": 0}) for i in range(n)]
 
     def batch_generate(self, specs, stream=None):
         """Fields for many volumes in one launch.  specs: dicts with dims (X, Y, Z), kind ("blobs" /
@@ -524,27 +540,45 @@ class Context:
 
 
 def _volume_table(fields, isos, boxes=None, outputs=None):
-    """ctypes array of Volume for torch fields shaped (Z, Y, X)."""
+    """Batch table (mdmc_volume records) for torch fields shaped (Z, Y, X)."""
     n = len(fields)
-    arr = (Volume * n)()
-    for i, f in enumerate(fields):
-        if f.dtype.is_floating_point is False or f.dim() != 3 or not f.is_contiguous():
+    if len(isos) != n or (boxes is not None and len(boxes) != n) or (outputs is not None and len(outputs) != n):
+        raise ValueError("isos, boxes and outputs need one entry per field")
+    tab = np.zeros(n, dtype=_VOLUME_DTYPE)
+    if n == 0:
+        return tab
+    import torch
+
+    ptrs, shapes = [], []
+    for f in fields:
+        if f.dtype != torch.float32 or f.dim() != 3 or not f.is_contiguous():
             raise ValueError("batch fields must be contiguous 3-D float32 CUDA tensors")
-        Z, Y, X = f.shape
-        v = arr[i]
-        v.field = f.data_ptr()
-        v.dims[0], v.dims[1], v.dims[2] = X, Y, Z
-        v.iso = float(isos[i])
-        mn, mx = boxes[i] if boxes is not None else _default_box((X, Y, Z))
-        for k in range(3):
-            v.xyz_min[k] = float(mn[k])
-            v.xyz_max[k] = float(mx[k])
-        if outputs is not None and outputs[i] is not None:
-            verts, quads = outputs[i]
-            v.vertices = verts.data_ptr() if verts is not None and verts.numel() else None
-            v.vertex_capacity = verts.numel() // 3 if verts is not None else 0
-            v.quads = quads.data_ptr() if quads is not None and quads.numel() else None
-            v.quad_capacity = quads.numel() // 4 if quads is not None else 0
-    return arr
+        ptrs.append(f.data_ptr())
+        shapes.append(f.shape)
+    tab["field"] = ptrs
+    zyx = np.array(shapes, dtype=np.uint32)
+    tab["dims"] = zyx[:, ::-1]
+    tab["iso"] = np.asarray(isos, dtype=np.float64)
+    if boxes is None:
+        tab["xyz_max"] = np.maximum(zyx[:, ::-1].astype(np.int64) - 1, 1)
+    else:
+        tab["xyz_min"] = [b[0] for b in boxes]
+        tab["xyz_max"] = [b[1] for b in boxes]
+    if outputs is not None:
+        for i, o in enumerate(outputs):
+            if o is None:
+                continue
+            verts, quads = o
+            if verts is not None and verts.numel():
+                tab["vertices"][i] = verts.data_ptr()
+                tab["vertex_capacity"][i] = verts.numel() // 3
+            if quads is not None and quads.numel():
+                tab["quads"][i] = quads.data_ptr()
+                tab["quad_capacity"][i] = quads.numel() // 4
+    return tab
+
+
+def _table_ptr(tab):
+    return tab.ctypes.data_as(ctypes.POINTER(Volume))
 
 

@@ -208,6 +208,13 @@ def test_field_generators_match_cpu(ctx, torch_cuda):
     cg = oracle.field_gyroid(dims, 24.0, 12, 0.05)
     assert np.array_equal(gb.cpu().numpy().view(np.uint32), cb.view(np.uint32))
     assert np.array_equal(gg.cpu().numpy().view(np.uint32), cg.view(np.uint32))
+    # several groups of 8 blobs (the last one padded) and tasks whose row
+    # count is odd (29 * 23 rows is 11 past a multiple of 16)
+    for dims, nb in (((67, 29, 23), 27), ((130, 17, 9), 32), ((5, 3, 7), 17)):
+        gb = ctx.field_blobs(dims, seed=nb, nblobs=nb, sigma=(2.0, 9.0))
+        torch_cuda.cuda.synchronize()
+        cb = oracle.field_blobs(dims, nb, nb, sigma=(2.0, 9.0))
+        assert np.array_equal(gb.cpu().numpy().view(np.uint32), cb.view(np.uint32)), (dims, nb)
 
 
 def test_host_entry_matches_device(ctx, torch_cuda):
