@@ -487,8 +487,10 @@ class Context:
         The vertex (quad) arrays of a batch are consecutive views of one allocation."""
         torch = self._torch
         n = len(fields)
-        counts = self.batch_counts_tensor(n)
         table = _volume_table(fields, isos, boxes)
+        if n == 0:
+            return []
+        counts = self.batch_counts_tensor(n)
         self.batch_count(fields, isos, counts, stream=stream, _table=table)
         (stream or torch.cuda.current_stream()).synchronize()
         c = counts.cpu().numpy().view(np.uint32)

@@ -133,6 +133,9 @@ def test_batch_with_empty_and_flat_volumes(fa_mod, torch_cuda):
     # all volumes empty
     res = ctx.batch_extract([tiny, flat], [0.0, 0.0])
     assert all(r[2]["num_vertices"] == 0 for r in res)
+    assert all(r[0].shape == (0, 3) and r[1].shape == (0, 4) for r in res)
+    # no volumes at all
+    assert ctx.batch_extract([], []) == []
     ctx.close()
     sctx.close()
 
