@@ -59,7 +59,7 @@ def _quad_owners(r, dims):
 def _predict(r0, dims):
     """One smoothing step in float64 from the unsmoothed mesh: predicted new
     edge points (nan rows: unchanged) and a mask of quads too close to a
-    decision boundary to judge."""
+    decision boundary to predict."""
     X, Y, Z = dims
     own = _quad_owners(r0, dims)
     V = r0["vertices"].astype(np.float64)
