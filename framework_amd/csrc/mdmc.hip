@@ -822,6 +822,8 @@ using namespace mdmc;
 // ---------------------------------------------------------------------------
 // C ABI
 
+int mdmc::set_error(int code, const char *msg) { return fail(code, msg); }
+
 extern "C" {
 
 const char *mdmc_status_string(int status) {
@@ -840,6 +842,7 @@ const char *mdmc_status_string(int status) {
 }
 
 const char *mdmc_last_error(void) { return g_last_error.c_str(); }
+
 
 int mdmc_setup(void) {
   int dev = 0;

@@ -166,9 +166,14 @@ void staged_release() {
 
 extern "C" int mdmc_write_obj(const char *path, const float *vertices, uint64_t num_vertices, const int32_t *quads,
                               uint64_t num_quads) {
-  if (!path || (num_vertices && !vertices) || (num_quads && !quads)) return MDMC_ERR_INVALID;
+  if (!path || (num_vertices && !vertices) || (num_quads && !quads))
+    return mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: null path or mesh pointer");
+  // every face must name an existing vertex; checked before the file is made
+  for (uint64_t k = 0; k < 4 * num_quads; ++k)
+    if (quads[k] < 0 || (uint64_t)quads[k] >= num_vertices)
+      return mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: quad index outside the vertex array");
   FILE *f = std::fopen(path, "wb");
-  if (!f) return MDMC_ERR_INVALID;
+  if (!f) return mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: cannot open the output file");
   std::vector<char> buf(1u << 20);
   size_t used = 0;
   bool ok = true;
@@ -198,5 +203,5 @@ extern "C" int mdmc_write_obj(const char *path, const float *vertices, uint64_t 
   }
   flush();
   if (std::fclose(f) != 0) ok = false;
-  return ok ? MDMC_OK : MDMC_ERR_INVALID;
+  return ok ? MDMC_OK : mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: write to the output file failed");
 }

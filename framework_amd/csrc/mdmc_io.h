@@ -11,4 +11,7 @@ namespace mdmc {
 int staged_copy(void *host, void *dev, size_t bytes, int dir);
 void staged_release();
 
+// Records msg as this thread's mdmc_last_error() and returns code.
+int set_error(int code, const char *msg);
+
 }  // namespace mdmc

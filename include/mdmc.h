@@ -125,7 +125,9 @@ void mdmc_device_mesh_free(mdmc_device_mesh *mesh);
 /* Writes a mesh as Wavefront OBJ, like the reference's dump_obj
  * (dmc.cu:2238-2246): "v x y z" per vertex (six significant digits, as the
  * reference's std::ostream default) and two 1-based triangle faces
- * (q0,q1,q2), (q2,q3,q0) per quad.  Host memory only; no device call. */
+ * (q0,q1,q2), (q2,q3,q0) per quad.  Host memory only; no device call.
+ * MDMC_ERR_INVALID, with no file written, if a quad index lies outside
+ * [0, num_vertices). */
 int mdmc_write_obj(const char *path, const float *vertices, uint64_t num_vertices, const int32_t *quads,
                    uint64_t num_quads);
 

@@ -86,3 +86,18 @@ def test_obj_writers_agree(tmp_path):
     assert lines[0].startswith("v ") and lines[257] == "f %d %d %d" % (q[0, 0] + 1, q[0, 1] + 1, q[0, 2] + 1)
     fa.write_obj(str(tmp_path / "c.obj"), v, q)
     assert (tmp_path / "c.obj").read_bytes() == a
+
+
+def test_obj_writer_rejects_out_of_range_quads(tmp_path):
+    """A face naming a vertex outside the array is an error, and no file is made."""
+    import numpy as np
+
+    v = np.zeros((4, 3), np.float32)
+    for bad in (4, -1):
+        q = np.array([[0, 1, 2, bad]], np.int32)
+        out = tmp_path / ("bad%d.obj" % bad)
+        with pytest.raises(fa.mdmc.MDMCError, match="outside the vertex array"):
+            fa.write_obj(str(out), v, q)
+        assert not out.exists()
+    fa.write_obj(str(tmp_path / "ok.obj"), v, np.array([[0, 1, 2, 3]], np.int32))
+    assert (tmp_path / "ok.obj").read_text().count("\nf ") == 2
