@@ -205,6 +205,7 @@ def main() -> int:
     pipe_gbs = B / (ms_step * 1e-3) / 1e9
 
     cpu = None
+    fp64_error = None
     if not args.no_cpu:
         import oracle
 
@@ -223,14 +224,12 @@ def main() -> int:
             assert r["num_quads"] == Q and r["num_vertices"] == V
             return sorted(times)[len(times) // 2], len(times)
 
-        # the per-cell loops on every core of this process's CPU share (at
-        # most 16), and one thread
-        try:
-            ncpu = len(os.sched_getaffinity(0))
-        except AttributeError:
-            ncpu = os.cpu_count() or 1
-        nthr = max(1, min(16, ncpu))
-        log(f"CPU baseline (oracle, {nthr} threads and 1 thread)")
+        # the per-cell loops on every core of this process's CPU share: the
+        # affinity set, within the OpenMP thread limit the host sets for the
+        # process (the GPU's share of a shared host), and on one thread
+        host = cpu_host_info()
+        nthr = host["threads_used"]
+        log(f"CPU baseline (oracle, {nthr} threads and 1 thread) on {host['model']}")
         tmul, nmul = time_oracle(nthr, args.cpu_seconds / 2)
         tone, none = time_oracle(1, args.cpu_seconds / 2)
         oracle.set_threads(1)
@@ -238,8 +237,22 @@ def main() -> int:
                "sample": f"full 512^3 blob field, {nmul} run(s), median {tmul:.3f} s (C oracle restating "
                          f"dmc::run_dmc, OpenMP over cell planes and active cells, {nthr} threads, on the host "
                          f"of the GPU that ran the steps)",
+               "host": host,
                "single_thread": {"value": N / tone / 1e9, "cores": 1,
                                  "sample": f"same field, {none} run(s), median {tone:.3f} s"}}
+        # vertex error of the FP32 path against the oracle's FP64 twin (the
+        # same algorithm with every coordinate and interpolation in double)
+        oracle.set_threads(nthr)
+        r64 = oracle.run(fnp, ISO, fp64=True)
+        oracle.set_threads(1)
+        vg = verts.cpu().numpy().astype(np.float64)
+        fin = np.isfinite(vg) & np.isfinite(r64["vertices64"])
+        err = np.abs(vg - r64["vertices64"])[fin]
+        same = np.array_equal(verts.cpu().numpy().view(np.uint32), r64["vertices"].view(np.uint32))
+        fp64_error = {"max_voxel_units": float(err.max()), "mean_voxel_units": float(err.mean()),
+                      "vertices": int(len(vg)), "fp32_bitwise_equal_to_oracle": bool(same),
+                      "workload": "512^3 blobs (the headline field), voxel units (box 0..511)"}
+        log(f"FP64 twin: {fp64_error}")
 
     host = None if args.skip_host else bench_host(fa, field.cpu().numpy(), V, Q)
     dense = None if args.skip_dense else bench_dense(fa, torch)
@@ -272,6 +285,7 @@ def main() -> int:
         "pipeline": {"algorithmic_bytes": B, "achieved_gbs": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
                      "kernel_avg_ms": per_kernel},
         "cpu_baseline": cpu,
+        "vertex_error_vs_fp64": fp64_error,
         "host_path": host,
         "dense": dense,
         "batched": batched,
@@ -280,6 +294,31 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
+
+
+def cpu_host_info() -> dict:
+    """The host CPU: model, hardware threads, this process's affinity set and
+    the threads the CPU baseline uses (the affinity set, capped by
+    OMP_NUM_THREADS when the host sets it for the process)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    used = affinity
+    if omp.isdigit() and int(omp) > 0:
+        used = min(used, int(omp))
+    return {"model": model, "hardware_concurrency": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads": omp or None, "threads_used": max(1, used)}
 
 
 def bench_host(fa, fnp, V: int, Q: int, reps: int = 5) -> dict:
@@ -367,12 +406,14 @@ def _log_uniform_int(rng, lo, hi):
     return int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
 
 
-def bench_batched(fa, torch, nvol: int = 256, reps: int = 5, tune: bool = True) -> dict:
-    """256 uneven volumes: plain per-volume loop vs one batched launch sequence
-    (launch shapes chosen by the autotuner unless tune is False)."""
+def batched_workload(nvol: int = 256, seed: int = 1):
+    """The batched workload: nvol volumes with dims drawn per axis
+    log-uniform in [32, 256], alternating blob sums (8-32 blobs, sigma
+    U[4,16]) and gyroid (period 24) + noise 0.05, isovalues U[-0.2, 0.2].
+    Returns (generator specs, isovalues)."""
     import numpy as np
 
-    rng = np.random.default_rng(1)
+    rng = np.random.default_rng(seed)
     specs, isos = [], []
     for i in range(nvol):
         dims = tuple(_log_uniform_int(rng, 32, 256) for _ in range(3))
@@ -382,6 +423,15 @@ def bench_batched(fa, torch, nvol: int = 256, reps: int = 5, tune: bool = True) 
         else:
             specs.append({"dims": dims, "kind": "gyroid", "seed": 1000 + i, "period": 24.0, "noise_amp": 0.05})
         isos.append(float(rng.uniform(-0.2, 0.2)))
+    return specs, isos
+
+
+def bench_batched(fa, torch, nvol: int = 256, reps: int = 5, tune: bool = True) -> dict:
+    """256 uneven volumes: plain per-volume loop vs one batched launch sequence
+    (launch shapes chosen by the autotuner unless tune is False)."""
+    import numpy as np
+
+    specs, isos = batched_workload(nvol, seed=1)
     ctx = fa.Context()
     log("batched: generating 256 fields")
     fields = ctx.batch_generate(specs)
@@ -512,7 +562,8 @@ def bench_served(fa, torch, nreq: int, budget: int = 64 << 30, options: dict | N
            "memory_peak_gb": st["memory_peak"] / 1e9, "budget_gb": budget / 1e9,
            "plain_loop_volumes_per_s": plain_vps, "plain_loop_sample": nplain,
            "plain_loop_p50_ms": float(np.percentile(plain["latency"], 50) * 1e3),
-           "speedup_vs_plain_loop": capacity / plain_vps}
+           "speedup_vs_plain_loop": capacity / plain_vps,
+           "speedup_stream_vs_plain_loop": (nreq / res["wall"]) / plain_vps}
     log(f"served: {out}")
     return out
 

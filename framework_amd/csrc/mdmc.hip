@@ -27,6 +27,7 @@ std::mutex g_setup_mutex;
 std::mutex g_host_entry_mutex;  // the host entries share the default context
 uint64_t g_setup_mask = 0;  // bit d: tables uploaded on device d
 mdmc_ctx *g_default_ctx = nullptr;
+mdmc_ctx *g_ooc_ctx = nullptr;  // the out-of-core entry's own buffers (kept to its budget)
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -48,6 +49,76 @@ int current_device(int *dev) {
 bool is_setup_on(int dev) {
   std::lock_guard<std::mutex> lk(g_setup_mutex);
   return dev >= 0 && dev < 64 && ((g_setup_mask >> dev) & 1ull);
+}
+
+// Device copies of the tables in the reference's formats (one block per
+// device, mdmc_setup_device_luts), guarded by g_setup_mutex.
+void *g_ref_tables[64] = {};
+
+// Byte offsets of the tables in that block.
+enum : size_t {
+  kTabLut1 = 0, kTabLut2 = 3072, kTabNv1 = 6144, kTabNv2 = 6400, kTabAmb = 6656, kTabAmbFace = 6692,
+  kTabOpposite = 6728, kTabFaceDir = 6734, kTabEdgeOwner = 6740, kTabRing = 6752, kTabLocal = 6764,
+  kTabBytes = 6784
+};
+
+// The tables of the reference's 11-pointer setup (dmc.cu:45-661), built from
+// the case-table builder and the cube's geometry (corner offsets, edge and
+// face corner lists) rather than typed in.
+void build_reference_tables(uint8_t *out) {
+  std::memset(out, 0, kTabBytes);
+  mdmc_luts L;
+  mdmc_build_luts(&L);
+  std::memcpy(out + kTabLut1, L.edge_label[0], 3072);
+  std::memcpy(out + kTabLut2, L.edge_label[1], 3072);
+  std::memcpy(out + kTabNv1, L.num_vertex[0], 256);
+  std::memcpy(out + kTabNv2, L.num_vertex[1], 256);
+  int n = 0;
+  for (int c = 0; c < 256; ++c)
+    if (L.amb_face[c] != MDMC_NOT_AMBIGUOUS) {
+      out[kTabAmb + n] = (uint8_t)c;
+      out[kTabAmbFace + n] = L.amb_face[c];
+      ++n;
+    }
+  // the opposite face shares no corner
+  for (int f = 0; f < 6; ++f)
+    for (int g = 0; g < 6; ++g) {
+      bool shared = false;
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) shared |= MDMC_FACE_C[f][a] == MDMC_FACE_C[g][b];
+      if (!shared) out[kTabOpposite + f] = (uint8_t)g;
+    }
+  for (int f = 0; f < 6; ++f) out[kTabFaceDir + f] = MDMC_FACE_DIR[f];
+  // An edge along axis A at offsets (b, c) in {0,1} on the other two axes is
+  // owned by the cell at offset (b - 1, c - 1) on those axes, where it is the
+  // edge at (1, 1): edge 10 for x, 9 for y, 6 for z (the edges of corner 6).
+  const int owner_edge[3] = {10, 9, 6};
+  for (int e = 0; e < 12; ++e) {
+    const uint8_t *p = MDMC_CORNER_D[MDMC_EDGE_C[e][0]], *q = MDMC_CORNER_D[MDMC_EDGE_C[e][1]];
+    int axis = 0;
+    while (p[axis] == q[axis]) ++axis;
+    uint8_t code = 0;
+    for (int d = 0; d < 3; ++d)
+      if (d != axis && p[d] == 0) code |= (uint8_t)(0x80u >> d);
+    out[kTabEdgeOwner + e] = code ? (uint8_t)(code | owner_edge[axis]) : 0xFFu;
+  }
+  // Ring of owned edge l along axis A (z, y, x for 6, 9, 10): with B, C the
+  // next axes in cyclic order, the cells at (B, C) offsets (0,0), (1,0),
+  // (1,1), (0,1) -- counter-clockwise looking along +A -- each holding the
+  // edge at (1 - dB, 1 - dC).
+  const int ring_axis[3] = {2, 1, 0};
+  const int ring_d[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+  for (int l = 0; l < 3; ++l) {
+    const int A = ring_axis[l], B = (A + 1) % 3, C = (A + 2) % 3;
+    for (int k = 0; k < 4; ++k) {
+      for (int e = 0; e < 12; ++e) {
+        const uint8_t *p = MDMC_CORNER_D[MDMC_EDGE_C[e][0]], *q = MDMC_CORNER_D[MDMC_EDGE_C[e][1]];
+        if (p[A] != q[A] && p[B] == 1 - ring_d[k][0] && q[B] == p[B] && p[C] == 1 - ring_d[k][1] && q[C] == p[C])
+          out[kTabRing + 4 * l + k] = (uint8_t)e;
+      }
+    }
+    out[kTabLocal + l] = out[kTabRing + 4 * l];
+  }
 }
 
 }  // namespace
@@ -192,6 +263,15 @@ struct mdmc_ctx {
   size_t ghost_bytes = 0;
   hipEvent_t ghost_ev = nullptr;
   bool ghost_ev_used = false;
+  // stream capture: a context used inside a capture belongs to that graph
+  // (its kernels and copies hold pointers to the scratch and staging
+  // buffers), so every later call outside that capture is refused
+  bool captured = false;
+  unsigned long long capture_id = 0;
+  bool capturing_now = false;  // the current call's stream is capturing
+  // size the scratch exactly (no headroom): set for the out-of-core
+  // context, which keeps to a device budget
+  bool exact_scratch = false;
   // tuning options (0 = default)
   int opt[mdmc::kNumOptions] = {};
   // per-kernel event timing
@@ -229,6 +309,30 @@ struct ProfScope {
 
 static size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Called first by every entry point that launches work on a context.  While
+// `st` is capturing, the context becomes the property of that capture (the
+// graph's nodes hold pointers to its scratch, staging tables and volume
+// table, which a later call would rewrite, regrow or free); calls outside
+// that capture, eager or in another capture, are refused from then on.  The
+// context stays usable by the graph until it is destroyed.
+int capture_guard(mdmc_ctx *ctx, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  unsigned long long id = 0;
+  HIP_TRY(hipStreamGetCaptureInfo(st, &cs, &id));
+  ctx->capturing_now = cs != hipStreamCaptureStatusNone;
+  if (ctx->capturing_now) {
+    if (ctx->captured && ctx->capture_id != id)
+      return fail(MDMC_ERR_INVALID, "context already belongs to another captured graph (use one context per graph)");
+    ctx->captured = true;
+    ctx->capture_id = id;
+    return MDMC_OK;
+  }
+  if (ctx->captured)
+    return fail(MDMC_ERR_INVALID,
+                "context belongs to a captured graph whose nodes use its buffers; use another context outside it");
+  return MDMC_OK;
+}
+
 struct ScratchLayout {
   size_t counts, vq, ne, chunk, state, ticket, totals, slots, gmap, coords, total;
 };
@@ -265,14 +369,18 @@ int ensure_scratch(mdmc_ctx *ctx, const Plan &p) {
                o_coords = lay.coords;
   if (off > ctx->scratch_bytes && ctx->scratch_external)
     return fail(MDMC_ERR_CAPACITY, "the scratch memory given to the context is too small");
-  if (off > ctx->scratch_bytes) {
+  const bool regrow = off > ctx->scratch_bytes;
+  if (regrow && ctx->capturing_now)
+    return fail(MDMC_ERR_INVALID, "scratch must be sized by a call on this context before stream capture");
+  if (regrow) {
     if (ctx->scratch) {
       hipError_t e = hipFree(ctx->scratch);
       (void)e;
       ctx->scratch = nullptr;
       ctx->scratch_bytes = 0;
     }
-    const size_t want = off + off / 8;  // headroom for slightly larger grids
+    // headroom for slightly larger grids, unless the context keeps to a budget
+    const size_t want = ctx->exact_scratch ? off : off + off / 8;
     if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
       (void)hipGetLastError();
       if (hipMalloc(&ctx->scratch, off) != hipSuccess) {
@@ -416,9 +524,15 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
   {
     ProfScope ps(ctx, 2, st);
-    if (!have_coords)
+    if (!have_coords) {
       hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0,
                          st, co);
+      // the tables now hold this box: a later emit of the same count with
+      // another box must rewrite them, one with this box may reuse them
+      std::memcpy(ctx->coords_box, mn, 3 * sizeof(float));
+      std::memcpy(ctx->coords_box + 3, mx, 3 * sizeof(float));
+      ctx->coords_valid = true;
+    }
     // one wave per chunk of 64 active cells, at most 8 workgroups per CU
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
     const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
@@ -500,6 +614,8 @@ int ensure_batch_buffers(mdmc_ctx *ctx, uint32_t n) {
   const size_t o_base = align256(n * sizeof(BatchVol));
   const size_t o_cnt = o_base + align256(n * sizeof(uint2));
   const size_t need = o_cnt + 256;
+  if ((need > ctx->bdev_bytes || n > ctx->bhost_cap) && ctx->capturing_now)
+    return fail(MDMC_ERR_INVALID, "batch tables must be sized by a call on this context before stream capture");
   if (need > ctx->bdev_bytes) {
     if (ctx->bdev) (void)hipFree(ctx->bdev);
     ctx->bdev = nullptr;
@@ -810,6 +926,9 @@ int batch_emit(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *
     hipLaunchKernelGGL(k_coords_batch, dim3(std::min<uint32_t>(n, (uint32_t)ctx->num_cus * 4)), dim3(256), 0, st,
                        static_cast<const BatchVol *>(batch_table_dev(ctx)), n);
     hipLaunchKernelGGL(k_emit_batch, dim3(blocks), dim3(kEmitThreads), 0, st, be);
+    hipLaunchKernelGGL(k_batch_status, dim3((n + 255) / 256), dim3(256), 0, st,
+                       static_cast<const BatchVol *>(batch_table_dev(ctx)), n,
+                       static_cast<const uint32_t *>(batch_counts_dev(ctx, n) + 3));
   }
   HIP_TRY(hipGetLastError());
   return MDMC_OK;
@@ -883,11 +1002,86 @@ int mdmc_cleanup(void) {
       mdmc_ctx_destroy(g_default_ctx);
       g_default_ctx = nullptr;
     }
+    if (g_ooc_ctx) {
+      mdmc_ctx_destroy(g_ooc_ctx);
+      g_ooc_ctx = nullptr;
+    }
   }
   staged_release();
   std::lock_guard<std::mutex> lk(g_setup_mutex);
-  if (dev >= 0 && dev < 64) g_setup_mask &= ~(1ull << dev);
+  if (dev >= 0 && dev < 64) {
+    g_setup_mask &= ~(1ull << dev);
+    if (g_ref_tables[dev]) {
+      (void)hipFree(g_ref_tables[dev]);
+      g_ref_tables[dev] = nullptr;
+    }
+  }
   return MDMC_OK;
+}
+
+static void fill_table_pointers(uint8_t *b, mdmc_device_luts *o) {
+  o->config_edge_lut1 = b + kTabLut1;
+  o->config_edge_lut2 = b + kTabLut2;
+  o->num_vertex_lut1 = b + kTabNv1;
+  o->num_vertex_lut2 = b + kTabNv2;
+  o->config_2B_3B_lut = b + kTabAmb;
+  o->config_2B_3B_ambiguous_face = b + kTabAmbFace;
+  o->opposite_face_lut = b + kTabOpposite;
+  o->face_to_check_dir_lut = b + kTabFaceDir;
+  o->edge_belonged_voxel_lut = b + kTabEdgeOwner;
+  o->circular_edge_lut = b + kTabRing;
+  o->voxel_local_edges = b + kTabLocal;
+}
+
+int mdmc_setup_device_luts(mdmc_device_luts *out) {
+  if (!out) return fail(MDMC_ERR_INVALID, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  int rc = mdmc_setup();
+  if (rc != MDMC_OK) return rc;
+  int dev = 0;
+  rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  std::lock_guard<std::mutex> lk(g_setup_mutex);
+  if (!g_ref_tables[dev]) {
+    uint8_t host[kTabBytes];
+    build_reference_tables(host);
+    void *d = nullptr;
+    if (hipMalloc(&d, kTabBytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(MDMC_ERR_NOMEM, "table allocation failed");
+    }
+    if (hipMemcpy(d, host, kTabBytes, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(d);
+      return fail(MDMC_ERR_HIP, "table upload failed");
+    }
+    g_ref_tables[dev] = d;
+  }
+  fill_table_pointers(static_cast<uint8_t *>(g_ref_tables[dev]), out);
+  return MDMC_OK;
+}
+
+int mdmc_reference_luts_host(uint8_t *out) {
+  static_assert(kTabBytes == MDMC_REF_LUTS_BYTES, "table block size");
+  if (!out) return fail(MDMC_ERR_INVALID, "null argument");
+  build_reference_tables(out);
+  return MDMC_OK;
+}
+
+int mdmc_cleanup_device_luts(const mdmc_device_luts *luts) {
+  if (!luts) return fail(MDMC_ERR_INVALID, "null argument");
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_setup_mutex);
+    mdmc_device_luts mine;
+    std::memset(&mine, 0, sizeof(mine));
+    if (g_ref_tables[dev]) fill_table_pointers(static_cast<uint8_t *>(g_ref_tables[dev]), &mine);
+    if (!g_ref_tables[dev] || std::memcmp(&mine, luts, sizeof(mine)) != 0)
+      return fail(MDMC_ERR_INVALID, "table pointers were not handed out by mdmc_setup_device_luts on this device");
+  }
+  return mdmc_cleanup();  // frees the table block too
 }
 
 int mdmc_ctx_create(mdmc_ctx **out) {
@@ -949,8 +1143,10 @@ static int count_impl(mdmc_ctx *ctx, const float *d_field, const uint32_t dims[3
                       mdmc_counts *d_counts, void *stream, const float *box_min, const float *box_max) {
   if (!ctx || !dims || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
+  int rc = capture_guard(ctx, static_cast<hipStream_t>(stream));
+  if (rc != MDMC_OK) return rc;
   Plan p;
-  int rc = plan_tiling(dims, &p, ctx->opt);
+  rc = plan_tiling(dims, &p, ctx->opt);
   if (rc != MDMC_OK) return rc;
   if (!p.empty && !d_field) return fail(MDMC_ERR_INVALID, "null field");
   rc = ensure_scratch(ctx, p);
@@ -976,6 +1172,8 @@ int mdmc_emit_device(mdmc_ctx *ctx, const float *d_field, const float xyz_min[3]
   if (!ctx || !xyz_min || !xyz_max || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (!ctx->counted) return fail(MDMC_ERR_INVALID, "emit without a preceding count");
   if (d_field != ctx->counted_field) return fail(MDMC_ERR_INVALID, "emit field differs from counted field");
+  const int grc = capture_guard(ctx, static_cast<hipStream_t>(stream));
+  if (grc != MDMC_OK) return grc;
   return launch_emit(ctx, ctx->plan, d_field, ctx->iso, xyz_min, xyz_max, d_vertices, vertex_capacity,
                      d_quads, quad_capacity, d_counts, static_cast<hipStream_t>(stream));
 }
@@ -996,6 +1194,8 @@ int mdmc_smooth_device(mdmc_ctx *ctx, float *d_vertices, uint64_t vertex_capacit
   if (!ctx || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (num_smooth == 0) return MDMC_OK;
   if (!ctx->emitted) return fail(MDMC_ERR_INVALID, "smoothing without a preceding emit on this context");
+  const int grc = capture_guard(ctx, static_cast<hipStream_t>(stream));
+  if (grc != MDMC_OK) return grc;
   return launch_smooth(ctx, d_vertices, vertex_capacity, const_cast<int32_t *>(d_quads), quad_capacity, num_smooth,
                        d_counts, static_cast<hipStream_t>(stream));
 }
@@ -1007,11 +1207,12 @@ int mdmc_ctx_set_option(mdmc_ctx *ctx, int option, int value) {
   return MDMC_OK;
 }
 
-static int check_batch_args(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts) {
+static int check_batch_args(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
+                            void *stream) {
   if (!ctx || !vols || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
   if (n == 0 || n > kMaxBatch) return fail(MDMC_ERR_INVALID, "batch size must be 1..1024");
   if (!is_setup_on(ctx->device)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
-  return MDMC_OK;
+  return capture_guard(ctx, static_cast<hipStream_t>(stream));
 }
 
 int mdmc_batch_scratch_bytes(const mdmc_volume *vols, uint32_t n, uint64_t *bytes) {
@@ -1027,6 +1228,7 @@ int mdmc_batch_scratch_bytes(const mdmc_volume *vols, uint32_t n, uint64_t *byte
 
 int mdmc_ctx_set_scratch(mdmc_ctx *ctx, void *d_scratch, uint64_t bytes) {
   if (!ctx) return fail(MDMC_ERR_INVALID, "null ctx");
+  if (ctx->captured) return fail(MDMC_ERR_INVALID, "context belongs to a captured graph whose nodes use its scratch");
   if (ctx->scratch && !ctx->scratch_external) HIP_TRY(hipFree(ctx->scratch));
   ctx->scratch = d_scratch;
   ctx->scratch_bytes = d_scratch ? bytes : 0;
@@ -1038,21 +1240,21 @@ int mdmc_ctx_set_scratch(mdmc_ctx *ctx, void *d_scratch, uint64_t bytes) {
 
 int mdmc_batch_count_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
                             void *stream) {
-  int rc = check_batch_args(ctx, vols, n, d_counts);
+  int rc = check_batch_args(ctx, vols, n, d_counts, stream);
   if (rc != MDMC_OK) return rc;
   return batch_count(ctx, vols, n, d_counts, static_cast<hipStream_t>(stream));
 }
 
 int mdmc_batch_emit_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
                            void *stream) {
-  int rc = check_batch_args(ctx, vols, n, d_counts);
+  int rc = check_batch_args(ctx, vols, n, d_counts, stream);
   if (rc != MDMC_OK) return rc;
   return batch_emit(ctx, vols, n, d_counts, static_cast<hipStream_t>(stream), true);
 }
 
 int mdmc_batch_extract_device(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *d_counts,
                               void *stream) {
-  int rc = check_batch_args(ctx, vols, n, d_counts);
+  int rc = check_batch_args(ctx, vols, n, d_counts, stream);
   if (rc != MDMC_OK) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   rc = batch_count(ctx, vols, n, d_counts, st);
@@ -1065,6 +1267,8 @@ int mdmc_cells_device(mdmc_ctx *ctx, uint32_t *d_cell_index, uint8_t *d_cfg, uin
   if (!ctx || !d_cell_index || !d_cfg || !d_flags || !d_vertex_begin)
     return fail(MDMC_ERR_INVALID, "null argument");
   if (!ctx->counted) return fail(MDMC_ERR_INVALID, "no counted extraction on this context");
+  const int grc = capture_guard(ctx, static_cast<hipStream_t>(stream));
+  if (grc != MDMC_OK) return grc;
   const Plan &p = ctx->plan;
   if (p.empty) return MDMC_OK;
   CellsArgs ca{};
@@ -1292,11 +1496,14 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
   int rc = current_device(&dev);
   if (rc != MDMC_OK) return rc;
   if (!is_setup_on(dev)) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called");
-  if (!g_default_ctx) {
-    rc = mdmc_ctx_create(&g_default_ctx);
+  // a context of its own, so that no buffer of the in-core entries counts
+  // against the budget, with its scratch sized exactly
+  if (!g_ooc_ctx) {
+    rc = mdmc_ctx_create(&g_ooc_ctx);
     if (rc != MDMC_OK) return rc;
+    g_ooc_ctx->exact_scratch = true;
   }
-  mdmc_ctx *ctx = g_default_ctx;
+  mdmc_ctx *ctx = g_ooc_ctx;
   const uint32_t X = dims[0], Y = dims[1], Z = dims[2];
   if (X < 2 || Y < 2 || Z < 2) return MDMC_OK;
   if (!field) return fail(MDMC_ERR_INVALID, "null field");
@@ -1305,6 +1512,22 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
   uint32_t S = 0;
   rc = ooc_slab_planes(dims, device_budget, ctx->opt, &S);
   if (rc != MDMC_OK) return rc;
+  {
+    // the scratch of the largest slab, exactly: every slab's layout fits in
+    // it, and a larger scratch left by an earlier call is given back first
+    const uint32_t sd[3] = {X, Y, S < cz ? std::min(S + 4, Z) : Z};
+    Plan pmax;
+    rc = plan_tiling(sd, &pmax, ctx->opt);
+    if (rc != MDMC_OK) return rc;
+    const size_t want = scratch_layout(pmax).total;
+    if (ctx->scratch && ctx->scratch_bytes != want) {
+      (void)hipFree(ctx->scratch);
+      ctx->scratch = nullptr;
+      ctx->scratch_bytes = 0;
+    }
+    rc = ensure_scratch(ctx, pmax);
+    if (rc != MDMC_OK) return rc;
+  }
   // point planes of a slab of cell planes [c0, c0 + n)
   auto slab_planes = [&](uint32_t c0, uint32_t n, uint32_t *lo) {
     const uint32_t l = c0 > 0 ? c0 - 1 : 0, hi = std::min(c0 + n + 2, Z - 1);
@@ -1549,9 +1772,13 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
   }
   const size_t head = align256(n * sizeof(GenVol));
   const size_t bytes = head + ndoubles * sizeof(double);
+  int grc = capture_guard(ctx, st);
+  if (grc != MDMC_OK) return grc;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   HIP_TRY(hipStreamIsCapturing(st, &cs));
   if (cs == hipStreamCaptureStatusNone && ctx->ghost_ev_used) HIP_TRY(hipEventSynchronize(ctx->ghost_ev));
+  if ((bytes > ctx->ghost_bytes || bytes > ctx->gdev_bytes) && cs != hipStreamCaptureStatusNone)
+    return fail(MDMC_ERR_INVALID, "generator tables must be sized by a call on this context before stream capture");
   if (bytes > ctx->ghost_bytes) {
     if (ctx->ghost) (void)hipHostFree(ctx->ghost);
     ctx->ghost = nullptr;

@@ -1656,6 +1656,18 @@ __global__ __launch_bounds__(256) void k_batch_counts(BatchCountArgs b) {
   }
 }
 
+// After a batch emit: status bits the emit could only attribute to the batch
+// (a chunk whose cell could not be located) go to every volume's counts, the
+// words callers read.
+__global__ __launch_bounds__(256) void k_batch_status(const BatchVol *vols, uint32_t nvol,
+                                                      const uint32_t *batch_status) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvol) return;
+  const uint32_t st = *batch_status;
+  uint32_t *const cnt = vols[v].counts;
+  if (st && cnt) atomicOr(cnt + 3, st);
+}
+
 // One workgroup per volume: its coordinate tables.
 __global__ __launch_bounds__(256) void k_coords_batch(const BatchVol *vols, uint32_t nvol) {
   for (uint32_t v = blockIdx.x; v < nvol; v += gridDim.x) {

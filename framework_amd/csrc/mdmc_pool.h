@@ -64,6 +64,22 @@ class Pool {
     return true;
   }
 
+  // Cuts a block returned by alloc down to its first `bytes` (rounded up to
+  // the alignment) and frees the rest; a block cut to nothing is released.
+  // False for an unknown offset or a size above the block's.
+  bool shrink(uint64_t off, uint64_t bytes) {
+    auto u = used_.find(off);
+    if (u == used_.end()) return false;
+    const uint64_t keep = (bytes + align_ - 1) / align_ * align_;
+    if (keep > u->second) return false;
+    if (keep == 0) return release(off);
+    if (keep == u->second) return true;
+    const uint64_t tail = off + keep, tail_len = u->second - keep;
+    u->second = keep;
+    used_[tail] = tail_len;  // in_use_ still counts it: release() takes it off
+    return release(tail);
+  }
+
   uint64_t size() const { return size_; }
   uint64_t in_use() const { return in_use_; }
   uint64_t largest_free() const {

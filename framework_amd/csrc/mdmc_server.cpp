@@ -2,10 +2,15 @@
 //
 // Memory: one device allocation of the budget, carved by a first-fit pool.
 // A request's footprint is its field (when the server generates it), its
-// share of the batch scratch and an output estimate; admission keeps the sum
-// of the footprints of unfinished requests within the budget.  Outputs are
-// allocated with their exact sizes once the count pass of their batch has
-// finished (the scan gives the sizes).
+// share of the batch scratch and the largest mesh its dims allow (4 iso
+// vertices and 3 edge points per cell, one quad per owned edge whose ring
+// lies in the grid).  A request is admitted when its field and worst-case
+// output regions can be carved from the pool; its batch then runs generate
+// -> count -> emit as one stream-ordered sequence (no host round trip between
+// the count and the emit), and once it has finished each output region is
+// cut down to the exact size the count gave.  So an admitted request always
+// has room for its output: none is rejected after its count.  A request whose
+// footprint exceeds the whole budget is rejected when it is submitted.
 //
 // Progress: a worker thread owned by the server drives the lanes (every
 // phase change is noticed within microseconds, whatever the caller is
@@ -14,12 +19,11 @@
 //
 // Scheduling: `num_streams` lanes, each with its own stream and extraction
 // context.  An idle lane takes the queued requests in arrival order (up to
-// max_batch, as long as their footprints fit), generates their fields, and
-// runs the batch count; the next poll that sees the count finished allocates
-// the outputs and runs the batch emit; the poll that sees the emit finished
-// frees fields and scratch and hands out the results.  Lanes run
-// independently, so a batch started for late arrivals overlaps with the
-// batches already on the device.
+// max_batch, as long as their regions can be carved), generates their
+// fields, counts and emits them, and copies the counts back; the worker that
+// sees that sequence finish trims the outputs, frees fields and scratch and
+// hands out the results.  Lanes run independently, so a batch started for
+// late arrivals overlaps with the batches already on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +46,7 @@
 namespace {
 
 constexpr uint64_t kNoOffset = ~0ull;
+constexpr uint64_t kPoolAlign = 256;
 
 struct Req {
   uint64_t id = 0;
@@ -50,7 +55,8 @@ struct Req {
   int state = 0;  // 0 queued, 1 in a batch, 2 finished, 3 returned to the caller
   double t_submit = 0, t_start = 0, t_done = 0;
   uint64_t field_off = kNoOffset, vert_off = kNoOffset, quad_off = kNoOffset;
-  uint64_t footprint = 0, scratch_est = 0, out_est = 0;
+  uint64_t vmax = 0, qmax = 0;  // worst-case mesh of the dims
+  uint64_t footprint = 0, scratch_est = 0;
   const float *field = nullptr;
   float *verts = nullptr;
   int32_t *quads = nullptr;
@@ -61,7 +67,6 @@ struct Batch {
   std::vector<Req *> reqs;
   std::vector<mdmc_volume> vols;
   uint64_t scratch_off = kNoOffset;
-  int phase = 0;  // 1 counting, 2 waiting for output memory, 3 emitting
 };
 
 struct Lane {
@@ -69,7 +74,7 @@ struct Lane {
   mdmc_ctx *ctx = nullptr;
   mdmc_counts *d_counts = nullptr;  // device, MDMC_MAX_BATCH entries
   mdmc_counts *h_counts = nullptr;  // pinned
-  hipEvent_t ev = nullptr;          // end of the current phase
+  hipEvent_t ev = nullptr;          // end of the batch's sequence
   Batch *batch = nullptr;
 };
 
@@ -93,6 +98,22 @@ uint64_t field_bytes(const mdmc_request &r) {
   return 4ull * r.dims[0] * r.dims[1] * r.dims[2];
 }
 
+uint64_t round_up(uint64_t v) { return (v + kPoolAlign - 1) / kPoolAlign * kPoolAlign; }
+
+// The largest mesh a grid of these dims can have: 4 iso vertices (the
+// configurations 0x5A and 0xA5) and 3 edge points per cell, and a quad per
+// owned edge (6, 9, 10) whose four-cell ring lies in the grid
+// (dmc.cu:1382-1406, 1696-1708).  A checkerboard field reaches both.
+void worst_case_mesh(const uint32_t d[3], uint64_t *V, uint64_t *Q) {
+  if (d[0] < 2 || d[1] < 2 || d[2] < 2) {
+    *V = *Q = 0;
+    return;
+  }
+  const uint64_t cx = d[0] - 1, cy = d[1] - 1, cz = d[2] - 1;
+  *V = 7 * cx * cy * cz;
+  *Q = (cx - 1) * (cy - 1) * cz + (cx - 1) * cy * (cz - 1) + cx * (cy - 1) * (cz - 1);
+}
+
 }  // namespace
 
 struct mdmc_server {
@@ -104,9 +125,6 @@ struct mdmc_server {
   std::unordered_map<uint64_t, Req *> reqs;
   std::deque<Req *> finished;  // not yet returned by poll
   uint64_t next_id = 1;
-  uint64_t reserved = 0;       // footprints of admitted, unfinished requests
-  uint64_t held = 0;           // outputs of finished results not yet released
-  double out_ratio = 1.0;      // output bytes per field byte used for estimates
   uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, peak = 0;
   double busy = 0, busy_since = -1;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -143,7 +161,7 @@ struct mdmc_server {
 namespace {
 
 bool pool_alloc(mdmc_server *s, uint64_t bytes, uint64_t *off) {
-  if (!s->pool.alloc(std::max<uint64_t>(bytes, 256), off)) return false;
+  if (!s->pool.alloc(std::max<uint64_t>(bytes, kPoolAlign), off)) return false;
   s->peak = std::max(s->peak, s->pool.in_use());
   return true;
 }
@@ -153,8 +171,18 @@ void pool_free(mdmc_server *s, uint64_t *off) {
   *off = kNoOffset;
 }
 
-// Scratch the count pass needs for one volume alone (the estimate admission
-// uses; a batch's exact scratch is allocated when it starts).
+// Cuts a region down to `bytes` (nothing: the region goes back to the pool).
+void pool_trim(mdmc_server *s, uint64_t *off, uint64_t bytes) {
+  if (*off == kNoOffset) return;
+  if (bytes == 0) {
+    pool_free(s, off);
+    return;
+  }
+  s->pool.shrink(*off, bytes);
+}
+
+// Scratch the count pass needs for one volume alone (the estimate in the
+// footprint; a batch's exact scratch is carved when it starts).
 uint64_t scratch_estimate(const mdmc_request &r) {
   mdmc_volume v{};
   std::memcpy(v.dims, r.dims, sizeof(v.dims));
@@ -172,24 +200,46 @@ void finish(mdmc_server *s, Req *r, int status) {
   if (r->status == MDMC_ERR_REJECTED) s->rejected += 1;
 }
 
-// Frees everything a batch holds except the outputs of successful requests.
+// Frees fields and scratch of a finished batch, and the output regions of
+// its failed requests.  The lane's stream must have finished the batch's
+// work (on an error path the caller synchronises it first).
 void retire_batch(mdmc_server *s, Lane &lane) {
   Batch *b = lane.batch;
   for (Req *r : b->reqs) {
     pool_free(s, &r->field_off);
-    s->reserved -= std::min(s->reserved, r->footprint);
     if (r->status != MDMC_OK) {
       pool_free(s, &r->vert_off);
       pool_free(s, &r->quad_off);
       r->verts = nullptr;
       r->quads = nullptr;
-    } else {
-      s->held += (uint64_t)(12ull * r->counts.num_vertices + 16ull * r->counts.num_quads);
     }
   }
   pool_free(s, &b->scratch_off);
   delete b;
   lane.batch = nullptr;
+}
+
+// Ends every request of the lane's batch with `rc` after anything already
+// queued on the lane's stream has run (its kernels may still use the
+// batch's regions, which go back to the pool here).
+void fail_batch(mdmc_server *s, Lane &lane, int rc) {
+  if (hipStreamSynchronize(lane.stream) != hipSuccess) (void)hipGetLastError();
+  for (Req *r : lane.batch->reqs) finish(s, r, rc);
+  retire_batch(s, lane);
+}
+
+// Carves a queued request's regions: its field (generated requests) and its
+// worst-case vertex and quad arrays.  False, holding nothing, if they do not
+// fit now.
+bool carve_request(mdmc_server *s, Req *r) {
+  if (r->spec.field_kind != MDMC_FIELD_DEVICE && !pool_alloc(s, field_bytes(r->spec), &r->field_off)) return false;
+  if ((r->vmax && !pool_alloc(s, 12 * r->vmax, &r->vert_off)) ||
+      (r->qmax && !pool_alloc(s, 16 * r->qmax, &r->quad_off))) {
+    pool_free(s, &r->field_off);
+    pool_free(s, &r->vert_off);
+    return false;
+  }
+  return true;
 }
 
 // Starts a batch on an idle lane from the head of the queue.
@@ -199,9 +249,8 @@ int start_batch(mdmc_server *s, Lane &lane) {
   std::vector<mdmc_gen_spec> gens;
   while (!s->queue.empty() && b->reqs.size() < s->cfg.max_batch) {
     Req *r = s->queue.front();
-    if (s->reserved + s->held + r->footprint > s->pool.size()) break;  // wait for memory (arrival order)
+    if (!carve_request(s, r)) break;  // wait for memory (arrival order)
     if (r->spec.field_kind != MDMC_FIELD_DEVICE) {
-      if (!pool_alloc(s, field_bytes(r->spec), &r->field_off)) break;
       r->field = reinterpret_cast<const float *>(s->base + r->field_off);
       mdmc_gen_spec g{};
       g.out = const_cast<float *>(r->field);
@@ -219,8 +268,9 @@ int start_batch(mdmc_server *s, Lane &lane) {
     } else {
       r->field = r->spec.field;
     }
+    r->verts = r->vmax ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
+    r->quads = r->qmax ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
     s->queue.pop_front();
-    s->reserved += r->footprint;
     r->state = 1;
     r->t_start = s->now();
     b->reqs.push_back(r);
@@ -230,6 +280,10 @@ int start_batch(mdmc_server *s, Lane &lane) {
     v.iso = r->spec.iso;
     std::memcpy(v.xyz_min, r->spec.xyz_min, sizeof(v.xyz_min));
     std::memcpy(v.xyz_max, r->spec.xyz_max, sizeof(v.xyz_max));
+    v.vertices = r->verts;
+    v.vertex_capacity = r->vmax;
+    v.quads = r->quads;
+    v.quad_capacity = r->qmax;
     b->vols.push_back(v);
   }
   const uint32_t n = (uint32_t)b->reqs.size();
@@ -237,26 +291,29 @@ int start_batch(mdmc_server *s, Lane &lane) {
     delete b;
     return MDMC_OK;
   }
-  TRACE("start batch of %u on stream %p (reserved %llu, in use %llu)", n, (void *)lane.stream,
-        (unsigned long long)s->reserved, (unsigned long long)s->pool.in_use());
+  TRACE("start batch of %u on stream %p (in use %llu)", n, (void *)lane.stream,
+        (unsigned long long)s->pool.in_use());
   lane.batch = b;
   s->batches += 1;
-  auto fail_batch = [&](int rc) {
-    for (Req *r : b->reqs) finish(s, r, rc);
-    retire_batch(s, lane);
-    return MDMC_OK;
-  };
   uint64_t sbytes = 0;
   int rc = mdmc_batch_scratch_bytes(b->vols.data(), n, &sbytes);
-  if (rc != MDMC_OK) return fail_batch(rc);
+  if (rc != MDMC_OK) {
+    fail_batch(s, lane, rc);
+    return MDMC_OK;
+  }
   TRACE("  scratch %llu bytes, %zu fields to generate", (unsigned long long)sbytes, gens.size());
   if (sbytes) {
     if (!pool_alloc(s, sbytes, &b->scratch_off)) {
-      // fragmentation: give the requests back to the queue, in order
+      // no room for the batch's scratch: give the requests back to the
+      // queue, in order (nothing was launched)
       for (auto it = b->reqs.rbegin(); it != b->reqs.rend(); ++it) {
         Req *r = *it;
         pool_free(s, &r->field_off);
-        s->reserved -= std::min(s->reserved, r->footprint);
+        pool_free(s, &r->vert_off);
+        pool_free(s, &r->quad_off);
+        r->field = nullptr;
+        r->verts = nullptr;
+        r->quads = nullptr;
         r->state = 0;
         s->queue.push_front(r);
       }
@@ -266,60 +323,22 @@ int start_batch(mdmc_server *s, Lane &lane) {
       return MDMC_OK;
     }
     rc = mdmc_ctx_set_scratch(lane.ctx, s->base + b->scratch_off, sbytes);
-    if (rc != MDMC_OK) return fail_batch(rc);
+    if (rc != MDMC_OK) {
+      fail_batch(s, lane, rc);
+      return MDMC_OK;
+    }
   }
   if (!gens.empty()) {
     rc = mdmc_batch_generate_device(lane.ctx, gens.data(), (uint32_t)gens.size(), lane.stream);
     TRACE("  generate rc %d", rc);
-    if (rc != MDMC_OK) return fail_batch(rc);
-  }
-  rc = mdmc_batch_count_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
-  TRACE("  count rc %d", rc);
-  if (rc != MDMC_OK) return fail_batch(rc);
-  if (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
-                     lane.stream) != hipSuccess ||
-      hipEventRecord(lane.ev, lane.stream) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail_batch(MDMC_ERR_HIP);
-  }
-  b->phase = 1;
-  return MDMC_OK;
-}
-
-// Allocates exact outputs and launches the emit; false if memory is short.
-bool start_emit(mdmc_server *s, Lane &lane) {
-  Batch *b = lane.batch;
-  const uint32_t n = (uint32_t)b->reqs.size();
-  std::vector<uint64_t> got;
-  bool ok = true;
-  for (uint32_t i = 0; i < n && ok; ++i) {
-    Req *r = b->reqs[i];
-    mdmc_volume &v = b->vols[i];
-    if (r->status != MDMC_OK) continue;
-    const uint64_t V = r->counts.num_vertices, Q = r->counts.num_quads;
-    if (V && r->vert_off == kNoOffset) {
-      if (pool_alloc(s, 12 * V, &r->vert_off)) got.push_back(r->vert_off); else ok = false;
+    if (rc != MDMC_OK) {
+      fail_batch(s, lane, rc);
+      return MDMC_OK;
     }
-    if (ok && Q && r->quad_off == kNoOffset) {
-      if (pool_alloc(s, 16 * Q, &r->quad_off)) got.push_back(r->quad_off); else ok = false;
-    }
-    if (!ok) break;
-    r->verts = V ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
-    r->quads = Q ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
-    v.vertices = r->verts;
-    v.vertex_capacity = V;
-    v.quads = r->quads;
-    v.quad_capacity = Q;
   }
-  if (!ok) {
-    for (Req *r : b->reqs) {
-      pool_free(s, &r->vert_off);
-      pool_free(s, &r->quad_off);
-    }
-    return false;
-  }
-  int rc = mdmc_batch_emit_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
-  TRACE("emit batch of %u rc %d", n, rc);
+  // count and emit in one sequence: the outputs have their worst-case size
+  rc = mdmc_batch_extract_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
+  TRACE("  extract rc %d", rc);
   if (rc == MDMC_OK &&
       (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
                       lane.stream) != hipSuccess ||
@@ -327,13 +346,8 @@ bool start_emit(mdmc_server *s, Lane &lane) {
     (void)hipGetLastError();
     rc = MDMC_ERR_HIP;
   }
-  if (rc != MDMC_OK) {
-    for (Req *r : b->reqs) finish(s, r, rc);
-    retire_batch(s, lane);
-    return true;
-  }
-  b->phase = 3;
-  return true;
+  if (rc != MDMC_OK) fail_batch(s, lane, rc);
+  return MDMC_OK;
 }
 
 // One pass over the lanes; returns true if anything changed.
@@ -342,50 +356,29 @@ bool pump(mdmc_server *s) {
   for (Lane &lane : s->lanes) {
     Batch *b = lane.batch;
     if (!b) continue;
-    if (b->phase == 1 || b->phase == 3) {
-      const hipError_t q = hipEventQuery(lane.ev);
-      if (q == hipErrorNotReady) continue;
-      if (q != hipSuccess) {
-        (void)hipGetLastError();
-        for (Req *r : b->reqs) finish(s, r, MDMC_ERR_HIP);
-        retire_batch(s, lane);
-        progress = true;
-        continue;
-      }
-      for (size_t i = 0; i < b->reqs.size(); ++i) b->reqs[i]->counts = lane.h_counts[i];
-      if (b->phase == 1) {
-        for (Req *r : b->reqs)
-          if (r->counts.status) r->status = MDMC_ERR_INTERNAL;
-        b->phase = 2;
-      } else {
-        uint64_t fb = 0, ob = 0;
-        for (Req *r : b->reqs) {
-          if (r->counts.status && r->status == MDMC_OK) r->status = MDMC_ERR_INTERNAL;
-          fb += field_bytes(r->spec);
-          ob += 12ull * r->counts.num_vertices + 16ull * r->counts.num_quads;
-          finish(s, r, MDMC_OK);
-        }
-        TRACE("batch of %zu done", b->reqs.size());
-        if (fb) s->out_ratio = std::max(0.05, 0.75 * s->out_ratio + 0.25 * 1.5 * (double)ob / (double)fb);
-        retire_batch(s, lane);
-        progress = true;
-        continue;
-      }
-      progress = true;
+    const hipError_t q = hipEventQuery(lane.ev);
+    if (q == hipErrorNotReady) continue;
+    progress = true;
+    if (q != hipSuccess) {
+      (void)hipGetLastError();
+      fail_batch(s, lane, MDMC_ERR_HIP);
+      continue;
     }
-    if (b->phase == 2) {
-      if (start_emit(s, lane)) {
-        progress = true;
-      } else if (s->held == 0 && std::all_of(s->lanes.begin(), s->lanes.end(), [](const Lane &l) {
-                   return !l.batch || l.batch->phase == 2;
-                 })) {
-        // Nothing in flight or held by unreleased results will free memory:
-        // the outputs cannot fit the budget.
-        for (Req *r : b->reqs) finish(s, r, MDMC_ERR_REJECTED);
-        retire_batch(s, lane);
-        progress = true;
+    for (size_t i = 0; i < b->reqs.size(); ++i) {
+      Req *r = b->reqs[i];
+      r->counts = lane.h_counts[i];
+      if (r->counts.status) r->status = MDMC_ERR_INTERNAL;
+      if (r->status == MDMC_OK) {
+        // keep exactly the mesh the count gave
+        pool_trim(s, &r->vert_off, 12ull * r->counts.num_vertices);
+        pool_trim(s, &r->quad_off, 16ull * r->counts.num_quads);
+        if (r->vert_off == kNoOffset) r->verts = nullptr;
+        if (r->quad_off == kNoOffset) r->quads = nullptr;
       }
+      finish(s, r, MDMC_OK);
     }
+    TRACE("batch of %zu done", b->reqs.size());
+    retire_batch(s, lane);
   }
   // an idle lane takes whatever is queued at once: holding a lane back for a
   // fuller batch lowered neither the GPU time per request nor the tail
@@ -413,7 +406,7 @@ void worker_loop(mdmc_server *s) {
     while (pump(s)) moved = true;
     watch.clear();
     for (const Lane &l : s->lanes)
-      if (l.batch && (l.batch->phase == 1 || l.batch->phase == 3)) watch.push_back(l.ev);
+      if (l.batch) watch.push_back(l.ev);
     if (moved) s->cv_done.notify_all();
     if (watch.empty()) {
       s->idle = true;
@@ -536,8 +529,9 @@ int mdmc_server_submit(mdmc_server *s, const mdmc_request *req, uint64_t *id) {
   r->t_submit = s->now();
   const uint64_t fb = field_bytes(*req);
   r->scratch_est = scratch_estimate(*req);
-  r->out_est = (uint64_t)(s->out_ratio * (double)fb);
-  r->footprint = (req->field_kind != MDMC_FIELD_DEVICE ? fb : 0) + r->scratch_est + r->out_est;
+  worst_case_mesh(req->dims, &r->vmax, &r->qmax);
+  r->footprint = (req->field_kind != MDMC_FIELD_DEVICE ? round_up(fb) : 0) + round_up(r->scratch_est) +
+                 round_up(12 * r->vmax) + round_up(16 * r->qmax);
   s->reqs[r->id] = r;
   s->submitted += 1;
   *id = r->id;
@@ -611,8 +605,6 @@ int mdmc_server_release(mdmc_server *s, uint64_t id) {
   auto it = s->reqs.find(id);
   if (it == s->reqs.end() || it->second->state < 2) return MDMC_ERR_INVALID;
   Req *r = it->second;
-  if (r->vert_off != kNoOffset || r->quad_off != kNoOffset)
-    s->held -= std::min<uint64_t>(s->held, 12ull * r->counts.num_vertices + 16ull * r->counts.num_quads);
   pool_free(s, &r->vert_off);
   pool_free(s, &r->quad_off);
   if (r->state == 2) {  // still in the finished list: drop it there too

@@ -13,7 +13,7 @@ import time
 
 import numpy as np
 
-from .mdmc import MDMCError, _check, _default_box, lib
+from .mdmc import MDMCError, _check, _default_box, _DeviceArray, lib
 
 _VP = ctypes.c_void_p
 
@@ -137,7 +137,8 @@ class Server:
             r = self._buf[i]
             out.append({"id": r.id, "tag": r.tag, "status": r.status, "num_active": r.counts.num_active,
                         "num_vertices": r.counts.num_vertices, "num_quads": r.counts.num_quads,
-                        "t_submit": r.t_submit, "t_start": r.t_start, "t_done": r.t_done})
+                        "t_submit": r.t_submit, "t_start": r.t_start, "t_done": r.t_done,
+                        "vertices_ptr": r.vertices or 0, "quads_ptr": r.quads or 0})
         return out
 
     def copy_result(self, res: dict):
@@ -146,6 +147,18 @@ class Server:
         q = np.empty((Q, 4), np.int32)
         _check(lib().mdmc_server_copy_result(self._h, res["id"], _VP(v.ctypes.data if V else None),
                                              _VP(q.ctypes.data if Q else None)), "mdmc_server_copy_result")
+        return v, q
+
+    def result_tensors(self, res: dict):
+        """(vertices (V,3), quads (Q,4)) torch tensors viewing the result's
+        device memory in place (valid until the result is released)."""
+        import torch
+
+        V, Q = res["num_vertices"], res["num_quads"]
+        v = (torch.as_tensor(_DeviceArray(res["vertices_ptr"], (V, 3), "<f4"), device="cuda") if V
+             else torch.zeros((0, 3), dtype=torch.float32, device="cuda"))
+        q = (torch.as_tensor(_DeviceArray(res["quads_ptr"], (Q, 4), "<i4"), device="cuda") if Q
+             else torch.zeros((0, 4), dtype=torch.int32, device="cuda"))
         return v, q
 
     def release(self, rid: int) -> None:
@@ -166,11 +179,13 @@ class Server:
             _check(lib().mdmc_server_set_option(self._h, int(opt), int(val)), "mdmc_server_set_option")
 
 
-def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set | None = None) -> dict:
+def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set | None = None,
+               on_result=None) -> dict:
     """Submits requests[i] at arrivals[i] seconds (server clock, from now) and
     collects results until all have finished.  Results whose index is in `keep`
-    are copied to the host before release.  Returns latencies, timings and
-    the kept meshes."""
+    are copied to the host before release; on_result(i, result) is called for
+    every result before its release.  Returns latencies, timings and the kept
+    meshes."""
     n = len(requests)
     t0 = server.now()
     done, lat, kept, status = 0, np.zeros(n), {}, np.zeros(n, np.int32)
@@ -194,6 +209,8 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set |
             status[k] = r["status"]
             if keep is not None and k in keep and r["status"] == 0:
                 kept[k] = server.copy_result(r)
+            if on_result is not None:
+                on_result(k, r)
             server.release(r["id"])
             done += 1
     t_end = server.now() - t0

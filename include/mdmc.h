@@ -75,6 +75,46 @@ int mdmc_setup(void);
  * releases the library's cached device state.  Idempotent. */
 int mdmc_cleanup(void);
 
+/* Device copies of the case tables in the reference's array formats, as the
+ * reference's 11-pointer setup hands them to its caller (the tables of
+ * dmc.cu:45-661).  All entries are bytes. */
+typedef struct mdmc_device_luts {
+  uint8_t *config_edge_lut1;            /* [256][12] edge -> iso-vertex label, 0xff none */
+  uint8_t *config_edge_lut2;            /* [256][12] the 2B/3B variant */
+  uint8_t *num_vertex_lut1;             /* [256] */
+  uint8_t *num_vertex_lut2;             /* [256] */
+  uint8_t *config_2B_3B_lut;            /* [36] the 2B/3B configurations, ascending */
+  uint8_t *config_2B_3B_ambiguous_face; /* [36] their ambiguous face, same order */
+  uint8_t *opposite_face_lut;           /* [6] */
+  uint8_t *face_to_check_dir_lut;       /* [6] POS_X=0, NEG_X, POS_Y, NEG_Y, POS_Z, NEG_Z=5 */
+  uint8_t *edge_belonged_voxel_lut;     /* [12] 0x80|0x40|0x20 = owner at -x|-y|-z, low bits
+                                           = owner's edge; 0xff = the cell's own edge */
+  uint8_t *circular_edge_lut;           /* [3][4] ring of edges 6, 9, 10 (ccw) */
+  uint8_t *voxel_local_edges;           /* [3] 6, 9, 10 */
+} mdmc_device_luts;
+
+/* Replaces the 11-pointer dmc::setup_device_luts(iso_vertex_m_type**, ...)
+ * (include/dmc.hpp:42-46, dmc.cu:721-779): mdmc_setup() plus device copies
+ * of the tables above on the current device, owned by the library (the same
+ * pointers on every call until cleanup).  The extraction itself does not read
+ * these copies (it uses its own constant-memory forms); they are there for
+ * callers that read the tables.  One difference: the 2B/3B list is in
+ * ascending configuration order (the reference lists 2B then 3B cases). */
+int mdmc_setup_device_luts(mdmc_device_luts *out);
+
+/* Replaces the 11-pointer dmc::cleanup_device_luts(iso_vertex_m_type*, ...)
+ * (include/dmc.hpp:50-54, dmc.cu:793-811): frees the table copies and runs
+ * mdmc_cleanup().  MDMC_ERR_INVALID, freeing nothing, when a pointer is not
+ * the one mdmc_setup_device_luts handed out on this device. */
+int mdmc_cleanup_device_luts(const mdmc_device_luts *luts);
+
+/* The same tables in host memory (no device call), one block of
+ * MDMC_REF_LUTS_BYTES: the fields of mdmc_device_luts in declaration order at
+ * byte offsets 0, 3072, 6144, 6400, 6656, 6692, 6728, 6734, 6740, 6752, 6764
+ * (the device copies have the same layout). */
+#define MDMC_REF_LUTS_BYTES 6784
+int mdmc_reference_luts_host(uint8_t *out);
+
 /* Replaces dmc::run_dmc(vertices, triangles, grid, xyz_min, xyz_max, iso,
  * num_smooth) (include/dmc.hpp:56-57, dmc.cu:2069-2208).  `field` is host
  * memory, dims[3] are point counts (x fastest).  num_smooth smoothing

@@ -68,6 +68,9 @@ def lib():
         L.oracle_dmc_run_smooth.argtypes = [f3, u3, f3, f3, ctypes.c_float, ctypes.c_int, ctypes.c_uint32,
                                              ctypes.POINTER(_Result)]
         L.oracle_dmc_run_smooth.restype = ctypes.c_int
+        L.oracle_dmc_run_window.argtypes = [f3, u3, u3, u3, f3, f3, ctypes.c_float, ctypes.c_int, ctypes.c_uint32,
+                                            ctypes.POINTER(_Result)]
+        L.oracle_dmc_run_window.restype = ctypes.c_int
         L.oracle_result_free.argtypes = [ctypes.POINTER(_Result)]
         L.oracle_luts_bytes.argtypes = [ctypes.POINTER(ctypes.c_uint8)]
         L.oracle_field_sphere.argtypes = [f3, u3, ctypes.POINTER(ctypes.c_double), ctypes.c_double]
@@ -100,22 +103,33 @@ def luts_bytes() -> bytes:
     return bytes(buf)
 
 
-def run(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, fp64: bool = False, num_smooth: int = 0) -> dict:
+def run(field: np.ndarray, iso: float, xyz_min=None, xyz_max=None, fp64: bool = False, num_smooth: int = 0,
+        grid_cells=None, offset=None) -> dict:
     """Extract the mesh of ``field`` (shape (Z, Y, X), float32, x fastest),
     with ``num_smooth`` smoothing iterations.
 
-    Defaults put vertices in voxel units (min 0, max = cells per axis)."""
+    Defaults put vertices in voxel units (min 0, max = cells per axis).  With
+    ``grid_cells`` (x, y, z) the field is a window of a grid of that many
+    cells, starting at point ``offset`` (x, y, z) of it: coordinates are the
+    whole grid's (default box: that grid's voxel units)."""
     field = np.ascontiguousarray(field, dtype=np.float32)
     Z, Y, X = field.shape
     if xyz_min is None:
         xyz_min = (0.0, 0.0, 0.0)
     if xyz_max is None:
-        xyz_max = (float(max(X - 1, 1)), float(max(Y - 1, 1)), float(max(Z - 1, 1)))
+        xyz_max = ((float(c) for c in grid_cells) if grid_cells is not None else
+                   (float(max(X - 1, 1)), float(max(Y - 1, 1)), float(max(Z - 1, 1))))
     mn = np.asarray(xyz_min, dtype=np.float32)
-    mx = np.asarray(xyz_max, dtype=np.float32)
+    mx = np.asarray(tuple(xyz_max), dtype=np.float32)
     r = _Result()
-    rc = lib().oracle_dmc_run_smooth(_fptr(field), _dims((X, Y, Z)), _fptr(mn), _fptr(mx),
-                                     ctypes.c_float(iso), int(fp64), int(num_smooth), ctypes.byref(r))
+    if grid_cells is None and offset is None:
+        rc = lib().oracle_dmc_run_smooth(_fptr(field), _dims((X, Y, Z)), _fptr(mn), _fptr(mx),
+                                         ctypes.c_float(iso), int(fp64), int(num_smooth), ctypes.byref(r))
+    else:
+        gc = grid_cells if grid_cells is not None else (X - 1, Y - 1, Z - 1)
+        of = offset if offset is not None else (0, 0, 0)
+        rc = lib().oracle_dmc_run_window(_fptr(field), _dims((X, Y, Z)), _dims(gc), _dims(of), _fptr(mn), _fptr(mx),
+                                         ctypes.c_float(iso), int(fp64), int(num_smooth), ctypes.byref(r))
     if rc != 0:
         raise MemoryError("oracle_dmc_run failed")
     try:

@@ -269,7 +269,8 @@ static int inside_tri(pt2 p0, pt2 p1, pt2 p2, pt2 t, float *al, float *be, float
 }
 
 static void smooth_pass(oracle_result *out, const cellrec *rec, uint32_t A, uint32_t cx, uint32_t cy,
-                        uint32_t cz, const mdmc_luts *L, const float xyz_min[3], const float range[3]) {
+                        uint32_t cz, const mdmc_luts *L, const float xyz_min[3], const float range[3],
+                        const uint32_t div[3], const uint32_t off[3]) {
   uint32_t a;
   float *V = out->vertices;
   for (a = 0; a < A; ++a) {
@@ -310,8 +311,10 @@ static void smooth_pass(oracle_result *out, const cellrec *rec, uint32_t A, uint
         s0 = npos < nneg ? ipos : ineg; /* the concave corner */
       }
       s1 = (s0 + 2) % 4;
-      o.x = ijk_to_xyz(l == 2 ? j + 1 : i + 1, l == 2 ? cy : cx, range[c0], xyz_min[c0]);
-      o.y = l == 0 ? ijk_to_xyz(j + 1, cy, range[1], xyz_min[1]) : ijk_to_xyz(k + 1, cz, range[2], xyz_min[2]);
+      o.x = l == 2 ? ijk_to_xyz(off[1] + j + 1, div[1], range[1], xyz_min[1])
+                   : ijk_to_xyz(off[0] + i + 1, div[0], range[0], xyz_min[0]);
+      o.y = l == 0 ? ijk_to_xyz(off[1] + j + 1, div[1], range[1], xyz_min[1])
+                   : ijk_to_xyz(off[2] + k + 1, div[2], range[2], xyz_min[2]);
       E = V + 3 * (size_t)edge_vertex_index(&rec[a], L, e);
       if (inside_tri(P[s0], P[(s0 + 1) % 4], P[s1], o, &al, &be, &ga)) {
         const float *va = V + 3 * (size_t)iv[s0], *vb = V + 3 * (size_t)iv[(s0 + 1) % 4], *vc = V + 3 * (size_t)iv[s1];
@@ -329,6 +332,12 @@ static void smooth_pass(oracle_result *out, const cellrec *rec, uint32_t A, uint
 int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const float xyz_min[3],
                           const float xyz_max[3], float iso, int want_fp64, uint32_t num_smooth,
                           oracle_result *out) {
+  return oracle_dmc_run_window(field, dims, NULL, NULL, xyz_min, xyz_max, iso, want_fp64, num_smooth, out);
+}
+
+int oracle_dmc_run_window(const float *field, const uint32_t dims[3], const uint32_t grid_cells[3],
+                          const uint32_t offset[3], const float xyz_min[3], const float xyz_max[3], float iso,
+                          int want_fp64, uint32_t num_smooth, oracle_result *out) {
   const mdmc_luts *L = luts();
   uint32_t X = dims[0], Y = dims[1], Z = dims[2];
   uint32_t cx, cy, cz;
@@ -339,11 +348,20 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
   int64_t kk, aa;
   float range[3];
   double range64[3];
+  uint32_t div[3], off[3];
 
   memset(out, 0, sizeof(*out));
   if (X < 2 || Y < 2 || Z < 2) return 0; /* no cells: empty mesh */
   cx = X - 1; cy = Y - 1; cz = Z - 1;
   C = (size_t)cx * cy * cz;
+  /* coordinates: index + offset over the grid's cells per axis (a window of
+   * a larger grid takes that grid's divisors and its own position in it) */
+  div[0] = grid_cells ? grid_cells[0] : cx;
+  div[1] = grid_cells ? grid_cells[1] : cy;
+  div[2] = grid_cells ? grid_cells[2] : cz;
+  off[0] = offset ? offset[0] : 0;
+  off[1] = offset ? offset[1] : 0;
+  off[2] = offset ? offset[2] : 0;
   /* xyz_range = xyz_max - xyz_min in float (dmc.cu:2149) */
   range[0] = xyz_max[0] - xyz_min[0];
   range[1] = xyz_max[1] - xyz_min[1];
@@ -481,12 +499,12 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
     uint32_t id = rec[a].index1d;
     uint32_t i = id % cx, j = (id / cx) % cy, k = id / (cx * cy);
     uint32_t w = rec[a].vertex_begin + L->num_vertex[(rec[a].info >> 7) & 1][rec[a].cfg];
-    float x1 = ijk_to_xyz(i + 1, cx, range[0], xyz_min[0]);
-    float y1 = ijk_to_xyz(j + 1, cy, range[1], xyz_min[1]);
-    float z1 = ijk_to_xyz(k + 1, cz, range[2], xyz_min[2]);
-    double X1 = ijk_to_xyz64(i + 1, cx, range64[0], xyz_min[0]);
-    double Y1 = ijk_to_xyz64(j + 1, cy, range64[1], xyz_min[1]);
-    double Z1 = ijk_to_xyz64(k + 1, cz, range64[2], xyz_min[2]);
+    float x1 = ijk_to_xyz(off[0] + i + 1, div[0], range[0], xyz_min[0]);
+    float y1 = ijk_to_xyz(off[1] + j + 1, div[1], range[1], xyz_min[1]);
+    float z1 = ijk_to_xyz(off[2] + k + 1, div[2], range[2], xyz_min[2]);
+    double X1 = ijk_to_xyz64(off[0] + i + 1, div[0], range64[0], xyz_min[0]);
+    double Y1 = ijk_to_xyz64(off[1] + j + 1, div[1], range64[1], xyz_min[1]);
+    double Z1 = ijk_to_xyz64(off[2] + k + 1, div[2], range64[2], xyz_min[2]);
     float v6 = F(field, X, Y, i + 1, j + 1, k + 1);
     int l;
     for (l = 0; l < 3; ++l) {
@@ -494,22 +512,22 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
       double q0[3], q1[3], T, OMT;
       if (!((rec[a].info >> l) & 1)) continue;
       if (l == 0) { /* edge 6, pt 2 -> 6 */
-        float zc = ijk_to_xyz(k, cz, range[2], xyz_min[2]);
+        float zc = ijk_to_xyz(off[2] + k, div[2], range[2], xyz_min[2]);
         p0[0] = x1; p0[1] = y1; p0[2] = zc; p1[0] = x1; p1[1] = y1; p1[2] = z1;
-        q0[0] = X1; q0[1] = Y1; q0[2] = ijk_to_xyz64(k, cz, range64[2], xyz_min[2]);
+        q0[0] = X1; q0[1] = Y1; q0[2] = ijk_to_xyz64(off[2] + k, div[2], range64[2], xyz_min[2]);
         q1[0] = X1; q1[1] = Y1; q1[2] = Z1;
         va = F(field, X, Y, i + 1, j + 1, k); vb = v6;
       } else if (l == 1) { /* edge 9, pt 5 -> 6 */
-        float yc = ijk_to_xyz(j, cy, range[1], xyz_min[1]);
+        float yc = ijk_to_xyz(off[1] + j, div[1], range[1], xyz_min[1]);
         p0[0] = x1; p0[1] = yc; p0[2] = z1; p1[0] = x1; p1[1] = y1; p1[2] = z1;
-        q0[0] = X1; q0[1] = ijk_to_xyz64(j, cy, range64[1], xyz_min[1]); q0[2] = Z1;
+        q0[0] = X1; q0[1] = ijk_to_xyz64(off[1] + j, div[1], range64[1], xyz_min[1]); q0[2] = Z1;
         q1[0] = X1; q1[1] = Y1; q1[2] = Z1;
         va = F(field, X, Y, i + 1, j, k + 1); vb = v6;
       } else { /* edge 10, pt 6 -> 7 */
-        float xc = ijk_to_xyz(i, cx, range[0], xyz_min[0]);
+        float xc = ijk_to_xyz(off[0] + i, div[0], range[0], xyz_min[0]);
         p0[0] = x1; p0[1] = y1; p0[2] = z1; p1[0] = xc; p1[1] = y1; p1[2] = z1;
         q0[0] = X1; q0[1] = Y1; q0[2] = Z1;
-        q1[0] = ijk_to_xyz64(i, cx, range64[0], xyz_min[0]); q1[1] = Y1; q1[2] = Z1;
+        q1[0] = ijk_to_xyz64(off[0] + i, div[0], range64[0], xyz_min[0]); q1[1] = Y1; q1[2] = Z1;
         va = v6; vb = F(field, X, Y, i, j + 1, k + 1);
       }
       t = (iso - va) / (vb - va);
@@ -592,7 +610,7 @@ int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const floa
   {
     uint32_t it;
     for (it = 0; it < num_smooth; ++it) {
-      smooth_pass(out, rec, A, cx, cy, cz, L, xyz_min, range);
+      smooth_pass(out, rec, A, cx, cy, cz, L, xyz_min, range, div, off);
       reaverage(out, rec, map, A, cx, cy, cz, L, 0, 0);
     }
   }

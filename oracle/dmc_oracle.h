@@ -44,6 +44,15 @@ int oracle_dmc_run(const float *field, const uint32_t dims[3], const float xyz_m
 int oracle_dmc_run_smooth(const float *field, const uint32_t dims[3], const float xyz_min[3],
                           const float xyz_max[3], float iso, int want_fp64, uint32_t num_smooth,
                           oracle_result *out);
+/* A window of a larger grid: `field`/`dims` hold the window, which starts at
+ * point `offset` of a grid of `grid_cells` cells per axis; coordinates are
+ * those of the whole grid, ijk_to_xyz(offset + i, grid_cells, range, min).
+ * Configurations, variants and vertices of cells off the window's faces equal
+ * the whole grid's (cells on a face may see fewer neighbours).  NULL
+ * grid_cells / offset: the window is the grid. */
+int oracle_dmc_run_window(const float *field, const uint32_t dims[3], const uint32_t grid_cells[3],
+                          const uint32_t offset[3], const float xyz_min[3], const float xyz_max[3], float iso,
+                          int want_fp64, uint32_t num_smooth, oracle_result *out);
 void oracle_result_free(oracle_result *r);
 /* Threads for the per-cell loops that are independent (the compaction and
  * prefix sums stay in cell order, so results do not depend on it); 1 by

@@ -41,12 +41,23 @@ int main() {
         }
         if (best >= len) return fail("missed fit", best, len);
       }
-    } else {
+    } else if ((rng() % 100) < 60) {
       auto it = live.begin();
       std::advance(it, rng() % live.size());
       if (!pool.release(it->first)) return fail("release", it->first);
       for (uint64_t g = it->first / 256; g < (it->first + it->second) / 256; ++g) owner[g] = 0;
       live.erase(it);
+    } else {
+      // cut a block down to a random prefix (possibly all or nothing of it)
+      auto it = live.begin();
+      std::advance(it, rng() % live.size());
+      const uint64_t off = it->first, len = it->second;
+      const uint64_t want = rng() % (len + 1);
+      const uint64_t keep = (want + 255) / 256 * 256;
+      if (!pool.shrink(off, want)) return fail("shrink", off, want);
+      for (uint64_t g = (off + keep) / 256; g < (off + len) / 256; ++g) owner[g] = 0;
+      if (keep == 0) live.erase(it); else it->second = keep;
+      if (pool.shrink(off, keep + 256)) return fail("grow accepted", off, keep);
     }
     uint64_t used = 0;
     for (auto &kv : live) used += kv.second;
@@ -55,6 +66,7 @@ int main() {
   for (auto &kv : live) pool.release(kv.first);
   if (pool.in_use() != 0 || pool.free_blocks() != 1 || pool.largest_free() != size) return fail("coalesce");
   if (pool.release(12345)) return fail("unknown offset accepted");
+  if (pool.shrink(12345, 0)) return fail("unknown offset shrunk");
   std::printf("OK\n");
   return 0;
 }

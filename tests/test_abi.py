@@ -101,3 +101,87 @@ def test_obj_writer_rejects_out_of_range_quads(tmp_path):
         assert not out.exists()
     fa.write_obj(str(tmp_path / "ok.obj"), v, np.array([[0, 1, 2, 3]], np.int32))
     assert (tmp_path / "ok.obj").read_text().count("\nf ") == 2
+
+
+def test_shim_utilities(tmp_path):
+    """include/dmc.hpp's constants, typedefs, Array3D iterators, index
+    helpers and argmin/argmax (tests/golden/shim_utils_check.cpp)."""
+    src = os.path.join(ROOT, "tests", "golden", "shim_utils_check.cpp")
+    # with the header's own vector types, and with HIP's (a caller that
+    # includes HIP's headers, as the reference's callers include CUDA's)
+    for i, extra in enumerate(([], ["-D__HIP_PLATFORM_AMD__", "-isystem", "/opt/rocm/include"])):
+        exe = tmp_path / f"shim_utils_check{i}"
+        subprocess.run(["g++", "-std=c++14", "-O1", "-Wall", "-Werror", *extra, "-I", os.path.join(ROOT, "include"),
+                        src, "-o", str(exe)], check=True)
+        out = subprocess.run([str(exe)], capture_output=True, text=True)
+        assert out.returncode == 0 and out.stdout.strip() == "OK", out.stdout + out.stderr
+
+
+def test_harness_builds(tmp_path):
+    """The test_dmc-style program (11-pointer setup, run_dmc, dump_obj,
+    11-pointer cleanup) compiles against the shim and links (run on the GPU
+    by tests/test_gpu_dropin.py)."""
+    exe = tmp_path / "dmc_harness"
+    subprocess.run(harness_build_command(str(exe)), check=True)
+    assert exe.exists()
+
+
+def harness_build_command(exe: str) -> list:
+    libdir = os.path.dirname(fa.mdmc.LIB_PATH)
+    return ["g++", "-std=c++14", "-O2", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+            "-isystem", "/opt/rocm/include", os.path.join(ROOT, "tests", "golden", "dmc_harness.cpp"), "-L", libdir, "-lmdmc",
+            "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath," + libdir, "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+
+
+# The reference's small tables (DMC_CUDA/DMC_CUDA/src/dmc.cu:622-661), as data.
+REF_OPPOSITE_FACE = [5, 3, 4, 1, 2, 0]                       # :622
+REF_FACE_TO_CHECK_DIR = [5, 3, 0, 2, 1, 4]                   # :631-632 (NEG_Z, NEG_Y, POS_X, POS_Y, NEG_X, POS_Z)
+REF_EDGE_BELONGED_VOXEL = [0x60 | 10, 0x20 | 9, 0x20 | 10, 0xA0 | 9, 0xC0 | 6, 0x40 | 6, 0xFF, 0x80 | 6,
+                           0x40 | 10, 0xFF, 0xFF, 0x80 | 9]  # :635-649
+REF_CIRCULAR_EDGE = [6, 7, 4, 5, 9, 1, 3, 11, 10, 8, 0, 2]   # :653-658
+REF_LOCAL_EDGES = [6, 9, 10]                                 # :661
+OFFSETS = [0, 3072, 6144, 6400, 6656, 6692, 6728, 6734, 6740, 6752, 6764, 6784]
+
+
+def reference_tables_host() -> list:
+    """mdmc_reference_luts_host split into the 11 tables of mdmc_device_luts."""
+    buf = (ctypes.c_uint8 * 6784)()
+    assert fa.lib().mdmc_reference_luts_host(buf) == 0
+    b = bytes(buf)
+    return [b[OFFSETS[i]:OFFSETS[i + 1]] for i in range(11)]
+
+
+def check_reference_tables(t: list) -> None:
+    """The 11 tables of the 11-pointer setup against the reference: SHA-256
+    pins of its case tables, its small tables as data."""
+    import hashlib
+
+    import numpy as np
+
+    from tests.test_luts import PINS
+
+    def sha(x):
+        return hashlib.sha256(bytes(x)).hexdigest()
+
+    assert sha(t[0]) == PINS["edge_label_v0"] and sha(t[1]) == PINS["edge_label_v1"]
+    assert sha(t[2]) == PINS["num_vertex_v0"] and sha(t[3]) == PINS["num_vertex_v1"]
+    amb = np.frombuffer(t[4], np.uint8)
+    assert sha(t[4]) == PINS["amb_sorted"] and sha(t[5]) == PINS["amb_faces_sorted"]
+    assert (np.diff(amb.astype(int)) > 0).all()
+    assert list(t[6]) == REF_OPPOSITE_FACE
+    assert list(t[7]) == REF_FACE_TO_CHECK_DIR
+    assert list(t[8]) == REF_EDGE_BELONGED_VOXEL
+    assert list(t[9]) == REF_CIRCULAR_EDGE
+    assert list(t[10][:3]) == REF_LOCAL_EDGES
+
+
+def test_reference_format_tables_on_host():
+    check_reference_tables(reference_tables_host())
+
+
+def test_table_cleanup_rejects_foreign_pointers():
+    """The 11-pointer cleanup frees only what the setup handed out (CPU: no
+    table was handed out, so any pointers are foreign)."""
+    raw = ctypes.CDLL(fa.mdmc.LIB_PATH)
+    ptrs = (ctypes.c_void_p * 11)(*[0x1000 + i for i in range(11)])
+    assert raw.mdmc_cleanup_device_luts(ptrs) in (-1, -2)  # invalid, or no device to ask

@@ -189,6 +189,19 @@ def test_batch_graph_replay(fa_mod, torch_cuda):
             assert torch.equal(outs[i][1], q), (rep, i)
             assert torch.equal(outs[i][0].view(torch.int32), v.view(torch.int32)), (rep, i)
         _ = torch.ones(1000, device="cuda").sum().item()  # eager work between replays
+    # the captured context's buffers belong to the graph: eager calls on it
+    # are refused (they would rewrite its staging tables and scratch), and the
+    # graph still replays correctly afterwards
+    with pytest.raises(fa_mod.MDMCError, match="captured graph"):
+        ctx.batch_extract_into(fields, isos, outs, counts)
+    with pytest.raises(fa_mod.MDMCError, match="captured graph"):
+        ctx.count(fields[0], isos[0])
+    g.replay()
+    torch.cuda.synchronize()
+    c = counts.cpu().numpy().view(np.uint32)
+    for i, (v, q, cnt) in enumerate(exact):
+        assert c[i, 1] == cnt["num_vertices"] and c[i, 3] == 0, i
+        assert torch.equal(outs[i][1], q), i
     ctx.close()
 
 
@@ -228,3 +241,76 @@ def test_batch_generator_matches_single_generators(fa_mod, torch_cuda):
             torch.cuda.synchronize()
             assert torch.equal(o.view(torch.int32), ref.view(torch.int32)), (rep, sp)
     ctx.close()
+
+
+def _regenerate_cpu(spec):
+    """The CPU generator's field for a batch spec (same definition, same bits)."""
+    if spec["kind"] == "blobs":
+        return oracle.field_blobs(spec["dims"], spec["seed"], spec["nblobs"], spec.get("sigma", (8.0, 32.0)),
+                                  spec.get("amp", (-1.0, 1.0)))
+    return oracle.field_gyroid(spec["dims"], spec.get("period", 24.0), spec["seed"], spec.get("noise_amp", 0.05))
+
+
+def test_batched_workload_at_size(fa_mod, torch_cuda):
+    """The batched workload itself (256 volumes, axes log-uniform in
+    [32, 256], blob sums and gyroid+noise alternating, isovalues U[-0.2, 0.2];
+    seed 1 as in the benchmark): every volume of the eager batch equals its
+    single extraction bitwise and the closed-form active-cell and quad counts
+    of its field; three HIP-graph replays equal the eager batch; 8 volumes
+    equal the oracle on fields regenerated on the CPU."""
+    import bench
+    from tests.test_gpu_parity import _closed_form_counts
+
+    torch = torch_cuda
+    specs, isos = bench.batched_workload(256, seed=1)
+    ctx, sctx = fa_mod.Context(), fa_mod.Context()
+    fields = ctx.batch_generate(specs)
+    torch.cuda.synchronize()
+    res = ctx.batch_extract(fields, isos)
+    for i, (f, iso) in enumerate(zip(fields, isos)):
+        v, q, c = sctx.extract(f, iso)
+        bv, bq, bc = res[i]
+        assert (bc["num_active"], bc["num_vertices"], bc["num_quads"]) == \
+            (c["num_active"], c["num_vertices"], c["num_quads"]), i
+        assert torch.equal(bq, q) and torch.equal(bv.view(torch.int32), v.view(torch.int32)), i
+        A, Q = _closed_form_counts(torch, f, iso)
+        assert (c["num_active"], c["num_quads"]) == (A, Q), (i, specs[i]["dims"])
+    print(f"batch of {len(fields)}: {sum(r[2]['num_vertices'] for r in res)} vertices", flush=True)
+    # graph replays against the eager batch
+    outs = [(torch.empty_like(v), torch.empty_like(q)) for v, q, _ in res]
+    counts = ctx.batch_counts_tensor(len(fields))
+    gctx = fa_mod.Context()  # a context of its own for the graph
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        gctx.batch_extract_into(fields, isos, outs, counts, stream=stream)
+    stream.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        gctx.batch_extract_into(fields, isos, outs, counts, stream=torch.cuda.current_stream())
+    for rep in range(3):
+        for v, q in outs:
+            v.fill_(-1.0)
+            q.fill_(-1)
+        counts.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        c = counts.cpu().numpy().view(np.uint32)
+        assert (c[:, 3] == 0).all(), rep
+        for i, (v, q, cnt) in enumerate(res):
+            assert c[i, 1] == cnt["num_vertices"] and c[i, 2] == cnt["num_quads"], (rep, i)
+            assert torch.equal(outs[i][1], q) and torch.equal(outs[i][0].view(torch.int32), v.view(torch.int32)), \
+                (rep, i)
+    # the oracle on 8 volumes, both field kinds, small and large
+    sizes = np.array([np.prod(s["dims"]) for s in specs])
+    pick = list(np.argsort(sizes)[[0, 1, 2, 3, 128, 129, 254, 255]])
+    for i in pick:
+        fnp = _regenerate_cpu(specs[i])
+        assert np.array_equal(fnp.view(np.uint32), fields[i].cpu().numpy().view(np.uint32)), i
+        ref = oracle.run(fnp, isos[i])
+        v, q, _ = res[i]
+        assert_same_mesh(v.cpu().numpy(), q.cpu().numpy(), ref)
+    del g
+    ctx.close()
+    sctx.close()
+    gctx.close()

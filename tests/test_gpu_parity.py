@@ -197,6 +197,10 @@ def test_world_box_through_both_coordinate_paths(ctx, torch_cuda):
     ctx.emit(ft, verts, quads, xyz_min=boxes[2][0], xyz_max=boxes[2][1])
     torch.cuda.synchronize()
     assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), ref)
+    # ... and back to the count's box: the tables must be rewritten again
+    ctx.emit(ft, verts, quads, xyz_min=mn, xyz_max=mx)
+    torch.cuda.synchronize()
+    assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), oracle.run(f, 0.0, mn, mx))
 
 
 def test_field_generators_match_cpu(ctx, torch_cuda):
@@ -226,7 +230,11 @@ def test_host_entry_matches_device(ctx, torch_cuda):
     ref = oracle.run(f, 0.1)
     assert_same_mesh(v, q, ref)
     verts, tris = fa.run_dmc(f, (0, 0, 0), (56, 60, 48), 0.1)
-    assert tris.shape == (2 * ref["num_quads"], 3)
+    assert tris.shape == (2 * ref["num_quads"], 3) and tris.dtype == np.uint32
+    # triangles (q0,q1,q2),(q2,q3,q0) per quad (dmc.cu:2039-2045), as values
+    q = ref["quads"].astype(np.uint32)
+    assert np.array_equal(tris[0::2], q[:, [0, 1, 2]]) and np.array_equal(tris[1::2], q[:, [2, 3, 0]])
+    assert np.array_equal(verts.view(np.uint32), ref["vertices"].view(np.uint32))
 
 
 def test_device_entry_with_library_mesh(ctx, torch_cuda):
@@ -311,8 +319,8 @@ def test_gyroid_1024_surface_dense(ctx, torch_cuda):
     Too large for the oracle, so: active-cell and quad counts against closed
     forms from the field, quad indices valid and distinct, vertices in the
     box, a repeat run bitwise equal, and an 80^3 corner window against the
-    oracle cell by cell (configuration, variant, vertex count and positions;
-    the window's coordinate division differs, hence a 1e-3 voxel tolerance)."""
+    oracle cell by cell (configuration, variant, vertex count and vertex bits;
+    the oracle takes the whole grid's coordinate divisors for the window)."""
     torch = torch_cuda
     n = 1024
     f = ctx.field_gyroid((n, n, n), period=24.0, seed=0, noise_amp=0.05)
@@ -345,9 +353,10 @@ def test_gyroid_1024_surface_dense(ctx, torch_cuda):
 def _window_check(torch, cells, verts, V, n, W, win):
     """Cells of an n^3 grid's extraction (tensors of ctx.cells(host=False)
     and the vertex tensor) against the oracle on the grid's W^3 corner
-    window, for cells at least two cells from the window's upper faces."""
+    window (with the grid's coordinate divisors), for cells at least three
+    cells from the window's upper faces."""
     dev = verts.device
-    ref = oracle.run(win, 0.0)
+    ref = oracle.run(win, 0.0, grid_cells=(n - 1, n - 1, n - 1), offset=(0, 0, 0))
     c = n - 1
     ci = cells["cell_index"].to(torch.int64) & 0xFFFFFFFF
     x, y, z = ci % c, (ci // c) % c, ci // (c * c)
@@ -375,6 +384,26 @@ def _window_check(torch, cells, verts, V, n, W, win):
     gidx = np.concatenate([np.arange(b, b + k) for b, k in zip(g_vb[og], g_nv[og])])
     g_pos = verts[torch.from_numpy(gidx).to(dev)].cpu().numpy()
     r_pos = np.concatenate([ref["vertices"][b:b + k] for b, k in zip(r_vb[rin][orf], r_nv[rin][orf])])
-    assert np.array_equal(np.isfinite(g_pos), np.isfinite(r_pos))
-    both = np.isfinite(g_pos) & np.isfinite(r_pos)
-    assert np.abs(g_pos[both] - r_pos[both]).max() <= 1e-3
+    # the window ran with the whole grid's coordinate divisors: bit for bit
+    assert np.array_equal(g_pos.view(np.uint32), r_pos.view(np.uint32))
+
+
+def test_hand_derived_winding_on_gpu(ctx, torch_cuda):
+    """The hand-derived single-quad rings (tests/test_oracle.py) and the
+    outward winding on sphere distance fields, through the HIP path."""
+    from tests.test_oracle import WINDING_CASES, _one_edge_field, _signed_volume
+
+    for edge, (p, ring) in WINDING_CASES.items():
+        for inside in (-1.0, 1.0):
+            verts, quads, cnt = ctx.extract(_dev(torch_cuda, _one_edge_field(p, inside)), 0.0)
+            cells = ctx.cells(cnt["num_active"])
+            first = dict(zip(cells["cell_index"].tolist(), cells["vertex_begin"].tolist()))
+            m = [first[c] for c in ring]
+            want = m if inside < 0 else [m[0], m[3], m[2], m[1]]
+            assert quads.cpu().numpy().tolist() == [want], (edge, inside)
+    r = 21.7
+    f = oracle.field_sphere((64, 60, 62), (31.5, 29.0, 30.2), r)
+    for sign in (1.0, -1.0):
+        verts, quads, _ = ctx.extract(_dev(torch_cuda, sign * f), 0.0)
+        vol = _signed_volume(verts.cpu().numpy(), quads.cpu().numpy())
+        assert np.sign(vol) == sign and abs(abs(vol) / (4.0 / 3.0 * np.pi * r ** 3) - 1.0) < 0.02, (sign, vol)

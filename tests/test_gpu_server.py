@@ -180,3 +180,71 @@ def test_clients_on_several_threads(fa_mod, torch_cuda):
         vs, qs = _single(fa_mod, reqs[k])
         assert np.array_equal(qm, qs), k
         assert np.array_equal(vm.view(np.uint32), vs.view(np.uint32)), k
+
+
+def _regenerate_cpu(r):
+    import oracle
+
+    dims = (r.dims[0], r.dims[1], r.dims[2])
+    if r.field_kind == 1:
+        return oracle.field_blobs(dims, r.seed, r.nblobs, (r.sigma_lo, r.sigma_hi), (r.amp_lo, r.amp_hi))
+    return oracle.field_gyroid(dims, r.period, r.seed, r.noise_amp)
+
+
+def test_served_workload_stream(fa_mod, torch_cuda):
+    """The served workload's distribution at size: 240 requests of the
+    benchmark's stream (seed 5: cubic dims log-uniform in [48, 384], blob or
+    gyroid fields from per-request seeds, isovalues U[-0.2, 0.2]) as a Poisson
+    stream through a 64 GiB server.  Every result equals its single
+    extraction bitwise (compared on the device); 12 results, the smallest and
+    a spread of sizes up to the largest, equal the oracle on fields
+    regenerated on the CPU; none is rejected; the peak stays in the budget."""
+    import bench
+    import oracle
+    from framework_amd.server import Server, run_stream
+    from tests.helpers import assert_same_mesh
+
+    torch = torch_cuda
+    reqs = bench._served_requests(240, seed=5)
+    d = np.array([r.dims[0] for r in reqs])
+    order = np.argsort(d)
+    to_oracle = set(order[:8].tolist()) | set(order[[80, 160, 200, len(reqs) - 1]].tolist())
+    assert d.min() >= 48 and d.max() <= 384 and d.max() > 300
+    budget = 64 << 30
+    srv = Server(budget, max_batch=32, num_streams=4)
+    gen, sctx = fa_mod.Context(), fa_mod.Context()
+    oracle.set_threads(8)
+    seen = {"single": 0, "oracle": 0}
+
+    def check(k, res):
+        assert res["status"] == 0, (k, res["status"])
+        r = reqs[k]
+        X, Y, Z = r.dims[0], r.dims[1], r.dims[2]
+        if r.field_kind == 1:
+            f = gen.field_blobs((X, Y, Z), r.seed, r.nblobs, (r.sigma_lo, r.sigma_hi), (r.amp_lo, r.amp_hi))
+        else:
+            f = gen.field_gyroid((X, Y, Z), r.period, r.seed, r.noise_amp)
+        v, q, c = sctx.extract(f, r.iso, tuple(r.xyz_min), tuple(r.xyz_max))
+        sv, sq = srv.result_tensors(res)
+        assert (res["num_active"], res["num_vertices"], res["num_quads"]) == \
+            (c["num_active"], c["num_vertices"], c["num_quads"]), k
+        assert torch.equal(sq, q) and torch.equal(sv.view(torch.int32), v.view(torch.int32)), k
+        seen["single"] += 1
+        if k in to_oracle:
+            fnp = _regenerate_cpu(r)
+            assert np.array_equal(fnp.view(np.uint32), f.cpu().numpy().view(np.uint32)), k
+            ref = oracle.run(fnp, r.iso, tuple(r.xyz_min), tuple(r.xyz_max))
+            assert_same_mesh(sv.cpu().numpy(), sq.cpu().numpy(), ref)
+            seen["oracle"] += 1
+
+    arrivals = np.cumsum(np.random.default_rng(6).exponential(1.0 / 4000.0, len(reqs)))
+    res = run_stream(srv, reqs, arrivals, on_result=check)
+    st = srv.stats()
+    srv.close()
+    oracle.set_threads(1)
+    gen.close()
+    sctx.close()
+    print(f"served {len(reqs)}: batches {st['batches']} peak {st['memory_peak'] / 1e9:.2f} GB", flush=True)
+    assert (res["status"] == 0).all() and st["rejected"] == 0
+    assert seen == {"single": len(reqs), "oracle": len(to_oracle)}
+    assert 0 < st["memory_peak"] <= budget and st["completed"] == len(reqs)

@@ -302,3 +302,109 @@ def test_threads_do_not_change_results():
                     assert val == many[key], key
     finally:
         oracle.set_threads(1)
+
+
+def test_window_mode_reproduces_the_whole_grid():
+    """A window of a grid, extracted with the grid's coordinate divisors and
+    its own offset, gives each interior cell (one cell or more from the
+    window's lower faces, three or more from its upper faces) the whole grid's
+    configuration, variant, vertex count and vertex bits.  Pins the window
+    mode by the whole-grid extraction, not by its own formula."""
+    full = oracle.field_gyroid((61, 57, 53), 9.0, 21, 0.3)
+    Zf, Yf, Xf = full.shape
+    gc = (Xf - 1, Yf - 1, Zf - 1)
+    box = ((-3.0, 1.0, 2.0), (7.5, 4.0, 9.25))
+    ref = oracle.run(full, 0.02, *box)
+    o = (9, 5, 13)
+    W = (33, 29, 31)
+    win = np.ascontiguousarray(full[o[2]:o[2] + W[2], o[1]:o[1] + W[1], o[0]:o[0] + W[0]])
+    got = oracle.run(win, 0.02, *box, grid_cells=gc, offset=o)
+
+    def by_cell(r, dims, shift):
+        cx, cy = dims[0] - 1, dims[1] - 1
+        ci = r["cell_index"].astype(np.int64)
+        x, y, z = ci % cx + shift[0], (ci // cx) % cy + shift[1], ci // (cx * cy) + shift[2]
+        vb = r["vertex_begin"].astype(np.int64)
+        nv = np.append(vb[1:], r["num_vertices"]) - vb
+        return {(int(a), int(b), int(c)): (int(r["cfg"][n]), int(r["info"][n]) >> 7, r["vertices"][vb[n]:vb[n] + nv[n]])
+                for n, (a, b, c) in enumerate(zip(x, y, z))}
+
+    G = by_cell(ref, (Xf, Yf, Zf), (0, 0, 0))
+    Wc = by_cell(got, W, o)
+    inner = [c for c in Wc if all(o[d] + 1 <= c[d] <= o[d] + W[d] - 4 for d in range(3))]
+    assert len(inner) > 2000
+    for c in inner:
+        cfg, l2, v = Wc[c]
+        rcfg, rl2, rv = G[c]
+        assert (cfg, l2) == (rcfg, rl2), c
+        assert v.shape == rv.shape and np.array_equal(v.view(np.uint32), rv.view(np.uint32)), c
+    # and without a window the mode is the plain extraction
+    same = oracle.run(full, 0.02, *box, grid_cells=gc, offset=(0, 0, 0))
+    assert np.array_equal(same["quads"], ref["quads"])
+    assert np.array_equal(same["vertices"].view(np.uint32), ref["vertices"].view(np.uint32))
+
+
+# One crossing grid edge per case, derived by hand from the reference's ring
+# table (circular_edge_lut, dmc.cu:653-658: edge 6's ring is the cells at
+# (x, y) offsets (0,0), (1,0), (1,1), (0,1), edge 9's at (z, x) offsets and
+# edge 10's at (y, z) offsets in that order) and its orientation rule
+# (use_ccw = v[lower end of the edge] <= iso, dmc.cu:1199-1216; otherwise the
+# ring is taken 0, 3, 2, 1, dmc.cu:1634-1644).  On a 3^3-point grid with +1
+# everywhere and -1 at one point P, the four cells around P's one interior
+# edge are active with one iso vertex each; the owner (cell 0) also holds the
+# edge's crossing point, so the cells' first vertices are 0, 2, 3, 4 in cell
+# order, and the single quad is (owner, ring 1, ring 2, ring 3) by those
+# numbers.  Flipping the field's sign flips the orientation.
+WINDING_CASES = {
+    # edge: (point P as (x, y, z), ring cells in ccw order as cell indices)
+    6: ((1, 1, 0), [0, 1, 3, 2]),
+    9: ((1, 0, 1), [0, 4, 5, 1]),
+    10: ((0, 1, 1), [0, 2, 6, 4]),
+}
+
+
+def _one_edge_field(p, inside_value):
+    f = np.full((3, 3, 3), -inside_value, np.float32)
+    f[p[2], p[1], p[0]] = inside_value
+    return f
+
+
+@pytest.mark.parametrize("edge", [6, 9, 10])
+def test_hand_derived_ring_and_winding(edge):
+    p, ring = WINDING_CASES[edge]
+    for inside in (-1.0, 1.0):
+        ref = oracle.run(_one_edge_field(p, inside), 0.0)
+        assert ref["num_active"] == 4 and ref["num_vertices"] == 5 and ref["num_quads"] == 1
+        assert sorted(ref["cell_index"].tolist()) == sorted(ring)
+        first = dict(zip(ref["cell_index"].tolist(), ref["vertex_begin"].tolist()))
+        m = [first[c] for c in ring]  # the ring's iso vertices in ccw order
+        # v at the edge's lower end is the -1/+1 point: ccw when it is <= iso
+        want = m if inside < 0 else [m[0], m[3], m[2], m[1]]
+        assert ref["quads"].tolist() == [want], (edge, inside, ref["quads"])
+
+
+def _signed_volume(verts, quads):
+    """Volume enclosed by the triangles (q0,q1,q2),(q2,q3,q0) (dmc.cu:2039-2045),
+    by the divergence theorem: positive when they face outward (right-hand
+    normals away from the enclosed region)."""
+    v = verts.astype(np.float64)
+    t = np.concatenate([quads[:, [0, 1, 2]], quads[:, [2, 3, 0]]])
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0)
+
+
+def test_winding_faces_increasing_field():
+    """The rules above make every quad's right-hand normal point toward
+    increasing field values, i.e. out of the region below the isovalue: on a
+    signed distance field of a sphere the enclosed volume is positive and
+    close to 4/3 pi r^3; on the negated field (the sphere's outside is now
+    below the isovalue) it is negative.  A global flip, a misread of the
+    use_ccw comparison or of which edge end is tested fails this."""
+    r = 11.3
+    f = oracle.field_sphere((40, 36, 38), (19.2, 17.6, 18.4), r)
+    exact = 4.0 / 3.0 * np.pi * r ** 3
+    for sign in (1.0, -1.0):
+        ref = oracle.run(sign * f, 0.0)
+        vol = _signed_volume(ref["vertices"], ref["quads"])
+        assert abs(abs(vol) / exact - 1.0) < 0.02, (sign, vol, exact)
+        assert np.sign(vol) == sign, (sign, vol)

@@ -91,3 +91,42 @@ def test_out_of_core_repeated_calls_reuse_buffers():
         v0, q0 = ref[id(f)]
         assert np.array_equal(q, q0), (f.shape, S)
         assert np.array_equal(v.view(np.uint32), v0.view(np.uint32)), (f.shape, S)
+
+
+@pytest.mark.gpu
+def test_out_of_core_keeps_to_its_budget():
+    """The out-of-core entry has its own device buffers: after a call (they
+    stay allocated), the device memory it holds is at most the budget plus one
+    slab's mesh, also when an in-core extraction of the same grid ran just
+    before it and when a call with a larger budget came first."""
+    import torch
+
+    import framework_amd as fa
+
+    torch.cuda.set_device(0)
+    fa.setup()
+    f = oracle.field_gyroid((96, 80, 128), 12.0, 3, 0.3)
+    Z, Y, X = f.shape
+    v0, q0 = fa.run_host(f, 0.0)
+    mesh = 12 * len(v0) + 16 * len(q0)  # the whole mesh bounds one slab's mesh
+    slack = 8 << 20                      # allocation granularity, counts record
+    for S in (60, 7, 19):
+        budget = _budget_for(fa, (X, Y, Z), S)
+        fa.cleanup()
+        fa.setup()
+        fa.run_host(f, 0.0)              # the in-core entry's buffers exist ...
+        free0 = torch.cuda.mem_get_info()[0]
+        v, q = fa.run_out_of_core(f, 0.0, budget)
+        held = free0 - torch.cuda.mem_get_info()[0]
+        print(f"slab planes {S} budget {budget} held {held}", flush=True)
+        assert held <= budget + mesh + slack, (S, budget, held)
+        assert np.array_equal(q, q0) and np.array_equal(v.view(np.uint32), v0.view(np.uint32))
+    # a smaller budget right after a larger one gives the larger buffers back
+    big, small = _budget_for(fa, (X, Y, Z), 60), _budget_for(fa, (X, Y, Z), 5)
+    fa.cleanup()
+    fa.setup()
+    free0 = torch.cuda.mem_get_info()[0]
+    fa.run_out_of_core(f, 0.0, big)
+    fa.run_out_of_core(f, 0.0, small)
+    held = free0 - torch.cuda.mem_get_info()[0]
+    assert held <= small + mesh + slack, (big, small, held)
