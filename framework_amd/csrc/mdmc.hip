@@ -55,6 +55,12 @@ bool is_setup_on(int dev) {
 // device, mdmc_setup_device_luts), guarded by g_setup_mutex.
 void *g_ref_tables[64] = {};
 
+// The no-allocation entry's scratch on each device (the address of the
+// `scratch` array below, taken by mdmc_setup on that device) and the
+// device's compute units, guarded by g_setup_mutex.
+void *g_noalloc_ptr[64] = {};
+int g_noalloc_cus[64] = {};
+
 // Byte offsets of the tables in that block.
 enum : size_t {
   kTabLut1 = 0, kTabLut2 = 3072, kTabNv1 = 6144, kTabNv2 = 6400, kTabAmb = 6656, kTabAmbFace = 6692,
@@ -217,6 +223,13 @@ int plan_tiling(const uint32_t dims[3], Plan *p, const int *opt) {
 }
 
 }  // namespace mdmc
+
+// ---------------------------------------------------------------------------
+// The no-allocation entry's scratch: a fixed array in device memory, one copy
+// per GPU, held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is a stand-in, not a device address): it
+// uses the address hipGetSymbolAddress gives for each device (g_noalloc_ptr).
+static __device__ __align__(256) unsigned char scratch[MDMC_NOALLOC_SCRATCH_BYTES];
 
 // ---------------------------------------------------------------------------
 // Context and scratch
@@ -988,6 +1001,18 @@ int mdmc_setup(void) {
     count[c] = w;
   }
   HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_cfg_count), count, sizeof(count)));
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(MDMC_ERR_HIP, "no device address for the no-allocation scratch");
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+      (void)hipGetLastError();
+      cus = 256;
+    }
+    g_noalloc_cus[dev] = cus;
+    g_noalloc_ptr[dev] = ptr;
+  }
   g_setup_mask |= 1ull << dev;
   return MDMC_OK;
 }
@@ -1187,6 +1212,50 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
   if (rc != MDMC_OK) return rc;
   return mdmc_emit_device(ctx, d_field, xyz_min, xyz_max, d_vertices, vertex_capacity, d_quads,
                           quad_capacity, d_counts, stream);
+}
+
+int mdmc_extract_device_noalloc(const float *d_field, const uint32_t dims[3], const float xyz_min[3],
+                                const float xyz_max[3], float iso, float *d_vertices, uint64_t vertex_capacity,
+                                int32_t *d_quads, uint64_t quad_capacity, mdmc_counts *d_counts, void *stream) {
+  if (!dims || !xyz_min || !xyz_max || !d_counts) return fail(MDMC_ERR_INVALID, "null argument");
+  int dev = 0;
+  int rc = current_device(&dev);
+  if (rc != MDMC_OK) return rc;
+  if (dev < 0 || dev >= 64) return fail(MDMC_ERR_INVALID, "device index out of range");
+  void *ptr = nullptr;
+  int cus = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_setup_mutex);
+    if ((g_setup_mask >> dev) & 1ull) {
+      ptr = g_noalloc_ptr[dev];
+      cus = g_noalloc_cus[dev];
+    }
+  }
+  if (!ptr) return fail(MDMC_ERR_NOT_SETUP, "mdmc_setup() not called on this device");
+  Plan p;
+  rc = plan_tiling(dims, &p, nullptr);
+  if (rc != MDMC_OK) return rc;
+  if (!p.empty && !d_field) return fail(MDMC_ERR_INVALID, "null field");
+  const size_t need = p.empty ? 0 : scratch_layout(p).total;
+  if (need > MDMC_NOALLOC_SCRATCH_BYTES)
+    return fail(MDMC_ERR_CAPACITY, "grid needs " + std::to_string(need) +
+                                       " bytes of scratch; the no-allocation entry has " +
+                                       std::to_string(MDMC_NOALLOC_SCRATCH_BYTES));
+  // a context on the stack over the fixed array: its scratch is external, so
+  // nothing below allocates (the launch helpers only carve and launch)
+  mdmc_ctx c;
+  c.device = dev;
+  c.num_cus = cus;
+  c.scratch = ptr;
+  c.scratch_bytes = MDMC_NOALLOC_SCRATCH_BYTES;
+  c.scratch_external = true;
+  rc = ensure_scratch(&c, p);
+  if (rc != MDMC_OK) return rc;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  rc = launch_count(&c, p, d_field, iso, d_counts, st, xyz_min, xyz_max);
+  if (rc != MDMC_OK) return rc;
+  return launch_emit(&c, p, d_field, iso, xyz_min, xyz_max, d_vertices, vertex_capacity, d_quads, quad_capacity,
+                     d_counts, st);
 }
 
 int mdmc_smooth_device(mdmc_ctx *ctx, float *d_vertices, uint64_t vertex_capacity, const int32_t *d_quads,

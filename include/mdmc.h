@@ -206,6 +206,23 @@ int mdmc_extract_device(mdmc_ctx *ctx, const float *d_field, const uint32_t dims
                         float *d_vertices, uint64_t vertex_capacity, int32_t *d_quads,
                         uint64_t quad_capacity, mdmc_counts *d_counts, void *stream);
 
+/* mdmc_extract_device without a context and without any allocation (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under stream capture, tight serving loops, harnesses that
+ * forbid it).  The scratch is a fixed device array inside the library, one
+ * per GPU, of MDMC_NOALLOC_SCRATCH_BYTES; its address is taken by
+ * mdmc_setup() on the current device, which must have run there.  Grids whose
+ * scratch does not fit (roughly: more than 256^3 points, or rows longer than
+ * 256 cells in grids of more than about 128^2 rows) return MDMC_ERR_CAPACITY
+ * before anything is launched.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams, other host threads) race and
+ * must be serialised by the caller.  Results are identical to
+ * mdmc_extract_device's. */
+#define MDMC_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int mdmc_extract_device_noalloc(const float *d_field, const uint32_t dims[3], const float xyz_min[3],
+                                const float xyz_max[3], float iso, float *d_vertices, uint64_t vertex_capacity,
+                                int32_t *d_quads, uint64_t quad_capacity, mdmc_counts *d_counts, void *stream);
+
 /* Smoothing iterations over the mesh the last mdmc_emit_device /
  * mdmc_extract_device of this context wrote (pass the same outputs): each
  * iteration is smooth_edge_vertices (dmc.cu:1868-1981) followed by
