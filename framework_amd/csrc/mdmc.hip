@@ -813,6 +813,7 @@ int batch_count(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts 
   if (rc != MDMC_OK) return rc;
   ctx->batch_counted = false;
   ctx->counted = false;  // the scratch now holds this batch
+  ctx->emitted = false;
   if (nseg_est == 0) {  // nothing to extract: zero every volume's counts
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, reinterpret_cast<uint32_t *>(d_counts), 4u * n);
     HIP_TRY(hipGetLastError());

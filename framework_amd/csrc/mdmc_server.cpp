@@ -242,103 +242,106 @@ bool carve_request(mdmc_server *s, Req *r) {
   return true;
 }
 
-// Starts a batch on an idle lane from the head of the queue.
+// Gives a request's regions back and returns it to the head of the queue.
+void uncarve_request(mdmc_server *s, Req *r) {
+  pool_free(s, &r->field_off);
+  pool_free(s, &r->vert_off);
+  pool_free(s, &r->quad_off);
+  r->field = nullptr;
+  r->verts = nullptr;
+  r->quads = nullptr;
+}
+
+// Starts a batch on an idle lane from the head of the queue.  Requests join
+// in arrival order while their regions and the scratch of the batch with
+// them can be carved (the scratch is re-carved for each candidate), so a
+// started batch always holds everything it needs.
 int start_batch(mdmc_server *s, Lane &lane) {
   Batch *b = new (std::nothrow) Batch();
   if (!b) return MDMC_ERR_NOMEM;
-  std::vector<mdmc_gen_spec> gens;
+  uint64_t sbytes = 0;
   while (!s->queue.empty() && b->reqs.size() < s->cfg.max_batch) {
     Req *r = s->queue.front();
     if (!carve_request(s, r)) break;  // wait for memory (arrival order)
-    if (r->spec.field_kind != MDMC_FIELD_DEVICE) {
-      r->field = reinterpret_cast<const float *>(s->base + r->field_off);
-      mdmc_gen_spec g{};
-      g.out = const_cast<float *>(r->field);
-      std::memcpy(g.dims, r->spec.dims, sizeof(g.dims));
-      g.kind = r->spec.field_kind;
-      g.seed = r->spec.seed;
-      g.nblobs = r->spec.nblobs;
-      g.sigma_lo = r->spec.sigma_lo;
-      g.sigma_hi = r->spec.sigma_hi;
-      g.amp_lo = r->spec.amp_lo;
-      g.amp_hi = r->spec.amp_hi;
-      g.period = r->spec.period;
-      g.noise_amp = r->spec.noise_amp;
-      gens.push_back(g);
-    } else {
-      r->field = r->spec.field;
+    mdmc_volume v{};
+    std::memcpy(v.dims, r->spec.dims, sizeof(v.dims));
+    b->vols.push_back(v);
+    uint64_t want = 0;
+    int rc = mdmc_batch_scratch_bytes(b->vols.data(), (uint32_t)b->vols.size(), &want);
+    bool ok = rc == MDMC_OK;
+    if (ok && want != sbytes) {
+      // give the old block back first, so that the new one may reuse it; if
+      // the new one does not fit, the old size fits again where it was freed
+      pool_free(s, &b->scratch_off);
+      if (want && !pool_alloc(s, want, &b->scratch_off)) {
+        ok = false;
+        if (sbytes && !pool_alloc(s, sbytes, &b->scratch_off)) sbytes = 0;  // cannot happen: it was just freed
+      } else {
+        sbytes = want;
+      }
     }
-    r->verts = r->vmax ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
-    r->quads = r->qmax ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
+    if (!ok) {
+      b->vols.pop_back();
+      uncarve_request(s, r);
+      break;
+    }
     s->queue.pop_front();
     r->state = 1;
     r->t_start = s->now();
+    r->field = r->spec.field_kind != MDMC_FIELD_DEVICE ? reinterpret_cast<const float *>(s->base + r->field_off)
+                                                       : r->spec.field;
+    r->verts = r->vmax ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
+    r->quads = r->qmax ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
     b->reqs.push_back(r);
-    mdmc_volume v{};
-    v.field = r->field;
-    std::memcpy(v.dims, r->spec.dims, sizeof(v.dims));
-    v.iso = r->spec.iso;
-    std::memcpy(v.xyz_min, r->spec.xyz_min, sizeof(v.xyz_min));
-    std::memcpy(v.xyz_max, r->spec.xyz_max, sizeof(v.xyz_max));
-    v.vertices = r->verts;
-    v.vertex_capacity = r->vmax;
-    v.quads = r->quads;
-    v.quad_capacity = r->qmax;
-    b->vols.push_back(v);
+    mdmc_volume &bv = b->vols.back();
+    bv.field = r->field;
+    bv.iso = r->spec.iso;
+    std::memcpy(bv.xyz_min, r->spec.xyz_min, sizeof(bv.xyz_min));
+    std::memcpy(bv.xyz_max, r->spec.xyz_max, sizeof(bv.xyz_max));
+    bv.vertices = r->verts;
+    bv.vertex_capacity = r->vmax;
+    bv.quads = r->quads;
+    bv.quad_capacity = r->qmax;
   }
   const uint32_t n = (uint32_t)b->reqs.size();
   if (n == 0) {
+    pool_free(s, &b->scratch_off);
     delete b;
     return MDMC_OK;
   }
-  TRACE("start batch of %u on stream %p (in use %llu)", n, (void *)lane.stream,
-        (unsigned long long)s->pool.in_use());
+  TRACE("start batch of %u on stream %p (scratch %llu, in use %llu)", n, (void *)lane.stream,
+        (unsigned long long)sbytes, (unsigned long long)s->pool.in_use());
   lane.batch = b;
   s->batches += 1;
-  uint64_t sbytes = 0;
-  int rc = mdmc_batch_scratch_bytes(b->vols.data(), n, &sbytes);
-  if (rc != MDMC_OK) {
-    fail_batch(s, lane, rc);
-    return MDMC_OK;
+  std::vector<mdmc_gen_spec> gens;
+  for (Req *r : b->reqs) {
+    if (r->spec.field_kind == MDMC_FIELD_DEVICE) continue;
+    mdmc_gen_spec g{};
+    g.out = const_cast<float *>(r->field);
+    std::memcpy(g.dims, r->spec.dims, sizeof(g.dims));
+    g.kind = r->spec.field_kind;
+    g.seed = r->spec.seed;
+    g.nblobs = r->spec.nblobs;
+    g.sigma_lo = r->spec.sigma_lo;
+    g.sigma_hi = r->spec.sigma_hi;
+    g.amp_lo = r->spec.amp_lo;
+    g.amp_hi = r->spec.amp_hi;
+    g.period = r->spec.period;
+    g.noise_amp = r->spec.noise_amp;
+    gens.push_back(g);
   }
-  TRACE("  scratch %llu bytes, %zu fields to generate", (unsigned long long)sbytes, gens.size());
-  if (sbytes) {
-    if (!pool_alloc(s, sbytes, &b->scratch_off)) {
-      // no room for the batch's scratch: give the requests back to the
-      // queue, in order (nothing was launched)
-      for (auto it = b->reqs.rbegin(); it != b->reqs.rend(); ++it) {
-        Req *r = *it;
-        pool_free(s, &r->field_off);
-        pool_free(s, &r->vert_off);
-        pool_free(s, &r->quad_off);
-        r->field = nullptr;
-        r->verts = nullptr;
-        r->quads = nullptr;
-        r->state = 0;
-        s->queue.push_front(r);
-      }
-      delete b;
-      lane.batch = nullptr;
-      s->batches -= 1;
-      return MDMC_OK;
-    }
-    rc = mdmc_ctx_set_scratch(lane.ctx, s->base + b->scratch_off, sbytes);
-    if (rc != MDMC_OK) {
-      fail_batch(s, lane, rc);
-      return MDMC_OK;
-    }
-  }
-  if (!gens.empty()) {
+  // the lane's context works in the batch's scratch (none: a batch of empty
+  // volumes, which needs none; the context then never touches a stale block)
+  int rc = mdmc_ctx_set_scratch(lane.ctx, sbytes ? s->base + b->scratch_off : nullptr, sbytes);
+  if (rc == MDMC_OK && !gens.empty()) {
     rc = mdmc_batch_generate_device(lane.ctx, gens.data(), (uint32_t)gens.size(), lane.stream);
-    TRACE("  generate rc %d", rc);
-    if (rc != MDMC_OK) {
-      fail_batch(s, lane, rc);
-      return MDMC_OK;
-    }
+    TRACE("  generate %zu rc %d", gens.size(), rc);
   }
   // count and emit in one sequence: the outputs have their worst-case size
-  rc = mdmc_batch_extract_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
-  TRACE("  extract rc %d", rc);
+  if (rc == MDMC_OK) {
+    rc = mdmc_batch_extract_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
+    TRACE("  extract rc %d", rc);
+  }
   if (rc == MDMC_OK &&
       (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
                       lane.stream) != hipSuccess ||
@@ -386,6 +389,17 @@ bool pump(mdmc_server *s) {
     if (lane.batch || s->queue.empty()) continue;
     if (start_batch(s, lane) != MDMC_OK) break;
     if (lane.batch) progress = true;
+  }
+  // With no batch in flight and no region in use (results all released), a
+  // request whose footprint fits the budget always starts; if the head still
+  // did not, it cannot (the footprint check at submission makes this a
+  // safeguard): reject it rather than wait for a wake-up that cannot come.
+  if (!s->queue.empty() && s->active_lanes() == 0 && s->pool.in_use() == 0) {
+    Req *r = s->queue.front();
+    s->queue.pop_front();
+    TRACE("reject %llu: does not fit an empty pool", (unsigned long long)r->id);
+    finish(s, r, MDMC_ERR_REJECTED);
+    progress = true;
   }
   s->note_busy();
   return progress;

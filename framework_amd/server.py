@@ -203,6 +203,9 @@ def run_stream(server: Server, requests: list, arrivals: np.ndarray, keep: set |
             continue
         if not res and i >= n:
             res = server.poll(wait=True)
+            if not res:
+                raise RuntimeError(f"server went idle with {n - done} request(s) unfinished "
+                                   "(their results were all released, so none can start)")
         for r in res:
             k = idx_of[r["id"]]
             lat[k] = r["t_done"] - r["t_submit"]

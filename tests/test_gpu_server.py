@@ -230,6 +230,8 @@ def test_served_workload_stream(fa_mod, torch_cuda):
             (c["num_active"], c["num_vertices"], c["num_quads"]), k
         assert torch.equal(sq, q) and torch.equal(sv.view(torch.int32), v.view(torch.int32)), k
         seen["single"] += 1
+        if seen["single"] % 40 == 0:
+            print(f"  {seen['single']} results checked", flush=True)
         if k in to_oracle:
             fnp = _regenerate_cpu(r)
             assert np.array_equal(fnp.view(np.uint32), f.cpu().numpy().view(np.uint32)), k
