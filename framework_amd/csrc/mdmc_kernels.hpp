@@ -1188,35 +1188,53 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   if ((uint64_t)qb + nq > vol.qcap) { atomicOr(vol.status, kStatusQuadOverflow); return; }
   // Ring cells n = 0 (+1,0,0), 1 (0,+1,0), 2 (0,0,+1), 3 (+1,+1,0),
   // 4 (+1,0,+1), 5 (0,+1,+1).  A ring cell contains the crossing edge, so it
-  // is active and its group-map entry is current.  A cell the quads do not
-  // need is replaced by this cell, so that every load below is in bounds and
-  // all of them issue together: group maps and segment bases, then records.
+  // is active.  Cells 1, 2 and 5 are found through their segments' group
+  // maps.  Cells 0, 3 and 4 follow from this cell, 1 and 2: a ring cell and
+  // its -x neighbour in the ring are both active, so the +x one is the next
+  // active cell of its row (the first of the next x-tile's segment when x is
+  // the tile's last cell).  Each record's x field confirms the cell found.
+  // A lookup the quads do not need reads this cell's segment instead, so that
+  // every load is in bounds.
   const uint32_t need = (e6 ? 0x0Bu : 0u) | (e9 ? 0x15u : 0u) | (e10 ? 0x26u : 0u);
   const bool wrap = xl + 1u == cap;  // +x lies in the next x-tile's segment
-  const uint32_t sx = wrap ? s + 1u : s, xl1 = wrap ? 0u : xl + 1u;
+  const uint32_t xl1 = wrap ? 0u : xl + 1u;
   const uint32_t dy = vol.XT, dz = vol.XT * g.cy;
-  const uint32_t ns0[6] = {sx, s + dy, s + dz, sx + dy, sx + dz, s + dy + dz};
-  const uint32_t nx0[6] = {xl1, xl, xl, xl1, xl1, xl};
-  uint32_t ns[6], nxl[6], vq0[6];
-  uint4 gm[6];
-#pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const bool use = (need >> n) & 1u;
-    ns[n] = use ? ns0[n] : s;
-    nxl[n] = use ? nx0[n] : xl;
-    gm[n] = a.gmap[(size_t)ns[n] * (cap >> 6) + (nxl[n] >> 6)];
-    vq0[n] = a.seg_vq[ns[n]].x;
+  const uint32_t s1 = (need & 0x0Au) ? s + dy : s, s2 = (need & 0x14u) ? s + dz : s;
+  const uint32_t s5 = (need & 0x20u) ? s + dy + dz : s;
+  const uint32_t G = cap >> 6;
+  const uint4 g1 = a.gmap[(size_t)s1 * G + (xl >> 6)], g2 = a.gmap[(size_t)s2 * G + (xl >> 6)];
+  const uint4 g5 = a.gmap[(size_t)s5 * G + (xl >> 6)];
+  uint32_t vq0[6];
+  vq0[1] = a.seg_vq[s1].x;
+  vq0[2] = a.seg_vq[s2].x;
+  vq0[5] = a.seg_vq[s5].x;
+  vq0[0] = vq.x;
+  vq0[3] = vq0[1];
+  vq0[4] = vq0[2];
+  if (wrap && (need & 0x19u)) {  // the next x-tile's segments of rows y, y+1 and plane z+1
+    vq0[0] = a.seg_vq[s + 1u].x;
+    vq0[3] = a.seg_vq[s1 + 1u].x;
+    vq0[4] = a.seg_vq[s2 + 1u].x;
   }
+  const uint32_t bit = xl & 63u;
+  const uint64_t below = (1ull << bit) - 1ull;
+  const uint64_t m1 = (uint64_t)g1.x | ((uint64_t)g1.y << 32), m2 = (uint64_t)g2.x | ((uint64_t)g2.y << 32);
+  const uint64_t m5 = (uint64_t)g5.x | ((uint64_t)g5.y << 32);
+  const uint32_t r1 = g1.z + (uint32_t)__popcll(m1 & below), r2 = g2.z + (uint32_t)__popcll(m2 & below);
+  const uint32_t r5 = g5.z + (uint32_t)__popcll(m5 & below);
+  const uint32_t ns[6] = {wrap ? s + 1u : s, s1, s2, wrap ? s1 + 1u : s1, wrap ? s2 + 1u : s2, s5};
+  const uint32_t nr[6] = {wrap ? 0u : rank + 1u, r1, r2, wrap ? 0u : r1 + 1u, wrap ? 0u : r2 + 1u, r5};
+  const uint32_t nx[6] = {xl1, xl, xl, xl1, xl1, xl};
+  const bool inmask[6] = {true, ((m1 >> bit) & 1ull) != 0, ((m2 >> bit) & 1ull) != 0, true, true,
+                          ((m5 >> bit) & 1ull) != 0};
   const uint32_t *slot_lo = reinterpret_cast<const uint32_t *>(a.slots);  // record bits 0..31
   uint32_t nrec[6];
   bool act[6];
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
-    const uint64_t mask = (uint64_t)gm[n].x | ((uint64_t)gm[n].y << 32);
-    const uint32_t bit = nxl[n] & 63u;
-    const uint32_t r = gm[n].z + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
-    act[n] = ((mask >> bit) & 1ull) && r < cap;
-    nrec[n] = slot_lo[2 * ((size_t)ns[n] * cap + (act[n] ? r : rank))];
+    const bool ok_r = ((need >> n) & 1u) && inmask[n] && nr[n] < cap;
+    nrec[n] = ok_r ? slot_lo[2 * ((size_t)ns[n] * cap + nr[n])] : 0u;
+    act[n] = ok_r && rec_x(nrec[n]) == nx[n];
   }
   bool ok = true;
   // vertex of ring cell n for its local edge e
