@@ -1080,6 +1080,7 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   if (x >= g.cx || xl >= cap) { atomicOr(vol.status, kStatusInternal); return; }
   if (k < vol.kmin || k >= vol.kmax) return;
   const uint2 vq = a.seg_vq[s];
+  const uint32_t nact = a.seg_counts[s].x & 0xFFFFu;
   const uint32_t vb = vq.x + rec_lvb(rec) - vol.vbase;  // volume-local numbers
   const uint32_t qb = vq.y + rec_lqb(rec) - vol.qbase;
 
@@ -1097,6 +1098,8 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   const float X0 = vol.xt_c[x], X1 = vol.xt_c[x + 1];
   const float Y0 = vol.yt_c[j], Y1 = vol.yt_c[j + 1];
   const float Z0 = vol.zt_c[k], Z1 = vol.zt_c[k + 1];
+  // a rank past the segment's count means inconsistent count-pass data
+  if (rank >= nact) { atomicOr(vol.status, kStatusInternal); return; }
 
   // Edge e runs from corner EF[e] to corner ET[e]: the orientation its
   // owning cell uses (owner edge 6: p2->p6, 9: p5->p6, 10: p6->p7), so the
@@ -1285,8 +1288,9 @@ __device__ __forceinline__ bool locate_cell(const EmitArgs &a, const Totals &tot
   s = (uint32_t)__shfl((int)wl.x, (int)m, kWave);
   if (act >= tot.num_active) return false;
   rank = act - ap;
-  if (u0 >= num_ne || u0 + m >= num_ne || s >= a.nsegs || act < ap ||
-      rank >= min(a.seg_counts[s].x & 0xFFFFu, a.cap)) {
+  // (the rank is checked against the segment's count by the caller, whose
+  // loads that check rides along with)
+  if (u0 >= num_ne || u0 + m >= num_ne || s >= a.nsegs || act < ap || rank >= a.cap) {
     atomicOr(a.v.status, kStatusInternal);
     return false;
   }
@@ -1415,7 +1419,7 @@ __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vo
   uint64_t rec;
   const CellPos c = cell_pos(a, vol, s, rank, rec);
   const FieldDesc &g = vol.g;
-  if (c.x >= g.cx) { atomicOr(vol.status, kStatusInternal); return; }
+  if (c.x >= g.cx || rank >= (a.seg_counts[s].x & 0xFFFFu)) { atomicOr(vol.status, kStatusInternal); return; }
   const uint32_t bip = owned_bipolar(c.cfg);
   const bool xin = c.x + 1 < g.cx, yin = c.j + 1 < g.cy, zin = c.k + 1 < g.cz;
   const uint32_t has = ((bip & 1u) && xin && yin ? 1u : 0u) | ((bip & 2u) && xin && zin ? 2u : 0u) |
@@ -1497,7 +1501,10 @@ __device__ __forceinline__ void reaverage_cell(const EmitArgs &a, const EmitVol 
   const CellPos c = cell_pos(a, vol, s, rank, rec);
   const FieldDesc &g = vol.g;
   const uint32_t cap = a.cap;
-  if (c.x >= g.cx || c.xl >= cap) { atomicOr(vol.status, kStatusInternal); return; }
+  if (c.x >= g.cx || c.xl >= cap || rank >= (a.seg_counts[s].x & 0xFFFFu)) {
+    atomicOr(vol.status, kStatusInternal);
+    return;
+  }
   const uint64_t labels = s_lab[c.lut2 * 256 + c.cfg];
   const uint32_t niso = iso_count(c.cfg, c.lut2);
   // owner cell (7 = this cell) and the owner's edge shift (0: e6, 1: e9, 2: e10)
