@@ -90,3 +90,39 @@ def test_noalloc_refuses_a_grid_beyond_its_scratch(env):
                                        _VP(torch.cuda.current_stream().cuda_stream))
     assert rc == -4  # MDMC_ERR_CAPACITY
     assert b"no-allocation entry has" in L.mdmc_last_error()
+
+
+def test_noalloc_under_graph_capture(env):
+    """The callers the entry is for: a HIP graph captured around it (no
+    allocation may happen inside a capture), replayed with other work between
+    replays; every replay equals the allocating entry's result."""
+    torch, fa, L = env
+    gen = fa.Context()
+    field = gen.field_gyroid((150, 130, 110), period=10.0, seed=8, noise_amp=0.1)
+    torch.cuda.synchronize()
+    v0, q0, c0 = gen.extract(field, 0.02)
+    Z, Y, X = field.shape
+    verts, quads = torch.empty_like(v0), torch.empty_like(q0)
+    counts = torch.zeros(4, dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        rc = L.mdmc_extract_device_noalloc(_VP(field.data_ptr()), _U3(X, Y, Z), _F3(0, 0, 0),
+                                           _F3(X - 1, Y - 1, Z - 1), ctypes.c_float(0.02), _VP(verts.data_ptr()),
+                                           verts.shape[0], _VP(quads.data_ptr()), quads.shape[0],
+                                           _VP(counts.data_ptr()), _VP(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0, L.mdmc_last_error()
+    for rep in range(3):
+        print(f"noalloc graph replay {rep}", flush=True)
+        verts.fill_(-1.0)
+        quads.fill_(-1)
+        counts.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        c = counts.cpu().numpy().view(np.uint32)
+        assert c[3] == 0 and (c[1], c[2]) == (c0["num_vertices"], c0["num_quads"]), rep
+        assert torch.equal(quads, q0) and torch.equal(verts.view(torch.int32), v0.view(torch.int32)), rep
+        _ = gen.extract(field, 0.1)  # other work on the GPU between replays
+    del g
+    gen.close()
