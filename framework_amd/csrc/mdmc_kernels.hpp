@@ -1063,6 +1063,22 @@ struct EmitArgs {
   const Totals *totals;
 };
 
+// Arrays seen through the constant address space: loads at wave-uniform
+// indices become scalar loads.  Only for arrays no kernel running at the
+// same time writes.
+__device__ __forceinline__ const __attribute__((address_space(4))) float *as_const(const float *p) {
+  return (const __attribute__((address_space(4))) float *)p;
+}
+// Entry i of an array of uint2 through the constant address space.
+__device__ __forceinline__ uint2 const_u2(const uint2 *p, uint32_t i) {
+  const __attribute__((address_space(4))) uint32_t *w = (const __attribute__((address_space(4))) uint32_t *)p;
+  return make_uint2(w[2 * i], w[2 * i + 1]);
+}
+// Its first word only.
+__device__ __forceinline__ uint32_t const_u2x(const uint2 *p, uint32_t i) {
+  return ((const __attribute__((address_space(4))) uint32_t *)p)[2 * i];
+}
+
 // One active cell (segment s, rank within it) of volume vol: its vertices and
 // the quads of its owned edges.
 __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
@@ -1079,8 +1095,33 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   const uint32_t x = xt * cap + xl;
   if (x >= g.cx || xl >= cap) { atomicOr(vol.status, kStatusInternal); return; }
   if (k < vol.kmin || k >= vol.kmax) return;
-  const uint2 vq = a.seg_vq[s];
-  const uint32_t nact = a.seg_counts[s].x & 0xFFFFu;
+  // The lanes of a chunk nearly always share one segment.  What depends only
+  // on the segment (its bases and count, the row's y and z coordinates, the
+  // ring segments' vertex bases below) comes through scalar loads for the
+  // lanes in the wave's first segment and through vector loads for the rest
+  // (these arrays are read-only in the emit).
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  // s0u: s0 in a scalar register that the compiler cannot equate with s, so
+  // that on the lanes where s == s0 it does not turn the loads below back
+  // into per-lane vector loads at s
+  uint32_t s0u = s0;
+  asm volatile("" : "+s"(s0u));
+  const uint32_t row0 = (s0u - vol.seg_off) / vol.XT;
+  const uint32_t j0 = row0 % g.cy, k0 = row0 / g.cy;
+  // (scalar loads first, outside any branch on the lane's segment)
+  uint2 vq = const_u2(a.seg_vq, s0u);
+  uint32_t nact = const_u2x(a.seg_counts, s0u) & 0xFFFFu;
+  float Y0 = as_const(vol.yt_c)[j0], Y1 = as_const(vol.yt_c)[j0 + 1];
+  float Z0 = as_const(vol.zt_c)[k0], Z1 = as_const(vol.zt_c)[k0 + 1];
+  const bool lead = s == s0;
+  if (!lead) {
+    vq = a.seg_vq[s];
+    nact = a.seg_counts[s].x & 0xFFFFu;
+    Y0 = vol.yt_c[j];
+    Y1 = vol.yt_c[j + 1];
+    Z0 = vol.zt_c[k];
+    Z1 = vol.zt_c[k + 1];
+  }
   const uint32_t vb = vq.x + rec_lvb(rec) - vol.vbase;  // volume-local numbers
   const uint32_t qb = vq.y + rec_lqb(rec) - vol.qbase;
 
@@ -1096,8 +1137,6 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   v[7] = p[sz + sy];
 
   const float X0 = vol.xt_c[x], X1 = vol.xt_c[x + 1];
-  const float Y0 = vol.yt_c[j], Y1 = vol.yt_c[j + 1];
-  const float Z0 = vol.zt_c[k], Z1 = vol.zt_c[k + 1];
   // a rank past the segment's count means inconsistent count-pass data
   if (rank >= nact) { atomicOr(vol.status, kStatusInternal); return; }
 
@@ -1207,10 +1246,18 @@ __device__ __forceinline__ void emit_cell(const EmitArgs &a, const EmitVol &vol,
   const uint32_t G = cap >> 6;
   const uint4 g1 = a.gmap[(size_t)s1 * G + (xl >> 6)], g2 = a.gmap[(size_t)s2 * G + (xl >> 6)];
   const uint4 g5 = a.gmap[(size_t)s5 * G + (xl >> 6)];
+  // the first segment's ring segments (clamped: a lane that needs one has it
+  // in the grid), scalar loads first as above
+  const uint32_t last = a.nsegs - 1u;
   uint32_t vq0[6];
-  vq0[1] = a.seg_vq[s1].x;
-  vq0[2] = a.seg_vq[s2].x;
-  vq0[5] = a.seg_vq[s5].x;
+  vq0[1] = const_u2x(a.seg_vq, min(s0u + dy, last));
+  vq0[2] = const_u2x(a.seg_vq, min(s0u + dz, last));
+  vq0[5] = const_u2x(a.seg_vq, min(s0u + dy + dz, last));
+  if (!lead) {
+    vq0[1] = a.seg_vq[s1].x;
+    vq0[2] = a.seg_vq[s2].x;
+    vq0[5] = a.seg_vq[s5].x;
+  }
   vq0[0] = vq.x;
   vq0[3] = vq0[1];
   vq0[4] = vq0[2];
