@@ -1285,6 +1285,15 @@ static int check_batch_args(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, 
   return capture_guard(ctx, static_cast<hipStream_t>(stream));
 }
 
+int mdmc_scratch_bytes(const uint32_t dims[3], uint64_t *bytes) {
+  if (!dims || !bytes) return fail(MDMC_ERR_INVALID, "null argument");
+  Plan p;
+  const int rc = plan_tiling(dims, &p, nullptr);  // the options do not change the layout
+  if (rc != MDMC_OK) return rc;
+  *bytes = p.empty ? 0 : scratch_layout(p).total;
+  return MDMC_OK;
+}
+
 int mdmc_batch_scratch_bytes(const mdmc_volume *vols, uint32_t n, uint64_t *bytes) {
   if (!vols || !bytes) return fail(MDMC_ERR_INVALID, "null argument");
   if (n == 0 || n > kMaxBatch) return fail(MDMC_ERR_INVALID, "batch size must be 1..1024");

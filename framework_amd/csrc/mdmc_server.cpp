@@ -22,7 +22,9 @@
 // max_batch, as long as their regions can be carved), generates their
 // fields, counts and emits them, and copies the counts back; the worker that
 // sees that sequence finish trims the outputs, frees fields and scratch and
-// hands out the results.  Lanes run independently, so a batch started for
+// hands out the results.  A request of at least `solo_cells` cells runs
+// alone on a lane through the single-volume extraction instead (a batch stops
+// before it, so arrival order holds).  Lanes run independently, so a batch started for
 // late arrivals overlaps with the batches already on the device.
 #include <hip/hip_runtime.h>
 
@@ -47,6 +49,11 @@ namespace {
 
 constexpr uint64_t kNoOffset = ~0ull;
 constexpr uint64_t kPoolAlign = 256;
+// Requests of at least this many cells run alone through the single-volume
+// kernels: on one large volume they beat the batch kernels (their classify
+// tiles and scan are shaped for one grid; a 384^3 gyroid takes 507 us alone
+// and 587 us as a batch of one), while small requests share launches.
+constexpr uint64_t kSoloCellsDefault = 200ull * 200 * 200;
 
 struct Req {
   uint64_t id = 0;
@@ -57,6 +64,7 @@ struct Req {
   uint64_t field_off = kNoOffset, vert_off = kNoOffset, quad_off = kNoOffset;
   uint64_t vmax = 0, qmax = 0;  // worst-case mesh of the dims
   uint64_t footprint = 0, scratch_est = 0;
+  bool solo = false;  // runs alone through the single-volume kernels
   const float *field = nullptr;
   float *verts = nullptr;
   int32_t *quads = nullptr;
@@ -125,6 +133,7 @@ struct mdmc_server {
   std::unordered_map<uint64_t, Req *> reqs;
   std::deque<Req *> finished;  // not yet returned by poll
   uint64_t next_id = 1;
+  uint64_t solo_cells = kSoloCellsDefault;  // 0: no request runs alone
   uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, peak = 0;
   double busy = 0, busy_since = -1;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -181,12 +190,18 @@ void pool_trim(mdmc_server *s, uint64_t *off, uint64_t bytes) {
   s->pool.shrink(*off, bytes);
 }
 
-// Scratch the count pass needs for one volume alone (the estimate in the
-// footprint; a batch's exact scratch is carved when it starts).
-uint64_t scratch_estimate(const mdmc_request &r) {
+uint64_t cells_of(const uint32_t d[3]) {
+  return d[0] < 2 || d[1] < 2 || d[2] < 2 ? 0 : (uint64_t)(d[0] - 1) * (d[1] - 1) * (d[2] - 1);
+}
+
+// Scratch one volume needs: exact for a request that runs alone; for one
+// that may join a batch, the estimate in its footprint (a batch's exact
+// scratch is carved when it starts).
+uint64_t scratch_estimate(const mdmc_request &r, bool solo) {
+  uint64_t b = 0;
+  if (solo) return mdmc_scratch_bytes(r.dims, &b) == MDMC_OK ? b : 0;
   mdmc_volume v{};
   std::memcpy(v.dims, r.dims, sizeof(v.dims));
-  uint64_t b = 0;
   if (mdmc_batch_scratch_bytes(&v, 1, &b) != MDMC_OK) return 0;
   return b;
 }
@@ -252,16 +267,96 @@ void uncarve_request(mdmc_server *s, Req *r) {
   r->quads = nullptr;
 }
 
+// Takes a carved request off the head of the queue into batch `b`.
+void admit(mdmc_server *s, Batch *b, Req *r) {
+  s->queue.pop_front();
+  r->state = 1;
+  r->t_start = s->now();
+  r->field = r->spec.field_kind != MDMC_FIELD_DEVICE ? reinterpret_cast<const float *>(s->base + r->field_off)
+                                                     : r->spec.field;
+  r->verts = r->vmax ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
+  r->quads = r->qmax ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
+  b->reqs.push_back(r);
+}
+
+// Generator specs of the batch's requests whose field the server makes.
+std::vector<mdmc_gen_spec> gen_specs(const Batch *b) {
+  std::vector<mdmc_gen_spec> gens;
+  for (const Req *r : b->reqs) {
+    if (r->spec.field_kind == MDMC_FIELD_DEVICE) continue;
+    mdmc_gen_spec g{};
+    g.out = const_cast<float *>(r->field);
+    std::memcpy(g.dims, r->spec.dims, sizeof(g.dims));
+    g.kind = r->spec.field_kind;
+    g.seed = r->spec.seed;
+    g.nblobs = r->spec.nblobs;
+    g.sigma_lo = r->spec.sigma_lo;
+    g.sigma_hi = r->spec.sigma_hi;
+    g.amp_lo = r->spec.amp_lo;
+    g.amp_hi = r->spec.amp_hi;
+    g.period = r->spec.period;
+    g.noise_amp = r->spec.noise_amp;
+    gens.push_back(g);
+  }
+  return gens;
+}
+
+// Queues the counts' copy to the host and the end-of-sequence event.
+int finish_launch(Lane &lane, uint32_t n) {
+  if (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
+                     lane.stream) != hipSuccess ||
+      hipEventRecord(lane.ev, lane.stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return MDMC_ERR_HIP;
+  }
+  return MDMC_OK;
+}
+
+// Starts the request at the head of the queue alone on an idle lane, through
+// the single-volume extraction, if its regions and its scratch can be carved.
+int start_solo(mdmc_server *s, Lane &lane) {
+  Req *r = s->queue.front();
+  Batch *b = new (std::nothrow) Batch();
+  if (!b) return MDMC_ERR_NOMEM;
+  if (!carve_request(s, r)) {
+    delete b;
+    return MDMC_OK;
+  }
+  const uint64_t sbytes = r->scratch_est;
+  if (sbytes && !pool_alloc(s, sbytes, &b->scratch_off)) {
+    uncarve_request(s, r);
+    delete b;
+    return MDMC_OK;
+  }
+  admit(s, b, r);
+  TRACE("start %llu alone on stream %p (scratch %llu, in use %llu)", (unsigned long long)r->id,
+        (void *)lane.stream, (unsigned long long)sbytes, (unsigned long long)s->pool.in_use());
+  lane.batch = b;
+  s->batches += 1;
+  const std::vector<mdmc_gen_spec> gens = gen_specs(b);
+  int rc = mdmc_ctx_set_scratch(lane.ctx, sbytes ? s->base + b->scratch_off : nullptr, sbytes);
+  if (rc == MDMC_OK && !gens.empty()) rc = mdmc_batch_generate_device(lane.ctx, gens.data(), 1, lane.stream);
+  if (rc == MDMC_OK)
+    rc = mdmc_extract_device(lane.ctx, r->field, r->spec.dims, r->spec.xyz_min, r->spec.xyz_max, r->spec.iso,
+                             r->verts, r->vmax, r->quads, r->qmax, lane.d_counts, lane.stream);
+  TRACE("  extract alone rc %d", rc);
+  if (rc == MDMC_OK) rc = finish_launch(lane, 1);
+  if (rc != MDMC_OK) fail_batch(s, lane, rc);
+  return MDMC_OK;
+}
+
 // Starts a batch on an idle lane from the head of the queue.  Requests join
 // in arrival order while their regions and the scratch of the batch with
 // them can be carved (the scratch is re-carved for each candidate), so a
 // started batch always holds everything it needs.
 int start_batch(mdmc_server *s, Lane &lane) {
+  if (!s->queue.empty() && s->queue.front()->solo) return start_solo(s, lane);
   Batch *b = new (std::nothrow) Batch();
   if (!b) return MDMC_ERR_NOMEM;
   uint64_t sbytes = 0;
   while (!s->queue.empty() && b->reqs.size() < s->cfg.max_batch) {
     Req *r = s->queue.front();
+    if (r->solo) break;  // it starts on the next idle lane (arrival order)
     if (!carve_request(s, r)) break;  // wait for memory (arrival order)
     mdmc_volume v{};
     std::memcpy(v.dims, r->spec.dims, sizeof(v.dims));
@@ -285,14 +380,7 @@ int start_batch(mdmc_server *s, Lane &lane) {
       uncarve_request(s, r);
       break;
     }
-    s->queue.pop_front();
-    r->state = 1;
-    r->t_start = s->now();
-    r->field = r->spec.field_kind != MDMC_FIELD_DEVICE ? reinterpret_cast<const float *>(s->base + r->field_off)
-                                                       : r->spec.field;
-    r->verts = r->vmax ? reinterpret_cast<float *>(s->base + r->vert_off) : nullptr;
-    r->quads = r->qmax ? reinterpret_cast<int32_t *>(s->base + r->quad_off) : nullptr;
-    b->reqs.push_back(r);
+    admit(s, b, r);
     mdmc_volume &bv = b->vols.back();
     bv.field = r->field;
     bv.iso = r->spec.iso;
@@ -313,23 +401,7 @@ int start_batch(mdmc_server *s, Lane &lane) {
         (unsigned long long)sbytes, (unsigned long long)s->pool.in_use());
   lane.batch = b;
   s->batches += 1;
-  std::vector<mdmc_gen_spec> gens;
-  for (Req *r : b->reqs) {
-    if (r->spec.field_kind == MDMC_FIELD_DEVICE) continue;
-    mdmc_gen_spec g{};
-    g.out = const_cast<float *>(r->field);
-    std::memcpy(g.dims, r->spec.dims, sizeof(g.dims));
-    g.kind = r->spec.field_kind;
-    g.seed = r->spec.seed;
-    g.nblobs = r->spec.nblobs;
-    g.sigma_lo = r->spec.sigma_lo;
-    g.sigma_hi = r->spec.sigma_hi;
-    g.amp_lo = r->spec.amp_lo;
-    g.amp_hi = r->spec.amp_hi;
-    g.period = r->spec.period;
-    g.noise_amp = r->spec.noise_amp;
-    gens.push_back(g);
-  }
+  const std::vector<mdmc_gen_spec> gens = gen_specs(b);
   // the lane's context works in the batch's scratch (none: a batch of empty
   // volumes, which needs none; the context then never touches a stale block)
   int rc = mdmc_ctx_set_scratch(lane.ctx, sbytes ? s->base + b->scratch_off : nullptr, sbytes);
@@ -342,13 +414,7 @@ int start_batch(mdmc_server *s, Lane &lane) {
     rc = mdmc_batch_extract_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
     TRACE("  extract rc %d", rc);
   }
-  if (rc == MDMC_OK &&
-      (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
-                      lane.stream) != hipSuccess ||
-       hipEventRecord(lane.ev, lane.stream) != hipSuccess)) {
-    (void)hipGetLastError();
-    rc = MDMC_ERR_HIP;
-  }
+  if (rc == MDMC_OK) rc = finish_launch(lane, n);
   if (rc != MDMC_OK) fail_batch(s, lane, rc);
   return MDMC_OK;
 }
@@ -542,7 +608,8 @@ int mdmc_server_submit(mdmc_server *s, const mdmc_request *req, uint64_t *id) {
   r->spec = *req;
   r->t_submit = s->now();
   const uint64_t fb = field_bytes(*req);
-  r->scratch_est = scratch_estimate(*req);
+  r->solo = s->solo_cells && cells_of(req->dims) >= s->solo_cells;
+  r->scratch_est = scratch_estimate(*req, r->solo);
   worst_case_mesh(req->dims, &r->vmax, &r->qmax);
   r->footprint = (req->field_kind != MDMC_FIELD_DEVICE ? round_up(fb) : 0) + round_up(r->scratch_est) +
                  round_up(12 * r->vmax) + round_up(16 * r->qmax);
@@ -634,6 +701,11 @@ int mdmc_server_release(mdmc_server *s, uint64_t id) {
 int mdmc_server_set_option(mdmc_server *s, int option, int value) {
   if (!s) return MDMC_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->mu);
+  if (option == MDMC_SERVER_OPT_SOLO_CELLS) {
+    if (value < 0) return MDMC_ERR_INVALID;
+    s->solo_cells = (uint64_t)value;
+    return MDMC_OK;
+  }
   for (Lane &l : s->lanes) {
     const int rc = mdmc_ctx_set_option(l.ctx, option, value);
     if (rc != MDMC_OK) return rc;

@@ -19,6 +19,10 @@ _VP = ctypes.c_void_p
 
 FIELD_DEVICE, FIELD_BLOBS, FIELD_GYROID = 0, 1, 2
 
+# server option: requests of at least this many cells run alone through the
+# single-volume kernels (0: every request may join a batch)
+OPT_SOLO_CELLS = 100
+
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget", ctypes.c_uint64), ("max_batch", ctypes.c_uint32),
@@ -174,7 +178,7 @@ class Server:
 
     def set_options(self, options: dict) -> None:
         """Launch options ({MDMC_OPT_* id: value}) for every lane, e.g. an
-        autotuner's batch choice."""
+        autotuner's batch choice, or server options (OPT_SOLO_CELLS)."""
         for opt, val in options.items():
             _check(lib().mdmc_server_set_option(self._h, int(opt), int(val)), "mdmc_server_set_option")
 

@@ -298,6 +298,10 @@ typedef struct mdmc_gen_spec {
 } mdmc_gen_spec;
 int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32_t n, void *stream);
 
+/* Device scratch bytes a context needs for mdmc_count_device /
+ * mdmc_extract_device on a grid of these dims, for mdmc_ctx_set_scratch (no
+ * counterpart in the reference, which allocates per call; host arithmetic). */
+int mdmc_scratch_bytes(const uint32_t dims[3], uint64_t *bytes);
 /* Device scratch bytes mdmc_batch_count_device needs for this batch (fields
  * may be NULL here; only dims are read). */
 int mdmc_batch_scratch_bytes(const mdmc_volume *vols, uint32_t n, uint64_t *bytes);
@@ -376,7 +380,16 @@ int mdmc_server_copy_result(mdmc_server *srv, uint64_t id, float *h_vertices, in
 /* Frees a finished result's device memory. */
 int mdmc_server_release(mdmc_server *srv, uint64_t id);
 int mdmc_server_get_stats(mdmc_server *srv, mdmc_server_stats *st);
-/* Sets a launch option (MDMC_OPT_*) on every lane's context. */
+/* Server options, set with mdmc_server_set_option; they apply to requests
+ * submitted afterwards. */
+enum {
+  /* requests of at least this many cells run alone, through the
+   * single-volume kernels, which are faster than the batch kernels on one
+   * large volume (0: every request may join a batch) */
+  MDMC_SERVER_OPT_SOLO_CELLS = 100
+};
+/* Sets a server option (MDMC_SERVER_OPT_*), or a launch option (MDMC_OPT_*)
+ * on every lane's context. */
 int mdmc_server_set_option(mdmc_server *srv, int option, int value);
 /* Seconds on the server clock (the clock of t_submit / t_done). */
 double mdmc_server_now(mdmc_server *srv);

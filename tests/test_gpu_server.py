@@ -83,13 +83,19 @@ def test_result_alone_equals_result_in_mix(fa_mod, torch_cuda):
         assert np.array_equal(va.view(np.uint32), vs.view(np.uint32)), k
 
 
-def test_every_result_in_a_stream_equals_its_single_extraction(fa_mod, torch_cuda):
+@pytest.mark.parametrize("solo_cells", [None, 30 ** 3, 1])
+def test_every_result_in_a_stream_equals_its_single_extraction(fa_mod, torch_cuda, solo_cells):
     """All results of a continuously batched stream (arrivals while batches
-    run, uneven sizes, both field kinds) against the single-volume path."""
-    from framework_amd.server import Server, run_stream
+    run, uneven sizes, both field kinds) against the single-volume path; with
+    the default routing (every request here is below the size that runs
+    alone), with the larger requests running alone between batches, and with
+    every request alone."""
+    from framework_amd.server import OPT_SOLO_CELLS, Server, run_stream
 
     reqs = _requests(24, seed=9, dmin=16, dmax=72)
     srv = Server(1 << 30, max_batch=5, num_streams=3)
+    if solo_cells is not None:
+        srv.set_options({OPT_SOLO_CELLS: solo_cells})
     arrivals = np.cumsum(np.random.default_rng(1).exponential(1e-4, len(reqs)))
     res = run_stream(srv, reqs, arrivals, keep=set(range(len(reqs))))
     st = srv.stats()
@@ -102,15 +108,19 @@ def test_every_result_in_a_stream_equals_its_single_extraction(fa_mod, torch_cud
         assert np.array_equal(vm.view(np.uint32), vs.view(np.uint32)), k
 
 
-def test_admission_under_a_tight_budget(fa_mod, torch_cuda):
+@pytest.mark.parametrize("solo_cells", [None, 40 ** 3])
+def test_admission_under_a_tight_budget(fa_mod, torch_cuda, solo_cells):
     """Requests larger than the budget are rejected; everything else finishes
-    with the peak memory within the budget."""
-    from framework_amd.server import Server, make_request, run_stream
+    with the peak memory within the budget (also with the larger requests
+    running alone, whose scratch is the single-volume one)."""
+    from framework_amd.server import OPT_SOLO_CELLS, Server, make_request, run_stream
 
     budget = 96 << 20
     reqs = _requests(30, seed=4, dmin=24, dmax=80)
     reqs.insert(7, make_request((400, 400, 400), 0.0, "gyroid", seed=1, tag=99))  # 256 MB field
     srv = Server(budget, max_batch=6, num_streams=2)
+    if solo_cells is not None:
+        srv.set_options({OPT_SOLO_CELLS: solo_cells})
     res = run_stream(srv, reqs, np.zeros(len(reqs)))
     st = srv.stats()
     srv.close()

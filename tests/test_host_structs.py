@@ -64,6 +64,24 @@ def test_batch_scratch_query_without_gpu():
     assert L.mdmc_batch_scratch_bytes(one, 1, ctypes.byref(b)) == 0 and b.value == 0
 
 
+def test_single_scratch_query_without_gpu():
+    """mdmc_scratch_bytes is host arithmetic: the record slots (8 bytes per
+    slot, one x-tile of at least 256 slots per (y, z) row of cells) bound it
+    from below, an empty grid needs none, a null argument is refused."""
+    L = fa.lib()
+    L.mdmc_scratch_bytes.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]
+    b = ctypes.c_uint64()
+    dims = (ctypes.c_uint32 * 3)(200, 150, 100)
+    assert L.mdmc_scratch_bytes(dims, ctypes.byref(b)) == 0
+    assert b.value >= 149 * 99 * 256 * 8
+    assert b.value < 149 * 99 * 1024 * 8
+    big = ctypes.c_uint64()
+    assert L.mdmc_scratch_bytes((ctypes.c_uint32 * 3)(400, 150, 100), ctypes.byref(big)) == 0
+    assert big.value > b.value
+    assert L.mdmc_scratch_bytes((ctypes.c_uint32 * 3)(1, 9, 9), ctypes.byref(b)) == 0 and b.value == 0
+    assert L.mdmc_scratch_bytes(None, ctypes.byref(b)) != 0
+
+
 def test_classify_tile_map_covers_every_tile_once(tmp_path):
     """The plane-tile classify's workgroup -> tile map (mdmc_tilemap.h), over
     3000 random grid shapes: every tile exactly once, XCD shares within one
