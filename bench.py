@@ -29,9 +29,10 @@ Four more legs go into the same line:
       against one batched launch sequence and its HIP-graph replay.
   "served": a Poisson stream of requests (cubic dims log-uniform in
       [48, 384], blob or gyroid fields generated from per-request seeds,
-      isovalues U[-0.2, 0.2]) through the request loop under a 64 GB budget
-      at 0.8x its measured capacity: volumes/s, p50/p99 latency, and the
-      plain per-volume loop on the same requests.
+      isovalues U[-0.2, 0.2]) through the request loop (8 lanes, batches of
+      up to 32) under a 64 GB budget at 0.8x its measured capacity:
+      volumes/s, p50/p99 latency, and the plain per-volume loop on the same
+      requests.
 """
 from __future__ import annotations
 
@@ -532,7 +533,7 @@ def bench_served(fa, torch, nreq: int, budget: int = 64 << 30, options: dict | N
     from framework_amd.server import Server, run_plain_loop, run_stream
 
     reqs = _served_requests(nreq, seed=5)
-    srv = Server(budget, max_batch=32, num_streams=4)
+    srv = Server(budget, max_batch=32, num_streams=8)
     if options:
         srv.set_options(options)
     # capacity: a saturating burst (everything arrives at once) on a sample

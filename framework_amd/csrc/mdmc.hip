@@ -1902,7 +1902,7 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
     if (npts && g.kind == MDMC_FIELD_BLOBS && g.nblobs > kBlobRegs)
       nb64 = std::min<uint64_t>((npts + 255) / 256, 4096);
     else if (npts)
-      nb64 = (gen_tasks(X, Y, Z) + kGenWaves - 1) / kGenWaves;
+      nb64 = ((g.kind == MDMC_FIELD_BLOBS ? blob_tasks(X, Y, Z) : gyroid_tasks(X, Y, Z)) + kGenWaves - 1) / kGenWaves;
     if (nb64 + blocks > 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "fields too large for the generator grid");
     G.nblocks = (uint32_t)nb64;
     blocks += G.nblocks;

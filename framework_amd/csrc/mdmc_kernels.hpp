@@ -1873,11 +1873,15 @@ struct BlobArgs {
   const double *ex, *ey, *ez, *amp;
 };
 
-// Generators work in wave tasks: 64 consecutive columns x of kGenRows
-// consecutive rows r = y + Y*z.  A lane owns one column; the row loop is
-// uniform over the wave, so per-row values are scalar loads and the stores
-// of a row are one contiguous 256-byte run.
+// Generators work in wave tasks of 64 consecutive columns x; a lane owns one
+// column and the loop over the task's rows is uniform over the wave, so
+// per-row values are scalar loads and the stores of a row are one contiguous
+// 256-byte run.  Blob tasks take kGenRows consecutive rows r = y + Y*z;
+// gyroid tasks take one row y in kGenPlanes consecutive planes z (what
+// depends on x and y only is then computed once per task).  Consecutive
+// tasks take consecutive column blocks first.
 constexpr uint32_t kGenRows = 16;
+constexpr uint32_t kGenPlanes = 16;
 constexpr uint32_t kGenThreads = 256;
 constexpr uint32_t kGenWaves = kGenThreads / kWave;
 constexpr uint32_t kBlobRegs = 32;  // blobs kept in registers (more: per-point path)
@@ -1887,8 +1891,11 @@ __host__ __device__ inline uint32_t blob_ldb(uint32_t nblobs) {
   return nblobs <= kBlobRegs ? kBlobRegs : nblobs;
 }
 
-__host__ __device__ inline uint64_t gen_tasks(uint32_t X, uint32_t Y, uint32_t Z) {
+__host__ __device__ inline uint64_t blob_tasks(uint32_t X, uint32_t Y, uint32_t Z) {
   return (uint64_t)((X + 63u) / 64u) * (((uint64_t)Y * Z + kGenRows - 1u) / kGenRows);
+}
+__host__ __device__ inline uint64_t gyroid_tasks(uint32_t X, uint32_t Y, uint32_t Z) {
+  return (uint64_t)((X + 63u) / 64u) * Y * ((Z + kGenPlanes - 1u) / kGenPlanes);
 }
 
 // Wave-uniform reads through the constant address space compile to scalar
@@ -1896,15 +1903,16 @@ __host__ __device__ inline uint64_t gen_tasks(uint32_t X, uint32_t Y, uint32_t Z
 typedef const __attribute__((address_space(4))) double *uniform_dptr;
 __device__ __forceinline__ uniform_dptr as_uniform(const double *p) { return (uniform_dptr)p; }
 
-// First row of a task as (y0, z0); row r0 + i is (y0 + i, z0) carried into z.
-struct GenTask {
+// A blob task: the lane's column x and rows r0 .. r0 + nr - 1, the first as
+// (y0, z0); row r0 + i is (y0 + i, z0) carried into z.
+struct RowTask {
   uint32_t x, nr, y0, z0;
   uint64_t r0;
 };
 
-__device__ __forceinline__ GenTask gen_task(uint32_t X, uint32_t Y, uint32_t Z, uint64_t task) {
+__device__ __forceinline__ RowTask row_task(uint32_t X, uint32_t Y, uint32_t Z, uint64_t task) {
   const uint32_t ncc = (X + 63u) / 64u;
-  GenTask t;
+  RowTask t;
   t.x = (uint32_t)(task % ncc) * 64u + (threadIdx.x & 63u);
   t.r0 = (task / ncc) * kGenRows;
   const uint64_t nrows = (uint64_t)Y * Z;
@@ -1914,7 +1922,7 @@ __device__ __forceinline__ GenTask gen_task(uint32_t X, uint32_t Y, uint32_t Z, 
   return t;
 }
 
-__device__ __forceinline__ void gen_row_yz(const GenTask &t, uint32_t Y, uint32_t i, uint32_t &y, uint32_t &z) {
+__device__ __forceinline__ void row_yz(const RowTask &t, uint32_t Y, uint32_t i, uint32_t &y, uint32_t &z) {
   y = t.y0 + i;
   z = t.z0;
   while (y >= Y) {
@@ -1923,14 +1931,34 @@ __device__ __forceinline__ void gen_row_yz(const GenTask &t, uint32_t Y, uint32_
   }
 }
 
+// A gyroid task: the lane's column x, its row y, and its planes
+// z0 .. z0 + nz - 1 (nz = 0 past the last task).
+struct PlaneTask {
+  uint32_t x, y, z0, nz;
+};
+
+__device__ __forceinline__ PlaneTask plane_task(uint32_t X, uint32_t Y, uint32_t Z, uint64_t task) {
+  const uint32_t ncc = (X + 63u) / 64u;
+  PlaneTask t;
+  t.x = (uint32_t)(task % ncc) * 64u + (threadIdx.x & 63u);
+  const uint64_t rest = task / ncc;
+  t.y = (uint32_t)(rest % Y);
+  const uint64_t zb = rest / Y;
+  t.z0 = (uint32_t)min<uint64_t>(zb * kGenPlanes, Z);
+  t.nz = min(kGenPlanes, Z - t.z0);
+  return t;
+}
+
 // Blob sums: a lane keeps amp[b] * ex[b][x] for every blob in registers and,
 // row by row, adds ((amp*ex) * ey) * ez blob by blob, the same operations in
 // the same order as the CPU generator.  Blobs go in groups of 8 (one group
 // of row factors is one wide scalar load); the zero entries that pad the
 // last group add +0.0 to a sum that starts at +0.0 and so can never be -0.0,
-// which leaves it unchanged.  Needs ldb == kBlobRegs.
+// which leaves it unchanged.  Needs ldb == kBlobRegs.  (Keeping the products
+// with ey per task instead, over planes of one row, issues a third fewer
+// FP64 operations but ran no faster.)
 __device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
-  const GenTask t = gen_task(a.X, a.Y, a.Z, task);
+  const RowTask t = row_task(a.X, a.Y, a.Z, task);
   if (t.nr == 0) return;
   const uint32_t xc = t.x < a.X ? t.x : a.X - 1u;
   const uniform_dptr amp = as_uniform(a.amp);
@@ -1947,7 +1975,7 @@ __device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
   float *out = a.out + t.r0 * a.X + t.x;
   for (uint32_t rr = 0; rr < t.nr; ++rr) {
     uint32_t y, z;
-    gen_row_yz(t, a.Y, rr, y, z);
+    row_yz(t, a.Y, rr, y, z);
     const uniform_dptr ey = as_uniform(a.ey + (size_t)y * kBlobRegs);
     const uniform_dptr ez = as_uniform(a.ez + (size_t)z * kBlobRegs);
     double acc = 0.0;
@@ -1983,17 +2011,6 @@ __device__ __forceinline__ void blob_points(const BlobArgs &a, size_t i0, size_t
   }
 }
 
-// Grid: ceil(gen_tasks / kGenWaves) workgroups when nblobs <= kBlobRegs,
-// else any grid over points.
-__global__ __launch_bounds__(kGenThreads) void k_field_blobs(BlobArgs a) {
-  if (a.nblobs <= kBlobRegs) {
-    const uint64_t task = (uint64_t)blockIdx.x * kGenWaves + wave_id();
-    if (task < gen_tasks(a.X, a.Y, a.Z)) blob_task(a, task);
-  } else {
-    blob_points(a, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
-  }
-}
-
 struct GyroidArgs {
   float *out;
   uint32_t X, Y, Z;
@@ -2008,31 +2025,33 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// Gyroid plus hash noise of point id = x + X * r:
-// sx*cy + sy*cz + sz*cx from the 1-D tables, in that order, then the noise.
+// Gyroid plus hash noise of point id = x + X * (y + Y * z):
+// sx*cy + sy*cz + sz*cx from the 1-D tables, in that order, then the noise of
+// mix64(seed + (id + 1) * golden).  Along the task's planes id grows by X*Y,
+// so the hash input grows by X*Y*golden (mod 2^64).
 __device__ __forceinline__ void gyroid_task(const GyroidArgs &a, uint64_t task) {
-  const GenTask t = gen_task(a.X, a.Y, a.Z, task);
-  if (t.x >= a.X) return;
+  constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+  const PlaneTask t = plane_task(a.X, a.Y, a.Z, task);
+  if (t.x >= a.X || t.nz == 0) return;
   const double sxx = a.sx[t.x], cxx = a.cx[t.x];
-  const uniform_dptr sy = as_uniform(a.sy), cy = as_uniform(a.cy), sz = as_uniform(a.sz), cz = as_uniform(a.cz);
-  for (uint32_t rr = 0; rr < t.nr; ++rr) {
-    uint32_t y, z;
-    gen_row_yz(t, a.Y, rr, y, z);
-    const double gval = sxx * cy[y] + sy[y] * cz[z] + sz[z] * cxx;
-    const uint64_t id = (t.r0 + rr) * a.X + t.x;
-    const uint64_t h = mix64(a.seed + (id + 1) * 0x9E3779B97F4A7C15ull);
+  const uniform_dptr sz = as_uniform(a.sz), cz = as_uniform(a.cz);
+  const double syy = as_uniform(a.sy)[t.y];
+  const double t1 = sxx * as_uniform(a.cy)[t.y];
+  const uint64_t plane = (uint64_t)a.X * a.Y;
+  const uint64_t id0 = (uint64_t)t.z0 * plane + (uint64_t)t.y * a.X + t.x;
+  const uint64_t step = plane * kGolden;
+  uint64_t hin = a.seed + (id0 + 1) * kGolden;
+  float *out = a.out + id0;
+  for (uint32_t kz = 0; kz < t.nz; ++kz, hin += step) {
+    const uint32_t z = t.z0 + kz;
+    const double gval = t1 + syy * cz[z] + sz[z] * cxx;
+    const uint64_t h = mix64(hin);
     const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
-    a.out[id] = (float)(gval + a.amp * (2.0 * u - 1.0));
+    out[kz * plane] = (float)(gval + a.amp * (2.0 * u - 1.0));
   }
 }
 
-// Grid: ceil(gen_tasks / kGenWaves) workgroups.
-__global__ __launch_bounds__(kGenThreads) void k_field_gyroid(GyroidArgs a) {
-  const uint64_t task = (uint64_t)blockIdx.x * kGenWaves + wave_id();
-  if (task < gen_tasks(a.X, a.Y, a.Z)) gyroid_task(a, task);
-}
-
-// Many fields in one launch (the generators above, same arithmetic).
+// The generator kernel: the fields of one or many volumes in one launch.
 // Workgroup b belongs to the volume whose range [block_off, block_off +
 // nblocks) holds it, so its waves share the volume and its kind.
 struct GenVol {
@@ -2056,13 +2075,13 @@ __global__ __launch_bounds__(kGenThreads) void k_field_batch(const GenVol *vols,
   if (v.kind == 1) {
     const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, blob_ldb(v.nblobs), v.t[0], v.t[1], v.t[2], v.t[3]};
     if (v.nblobs <= kBlobRegs) {
-      if (task < gen_tasks(v.X, v.Y, v.Z)) blob_task(ba, task);
+      if (task < blob_tasks(v.X, v.Y, v.Z)) blob_task(ba, task);
     } else {
       blob_points(ba, (size_t)local * blockDim.x + threadIdx.x, (size_t)v.nblocks * blockDim.x);
     }
     return;
   }
-  if (task < gen_tasks(v.X, v.Y, v.Z)) {
+  if (task < gyroid_tasks(v.X, v.Y, v.Z)) {
     const GyroidArgs ga{v.out, v.X, v.Y, v.Z, v.t[0], v.t[1], v.t[2], v.t[3], v.t[4], v.t[5], v.seed, v.amp};
     gyroid_task(ga, task);
   }

@@ -12,19 +12,22 @@
 // has room for its output: none is rejected after its count.  A request whose
 // footprint exceeds the whole budget is rejected when it is submitted.
 //
-// Progress: a worker thread owned by the server drives the lanes (every
-// phase change is noticed within microseconds, whatever the caller is
-// doing); submit, poll, release and the rest take the server's mutex and
-// wake it.
+// Progress: a worker thread owned by the server schedules (every phase
+// change is noticed within microseconds, whatever the caller is doing);
+// submit, poll, release and the rest take the server's mutex and wake it.
 //
-// Scheduling: `num_streams` lanes, each with its own stream and extraction
-// context.  An idle lane takes the queued requests in arrival order (up to
-// max_batch, as long as their regions can be carved), generates their
-// fields, counts and emits them, and copies the counts back; the worker that
-// sees that sequence finish trims the outputs, frees fields and scratch and
-// hands out the results.  A request of at least `solo_cells` cells runs
-// alone on a lane through the single-volume extraction instead (a batch stops
-// before it, so arrival order holds).  Lanes run independently, so a batch started for
+// Scheduling: `num_streams` lanes, each with its own stream, extraction
+// context and launcher thread.  For an idle lane the worker takes the queued
+// requests in arrival order (up to max_batch, as long as their regions can be
+// carved) and hands the batch to the lane's thread, which generates the
+// fields, counts and emits them and queues the copy of the counts back (the
+// generator tables and the launches cost host time per request, so the lanes
+// do this in parallel and outside the server's mutex); the worker that sees
+// that sequence finish trims the outputs, frees fields and scratch and hands
+// out the results.  A request of at least `solo_cells` cells runs alone on a
+// lane through the single-volume extraction instead; batches take the
+// smaller requests queued behind it, and it starts on the next free lane
+// once it heads the queue.  Lanes run independently, so a batch started for
 // late arrivals overlaps with the batches already on the device.
 #include <hip/hip_runtime.h>
 
@@ -74,9 +77,14 @@ struct Req {
 struct Batch {
   std::vector<Req *> reqs;
   std::vector<mdmc_volume> vols;
-  uint64_t scratch_off = kNoOffset;
+  uint64_t scratch_off = kNoOffset, scratch_bytes = 0;
+  void *base_ptr = nullptr;  // the scratch block's device address
+  bool solo = false;         // one request, through the single-volume extraction
 };
 
+// A lane: a stream and an extraction context, and a thread of its own that
+// issues the work of the batches handed to it (building generator tables and
+// launching kernels cost host time per request; lanes do it in parallel).
 struct Lane {
   hipStream_t stream = nullptr;
   mdmc_ctx *ctx = nullptr;
@@ -84,6 +92,13 @@ struct Lane {
   mdmc_counts *h_counts = nullptr;  // pinned
   hipEvent_t ev = nullptr;          // end of the batch's sequence
   Batch *batch = nullptr;
+  // phase (guarded by the server's mutex): 0 idle, 1 handed to the lane's
+  // thread, 2 launched (launch_rc says whether the launches went through)
+  int phase = 0;
+  int launch_rc = MDMC_OK;
+  std::thread thread;
+  std::condition_variable cv;  // with the server's mutex: a batch was handed over, or stop
+  std::mutex ctx_mu;           // held by the lane's thread while it uses ctx and stream
 };
 
 // MDMC_SERVER_TRACE=1: one line per scheduling event on stderr.
@@ -128,7 +143,7 @@ struct mdmc_server {
   mdmc_server_config cfg{};
   char *base = nullptr;
   mdmc::Pool pool;
-  std::vector<Lane> lanes;
+  std::deque<Lane> lanes;  // (a deque: lanes hold a mutex and never move)
   std::deque<Req *> queue;
   std::unordered_map<uint64_t, Req *> reqs;
   std::deque<Req *> finished;  // not yet returned by poll
@@ -232,11 +247,13 @@ void retire_batch(mdmc_server *s, Lane &lane) {
   pool_free(s, &b->scratch_off);
   delete b;
   lane.batch = nullptr;
+  lane.phase = 0;
 }
 
 // Ends every request of the lane's batch with `rc` after anything already
 // queued on the lane's stream has run (its kernels may still use the
-// batch's regions, which go back to the pool here).
+// batch's regions, which go back to the pool here).  The lane's thread must
+// be done with the batch (phase 2).
 void fail_batch(mdmc_server *s, Lane &lane, int rc) {
   if (hipStreamSynchronize(lane.stream) != hipSuccess) (void)hipGetLastError();
   for (Req *r : lane.batch->reqs) finish(s, r, rc);
@@ -267,9 +284,9 @@ void uncarve_request(mdmc_server *s, Req *r) {
   r->quads = nullptr;
 }
 
-// Takes a carved request off the head of the queue into batch `b`.
+// Takes a carved request out of the queue into batch `b`.
 void admit(mdmc_server *s, Batch *b, Req *r) {
-  s->queue.pop_front();
+  s->queue.erase(std::find(s->queue.begin(), s->queue.end(), r));
   r->state = 1;
   r->t_start = s->now();
   r->field = r->spec.field_kind != MDMC_FIELD_DEVICE ? reinterpret_cast<const float *>(s->base + r->field_off)
@@ -301,15 +318,46 @@ std::vector<mdmc_gen_spec> gen_specs(const Batch *b) {
   return gens;
 }
 
-// Queues the counts' copy to the host and the end-of-sequence event.
-int finish_launch(Lane &lane, uint32_t n) {
-  if (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
-                     lane.stream) != hipSuccess ||
-      hipEventRecord(lane.ev, lane.stream) != hipSuccess) {
-    (void)hipGetLastError();
-    return MDMC_ERR_HIP;
+// Gives a formed batch to an idle lane's thread.
+void hand_over(mdmc_server *s, Lane &lane, Batch *b) {
+  lane.batch = b;
+  lane.phase = 1;
+  lane.launch_rc = MDMC_OK;
+  s->batches += 1;
+  lane.cv.notify_one();
+}
+
+// The batch's work on the lane's stream, issued by the lane's thread without
+// the server's mutex: generate the fields, count and emit (the outputs have
+// their worst-case size, so no host round trip between them), copy the
+// counts back, record the end event.
+int launch(Lane &lane, const Batch *b) {
+  const uint32_t n = (uint32_t)b->reqs.size();
+  const std::vector<mdmc_gen_spec> gens = gen_specs(b);
+  // the context works in the batch's scratch (none: a batch of empty volumes,
+  // which needs none; the context then never touches a stale block)
+  int rc = mdmc_ctx_set_scratch(lane.ctx, b->scratch_bytes ? b->base_ptr : nullptr, b->scratch_bytes);
+  if (rc == MDMC_OK && !gens.empty()) {
+    rc = mdmc_batch_generate_device(lane.ctx, gens.data(), (uint32_t)gens.size(), lane.stream);
+    TRACE("  generate %zu rc %d", gens.size(), rc);
   }
-  return MDMC_OK;
+  if (rc == MDMC_OK && b->solo) {
+    const Req *r = b->reqs[0];
+    rc = mdmc_extract_device(lane.ctx, r->field, r->spec.dims, r->spec.xyz_min, r->spec.xyz_max, r->spec.iso,
+                             r->verts, r->vmax, r->quads, r->qmax, lane.d_counts, lane.stream);
+    TRACE("  extract alone rc %d", rc);
+  } else if (rc == MDMC_OK) {
+    rc = mdmc_batch_extract_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
+    TRACE("  extract rc %d", rc);
+  }
+  if (rc == MDMC_OK &&
+      (hipMemcpyAsync(lane.h_counts, lane.d_counts, n * sizeof(mdmc_counts), hipMemcpyDeviceToHost,
+                      lane.stream) != hipSuccess ||
+       hipEventRecord(lane.ev, lane.stream) != hipSuccess)) {
+    (void)hipGetLastError();
+    rc = MDMC_ERR_HIP;
+  }
+  return rc;
 }
 
 // Starts the request at the head of the queue alone on an idle lane, through
@@ -331,32 +379,32 @@ int start_solo(mdmc_server *s, Lane &lane) {
   admit(s, b, r);
   TRACE("start %llu alone on stream %p (scratch %llu, in use %llu)", (unsigned long long)r->id,
         (void *)lane.stream, (unsigned long long)sbytes, (unsigned long long)s->pool.in_use());
-  lane.batch = b;
-  s->batches += 1;
-  const std::vector<mdmc_gen_spec> gens = gen_specs(b);
-  int rc = mdmc_ctx_set_scratch(lane.ctx, sbytes ? s->base + b->scratch_off : nullptr, sbytes);
-  if (rc == MDMC_OK && !gens.empty()) rc = mdmc_batch_generate_device(lane.ctx, gens.data(), 1, lane.stream);
-  if (rc == MDMC_OK)
-    rc = mdmc_extract_device(lane.ctx, r->field, r->spec.dims, r->spec.xyz_min, r->spec.xyz_max, r->spec.iso,
-                             r->verts, r->vmax, r->quads, r->qmax, lane.d_counts, lane.stream);
-  TRACE("  extract alone rc %d", rc);
-  if (rc == MDMC_OK) rc = finish_launch(lane, 1);
-  if (rc != MDMC_OK) fail_batch(s, lane, rc);
+  b->solo = true;
+  b->scratch_bytes = sbytes;
+  b->base_ptr = sbytes ? s->base + b->scratch_off : nullptr;
+  hand_over(s, lane, b);
   return MDMC_OK;
 }
 
 // Starts a batch on an idle lane from the head of the queue.  Requests join
 // in arrival order while their regions and the scratch of the batch with
 // them can be carved (the scratch is re-carved for each candidate), so a
-// started batch always holds everything it needs.
+// started batch always holds everything it needs.  Requests that run alone
+// are passed over (each starts on a free lane once it heads the queue, and
+// while it heads the queue and does not fit, no batch starts either, so it
+// is never starved of memory).
 int start_batch(mdmc_server *s, Lane &lane) {
   if (!s->queue.empty() && s->queue.front()->solo) return start_solo(s, lane);
   Batch *b = new (std::nothrow) Batch();
   if (!b) return MDMC_ERR_NOMEM;
   uint64_t sbytes = 0;
-  while (!s->queue.empty() && b->reqs.size() < s->cfg.max_batch) {
-    Req *r = s->queue.front();
-    if (r->solo) break;  // it starts on the next idle lane (arrival order)
+  size_t qi = 0;
+  while (qi < s->queue.size() && b->reqs.size() < s->cfg.max_batch) {
+    Req *r = s->queue[qi];
+    if (r->solo) {  // runs alone once it heads the queue; smaller ones may pass it
+      ++qi;
+      continue;
+    }
     if (!carve_request(s, r)) break;  // wait for memory (arrival order)
     mdmc_volume v{};
     std::memcpy(v.dims, r->spec.dims, sizeof(v.dims));
@@ -399,23 +447,9 @@ int start_batch(mdmc_server *s, Lane &lane) {
   }
   TRACE("start batch of %u on stream %p (scratch %llu, in use %llu)", n, (void *)lane.stream,
         (unsigned long long)sbytes, (unsigned long long)s->pool.in_use());
-  lane.batch = b;
-  s->batches += 1;
-  const std::vector<mdmc_gen_spec> gens = gen_specs(b);
-  // the lane's context works in the batch's scratch (none: a batch of empty
-  // volumes, which needs none; the context then never touches a stale block)
-  int rc = mdmc_ctx_set_scratch(lane.ctx, sbytes ? s->base + b->scratch_off : nullptr, sbytes);
-  if (rc == MDMC_OK && !gens.empty()) {
-    rc = mdmc_batch_generate_device(lane.ctx, gens.data(), (uint32_t)gens.size(), lane.stream);
-    TRACE("  generate %zu rc %d", gens.size(), rc);
-  }
-  // count and emit in one sequence: the outputs have their worst-case size
-  if (rc == MDMC_OK) {
-    rc = mdmc_batch_extract_device(lane.ctx, b->vols.data(), n, lane.d_counts, lane.stream);
-    TRACE("  extract rc %d", rc);
-  }
-  if (rc == MDMC_OK) rc = finish_launch(lane, n);
-  if (rc != MDMC_OK) fail_batch(s, lane, rc);
+  b->scratch_bytes = sbytes;
+  b->base_ptr = sbytes ? s->base + b->scratch_off : nullptr;
+  hand_over(s, lane, b);
   return MDMC_OK;
 }
 
@@ -424,7 +458,12 @@ bool pump(mdmc_server *s) {
   bool progress = false;
   for (Lane &lane : s->lanes) {
     Batch *b = lane.batch;
-    if (!b) continue;
+    if (!b || lane.phase != 2) continue;  // idle, or its thread is still launching
+    if (lane.launch_rc != MDMC_OK) {
+      progress = true;
+      fail_batch(s, lane, lane.launch_rc);
+      continue;
+    }
     const hipError_t q = hipEventQuery(lane.ev);
     if (q == hipErrorNotReady) continue;
     progress = true;
@@ -485,13 +524,20 @@ void worker_loop(mdmc_server *s) {
     bool moved = false;
     while (pump(s)) moved = true;
     watch.clear();
-    for (const Lane &l : s->lanes)
-      if (l.batch) watch.push_back(l.ev);
+    bool launching = false;
+    for (const Lane &l : s->lanes) {
+      if (l.batch && l.phase == 2) watch.push_back(l.ev);
+      launching = launching || (l.batch && l.phase == 1);
+    }
     if (moved) s->cv_done.notify_all();
     if (watch.empty()) {
-      s->idle = true;
-      s->idle_epoch = seen;
-      s->cv_done.notify_all();
+      // nothing on the device to watch: wait for a call or for a lane's
+      // thread to finish launching (either bumps the epoch)
+      if (!launching) {
+        s->idle = true;
+        s->idle_epoch = seen;
+        s->cv_done.notify_all();
+      }
       s->cv_work.wait(lk, [&] { return s->stop.load() || s->epoch.load() != seen; });
       continue;
     }
@@ -517,6 +563,28 @@ void wake(mdmc_server *s) {
   s->cv_work.notify_one();
 }
 
+// A lane's thread: launches each batch handed to it, then reports back to
+// the worker (which watches the batch's end event).
+void lane_loop(mdmc_server *s, Lane *lane) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    lane->cv.wait(lk, [&] { return s->stop.load() || lane->phase == 1; });
+    if (lane->phase != 1) break;  // stopping
+    const Batch *b = lane->batch;
+    lk.unlock();
+    int rc;
+    {
+      std::lock_guard<std::mutex> cl(lane->ctx_mu);
+      rc = launch(*lane, b);
+    }
+    lk.lock();
+    lane->launch_rc = rc;
+    lane->phase = 2;
+    wake(s);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -538,7 +606,7 @@ int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out) {
     return MDMC_ERR_NOMEM;
   }
   s->pool.reset(cfg->memory_budget);
-  s->lanes.resize(cfg->num_streams);
+  for (uint32_t i = 0; i < cfg->num_streams; ++i) s->lanes.emplace_back();
   for (Lane &l : s->lanes) {
     bool ok = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&l.ev, hipEventDisableTiming) == hipSuccess &&
@@ -558,6 +626,7 @@ int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out) {
     return MDMC_ERR_HIP;
   }
   try {
+    for (Lane &l : s->lanes) l.thread = std::thread(lane_loop, s, &l);
     s->worker = std::thread(worker_loop, s);
   } catch (...) {
     mdmc_server_destroy(s);
@@ -569,14 +638,15 @@ int mdmc_server_create(const mdmc_server_config *cfg, mdmc_server **out) {
 
 void mdmc_server_destroy(mdmc_server *s) {
   if (!s) return;
-  if (s->worker.joinable()) {
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->stop.store(true);
-    }
-    s->cv_work.notify_all();
-    s->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop.store(true);
+    for (Lane &l : s->lanes) l.cv.notify_all();
   }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (Lane &l : s->lanes)
+    if (l.thread.joinable()) l.thread.join();
   for (Lane &l : s->lanes) {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     if (l.batch) delete l.batch;
@@ -707,6 +777,7 @@ int mdmc_server_set_option(mdmc_server *s, int option, int value) {
     return MDMC_OK;
   }
   for (Lane &l : s->lanes) {
+    std::lock_guard<std::mutex> cl(l.ctx_mu);  // not while the lane's thread launches
     const int rc = mdmc_ctx_set_option(l.ctx, option, value);
     if (rc != MDMC_OK) return rc;
   }

@@ -212,13 +212,17 @@ def test_field_generators_match_cpu(ctx, torch_cuda):
     cg = oracle.field_gyroid(dims, 24.0, 12, 0.05)
     assert np.array_equal(gb.cpu().numpy().view(np.uint32), cb.view(np.uint32))
     assert np.array_equal(gg.cpu().numpy().view(np.uint32), cg.view(np.uint32))
-    # several groups of 8 blobs (the last one padded) and tasks whose row
-    # count is odd (29 * 23 rows is 11 past a multiple of 16)
-    for dims, nb in (((67, 29, 23), 27), ((130, 17, 9), 32), ((5, 3, 7), 17)):
+    # several groups of 8 blobs (the last one padded), more blobs than the
+    # registers hold (the per-point path), and generator tasks (64 columns of
+    # one row in up to 16 planes) cut short in x and in z (23 planes are 16 + 7)
+    for dims, nb in (((67, 29, 23), 27), ((130, 17, 9), 32), ((5, 3, 7), 17), ((70, 9, 37), 40)):
         gb = ctx.field_blobs(dims, seed=nb, nblobs=nb, sigma=(2.0, 9.0))
+        gg = ctx.field_gyroid(dims, period=7.0, seed=nb, noise_amp=0.3)
         torch_cuda.cuda.synchronize()
         cb = oracle.field_blobs(dims, nb, nb, sigma=(2.0, 9.0))
+        cg = oracle.field_gyroid(dims, 7.0, nb, 0.3)
         assert np.array_equal(gb.cpu().numpy().view(np.uint32), cb.view(np.uint32)), (dims, nb)
+        assert np.array_equal(gg.cpu().numpy().view(np.uint32), cg.view(np.uint32)), (dims, nb)
 
 
 def test_host_entry_matches_device(ctx, torch_cuda):

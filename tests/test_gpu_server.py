@@ -221,7 +221,7 @@ def test_served_workload_stream(fa_mod, torch_cuda):
     to_oracle = set(order[:8].tolist()) | set(order[[80, 160, 200, len(reqs) - 1]].tolist())
     assert d.min() >= 48 and d.max() <= 384 and d.max() > 300
     budget = 64 << 30
-    srv = Server(budget, max_batch=32, num_streams=4)
+    srv = Server(budget, max_batch=32, num_streams=8)
     gen, sctx = fa_mod.Context(), fa_mod.Context()
     oracle.set_threads(8)
     seen = {"single": 0, "oracle": 0}
