@@ -151,23 +151,38 @@ def test_device_field_requests(fa_mod, torch_cuda):
 
 
 def test_clients_on_several_threads(fa_mod, torch_cuda):
-    """Four host threads submit while the caller's thread waits on poll: the
-    server's worker drives the lanes, every request comes back once, and the
-    results equal their single extractions; a wait with nothing in flight
-    returns at once."""
+    """Four host threads submit while the caller's thread waits on poll and a
+    fifth keeps changing launch options and the size that runs alone (options
+    wait for a lane's launcher to finish with its context): the server's
+    threads drive the lanes, every request comes back once, and the results
+    equal their single extractions; a wait with nothing in flight returns at
+    once."""
     import threading
+    import time
 
-    from framework_amd.server import Server
+    from framework_amd.mdmc import OPT_BATCH_PLANES, OPT_EMIT_BLOCKS_PER_CU
+    from framework_amd.server import OPT_SOLO_CELLS, Server
 
     reqs = _requests(32, seed=13, dmin=16, dmax=64)
     srv = Server(1 << 30, max_batch=4, num_streams=3)
     ids = [None] * len(reqs)
+    stop = threading.Event()
 
     def client(k0):
         for k in range(k0, len(reqs), 4):
             ids[k] = srv.submit(reqs[k])
 
+    def tuner():
+        i = 0
+        while not stop.is_set():
+            srv.set_options({OPT_EMIT_BLOCKS_PER_CU: (0, 4, 16)[i % 3], OPT_BATCH_PLANES: (0, 1, 2)[i % 3],
+                             OPT_SOLO_CELLS: (0, 30 ** 3)[i % 2]})
+            i += 1
+            time.sleep(2e-4)
+
     threads = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+    opt_thread = threading.Thread(target=tuner)
+    opt_thread.start()
     for t in threads:
         t.start()
     got, meshes = {}, {}
@@ -180,6 +195,8 @@ def test_clients_on_several_threads(fa_mod, torch_cuda):
             srv.release(r["id"])
     for t in threads:
         t.join()
+    stop.set()
+    opt_thread.join()
     assert sorted(got) == sorted(ids)
     assert all(r["status"] == 0 for r in got.values())
     assert srv.poll(wait=True) == []
