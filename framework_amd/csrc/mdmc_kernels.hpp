@@ -834,7 +834,7 @@ __global__ __launch_bounds__(256) void k_coords(CoordArgs c) {
 // segment holding its first cell; the grand totals.
 
 constexpr int kScanThreads = 256;
-constexpr int kScanIpt = 16;  // segments per thread
+constexpr int kScanIpt = 8;  // segments per thread (16: 1 us slower per 512^3 extraction)
 constexpr int kScanTile = kScanThreads * kScanIpt;
 constexpr uint32_t kSpinLimit = 1u << 24;
 
