@@ -30,9 +30,9 @@ from .mdmc import (OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS
                    OPT_PT_PLANES, OPT_PT_ROWS, MDMCError)
 
 SINGLE_SPACE = {OPT_PT_ROWS: [0, 15], OPT_PT_PLANES: [0, 1, 2, 3], OPT_PT_BAND: [0, 2, 4],
-                OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}
+                OPT_EMIT_BLOCKS_PER_CU: [0, 8, 32, 256]}
 BATCH_SPACE = {OPT_BATCH_PLANES: [0, 1, 2, 4], OPT_CLASSIFY_BLOCKS_PER_CU: [0, 4, 8, 12],
-               OPT_EMIT_BLOCKS_PER_CU: [0, 4, 16]}
+               OPT_EMIT_BLOCKS_PER_CU: [0, 8, 32, 256]}
 OPTION_NAMES = {OPT_BATCH_PLANES: "batch_planes", OPT_CLASSIFY_BLOCKS_PER_CU: "classify_blocks_per_cu",
                 OPT_EMIT_BLOCKS_PER_CU: "emit_blocks_per_cu", OPT_PT_PLANES: "pt_planes", OPT_PT_BAND: "pt_band",
                 OPT_PT_ROWS: "pt_rows"}

@@ -546,9 +546,9 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
       std::memcpy(ctx->coords_box + 3, mx, 3 * sizeof(float));
       ctx->coords_valid = true;
     }
-    // one wave per chunk of 64 active cells, at most 8 workgroups per CU
+    // one wave per chunk of 64 active cells, at most kEmitBlocksPerCu workgroups per CU
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
-    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitBlocksPerCu;
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
     hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
   }
@@ -570,7 +570,7 @@ int launch_smooth(mdmc_ctx *ctx, float *d_vertices, uint64_t vcap, int32_t *d_qu
   ea.v.qcap = d_quads ? qcap : 0;
   ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
   const uint64_t want = (ctx->plan.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
-  const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+  const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitBlocksPerCu;
   const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
   ProfScope ps(ctx, 3, st);
   for (uint32_t it = 0; it < num_smooth; ++it) {
@@ -934,7 +934,7 @@ int batch_emit(mdmc_ctx *ctx, const mdmc_volume *vols, uint32_t n, mdmc_counts *
   be.nvol = n;
   {
     ProfScope ps(ctx, 2, st);
-    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitBlocksPerCu;
     const uint64_t want = (agg.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
     hipLaunchKernelGGL(k_coords_batch, dim3(std::min<uint32_t>(n, (uint32_t)ctx->num_cus * 4)), dim3(256), 0, st,
@@ -1761,7 +1761,7 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0,
                        nullptr, co);
     const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
-    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : 8;
+    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitBlocksPerCu;
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
     hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, nullptr, ea);
     if (hipGetLastError() != hipSuccess) return bail(MDMC_ERR_HIP, "slab emit launch");

@@ -1363,18 +1363,35 @@ __device__ __forceinline__ ChunkIter chunk_iter(uint32_t n, uint32_t wave) {
   return ChunkIter{lo + slot * wpb + wave, hi, gx * wpb};
 }
 
+// Emit grid: at most this many workgroups per CU (MDMC_OPT_EMIT_BLOCKS_PER_CU
+// overrides it).  The grid is sized for the chunk capacity because the count
+// is still on the device when the emit is launched; on a surface-dense 1024^3
+// field 128 per CU beat 8 by 8% (more, shorter-lived workgroups keep each
+// XCD's chunks in dispatch order), and past 256 the workgroups that find no
+// chunk cost a sparse extraction several microseconds.
+constexpr int kEmitBlocksPerCu = 128;
+
+// Whether any wave of this workgroup has a chunk (its wave 0 has the lowest
+// one).  The grid is sized for the chunk capacity, so when the mesh is
+// sparse most workgroups leave at once, before loading their tables.
+__device__ __forceinline__ bool workgroup_has_chunks(uint32_t n) {
+  const ChunkIter it = chunk_iter(n, 0);
+  return it.next < it.end;
+}
+
 // 8 waves per SIMD: the emit is latency-bound, the register cap costs a few
 // scalar spills only
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_emit(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
-  for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
-  __syncthreads();
-  const uint32_t lane = lane_id(), wave = wave_id();
   const Totals tot = *a.totals;
   // Counts are clamped to buffer capacities so that no access can leave the
   // scratch buffers even if the count pass produced inconsistent data.
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  if (!workgroup_has_chunks(num_chunks)) return;  // (uniform: before the barrier)
+  for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
+  __syncthreads();
+  const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
   const ChunkIter it = chunk_iter(num_chunks, wave);
   for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
@@ -1656,11 +1673,12 @@ __global__ __launch_bounds__(kEmitThreads) void k_smooth(EmitArgs a) {
 __global__ __launch_bounds__(kEmitThreads) void k_reaverage(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
+  const Totals tot = *a.totals;
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  if (!workgroup_has_chunks(num_chunks)) return;  // (uniform: before the barrier)
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   __syncthreads();
   const uint32_t lane = lane_id(), wave = wave_id();
-  const Totals tot = *a.totals;
-  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
   const ChunkIter it = chunk_iter(num_chunks, wave);
   for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
@@ -1766,13 +1784,14 @@ __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8,
   // first segment of each volume (the search below reads entries 1 .. nvol-1;
   // kMaxBatch entries keep the workgroup's LDS at 20 KB, 8 workgroups per CU)
   __shared__ uint32_t s_soff[kMaxBatch];
+  const EmitArgs &a = b.a;
+  const Totals tot = *a.totals;
+  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
+  if (!workgroup_has_chunks(num_chunks)) return;  // (uniform: before the barrier)
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   for (uint32_t i = threadIdx.x; i < b.nvol; i += kEmitThreads) s_soff[i] = b.vols[i].seg_off;
   __syncthreads();
-  const EmitArgs &a = b.a;
   const uint32_t lane = lane_id(), wave = wave_id();
-  const Totals tot = *a.totals;
-  const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
   const ChunkIter it = chunk_iter(num_chunks, wave);
   for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
