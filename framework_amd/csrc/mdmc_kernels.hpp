@@ -1890,17 +1890,18 @@ struct BlobArgs {
   float *out;
   uint32_t X, Y, Z, nblobs, ldb;
   const double *ex, *ey, *ez, *amp;
+  uint32_t per;  // rows per task
 };
 
 // Generators work in wave tasks of 64 consecutive columns x; a lane owns one
 // column and the loop over the task's rows is uniform over the wave, so
 // per-row values are scalar loads and the stores of a row are one contiguous
-// 256-byte run.  Blob tasks take kGenRows consecutive rows r = y + Y*z;
-// gyroid tasks take one row y in kGenPlanes consecutive planes z (what
-// depends on x and y only is then computed once per task).  Consecutive
-// tasks take consecutive column blocks first.
-constexpr uint32_t kGenRows = 16;
-constexpr uint32_t kGenPlanes = 16;
+// 256-byte run.  Blob tasks take `per` consecutive rows r = y + Y*z; gyroid
+// tasks take one row y in `per` consecutive planes z (what depends on x and y
+// only is then computed once per task).  Consecutive tasks take consecutive
+// column blocks first.  `per` is at most kGenPer, fewer for a field too small
+// to give the chip enough waves (a task's rows run one after another).
+constexpr uint32_t kGenPer = 16;
 constexpr uint32_t kGenThreads = 256;
 constexpr uint32_t kGenWaves = kGenThreads / kWave;
 constexpr uint32_t kBlobRegs = 32;  // blobs kept in registers (more: per-point path)
@@ -1910,11 +1911,11 @@ __host__ __device__ inline uint32_t blob_ldb(uint32_t nblobs) {
   return nblobs <= kBlobRegs ? kBlobRegs : nblobs;
 }
 
-__host__ __device__ inline uint64_t blob_tasks(uint32_t X, uint32_t Y, uint32_t Z) {
-  return (uint64_t)((X + 63u) / 64u) * (((uint64_t)Y * Z + kGenRows - 1u) / kGenRows);
+__host__ __device__ inline uint64_t blob_tasks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per) {
+  return (uint64_t)((X + 63u) / 64u) * (((uint64_t)Y * Z + per - 1u) / per);
 }
-__host__ __device__ inline uint64_t gyroid_tasks(uint32_t X, uint32_t Y, uint32_t Z) {
-  return (uint64_t)((X + 63u) / 64u) * Y * ((Z + kGenPlanes - 1u) / kGenPlanes);
+__host__ __device__ inline uint64_t gyroid_tasks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per) {
+  return (uint64_t)((X + 63u) / 64u) * Y * ((Z + per - 1u) / per);
 }
 
 // Wave-uniform reads through the constant address space compile to scalar
@@ -1929,13 +1930,13 @@ struct RowTask {
   uint64_t r0;
 };
 
-__device__ __forceinline__ RowTask row_task(uint32_t X, uint32_t Y, uint32_t Z, uint64_t task) {
+__device__ __forceinline__ RowTask row_task(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per, uint64_t task) {
   const uint32_t ncc = (X + 63u) / 64u;
   RowTask t;
   t.x = (uint32_t)(task % ncc) * 64u + (threadIdx.x & 63u);
-  t.r0 = (task / ncc) * kGenRows;
+  t.r0 = (task / ncc) * per;
   const uint64_t nrows = (uint64_t)Y * Z;
-  t.nr = t.r0 < nrows ? (uint32_t)min<uint64_t>(kGenRows, nrows - t.r0) : 0u;
+  t.nr = t.r0 < nrows ? (uint32_t)min<uint64_t>(per, nrows - t.r0) : 0u;
   t.y0 = (uint32_t)(t.r0 % Y);
   t.z0 = (uint32_t)(t.r0 / Y);
   return t;
@@ -1956,15 +1957,15 @@ struct PlaneTask {
   uint32_t x, y, z0, nz;
 };
 
-__device__ __forceinline__ PlaneTask plane_task(uint32_t X, uint32_t Y, uint32_t Z, uint64_t task) {
+__device__ __forceinline__ PlaneTask plane_task(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per, uint64_t task) {
   const uint32_t ncc = (X + 63u) / 64u;
   PlaneTask t;
   t.x = (uint32_t)(task % ncc) * 64u + (threadIdx.x & 63u);
   const uint64_t rest = task / ncc;
   t.y = (uint32_t)(rest % Y);
   const uint64_t zb = rest / Y;
-  t.z0 = (uint32_t)min<uint64_t>(zb * kGenPlanes, Z);
-  t.nz = min(kGenPlanes, Z - t.z0);
+  t.z0 = (uint32_t)min<uint64_t>(zb * per, Z);
+  t.nz = min(per, Z - t.z0);
   return t;
 }
 
@@ -1977,7 +1978,7 @@ __device__ __forceinline__ PlaneTask plane_task(uint32_t X, uint32_t Y, uint32_t
 // with ey per task instead, over planes of one row, issues a third fewer
 // FP64 operations but ran no faster.)
 __device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
-  const RowTask t = row_task(a.X, a.Y, a.Z, task);
+  const RowTask t = row_task(a.X, a.Y, a.Z, a.per, task);
   if (t.nr == 0) return;
   const uint32_t xc = t.x < a.X ? t.x : a.X - 1u;
   const uniform_dptr amp = as_uniform(a.amp);
@@ -2036,6 +2037,7 @@ struct GyroidArgs {
   const double *sx, *cx, *sy, *cy, *sz, *cz;
   uint64_t seed;
   double amp;
+  uint32_t per;  // planes per task
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -2050,7 +2052,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 // so the hash input grows by X*Y*golden (mod 2^64).
 __device__ __forceinline__ void gyroid_task(const GyroidArgs &a, uint64_t task) {
   constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
-  const PlaneTask t = plane_task(a.X, a.Y, a.Z, task);
+  const PlaneTask t = plane_task(a.X, a.Y, a.Z, a.per, task);
   if (t.x >= a.X || t.nz == 0) return;
   const double sxx = a.sx[t.x], cxx = a.cx[t.x];
   const uniform_dptr sz = as_uniform(a.sz), cz = as_uniform(a.cz);
@@ -2076,7 +2078,7 @@ __device__ __forceinline__ void gyroid_task(const GyroidArgs &a, uint64_t task) 
 struct GenVol {
   float *out;
   uint32_t X, Y, Z, kind;  // kind 1: blobs, 2: gyroid
-  uint32_t nblobs, block_off, nblocks, pad;
+  uint32_t nblobs, block_off, nblocks, per;  // per: rows (blobs) or planes (gyroid) per task
   const double *t[6];      // blobs: ex, ey, ez, amp; gyroid: sx, cx, sy, cy, sz, cz
   uint64_t seed;
   double amp;              // gyroid noise amplitude
@@ -2092,16 +2094,16 @@ __global__ __launch_bounds__(kGenThreads) void k_field_batch(const GenVol *vols,
   const uint32_t local = blockIdx.x - v.block_off;
   const uint64_t task = (uint64_t)local * kGenWaves + wave_id();
   if (v.kind == 1) {
-    const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, blob_ldb(v.nblobs), v.t[0], v.t[1], v.t[2], v.t[3]};
+    const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, blob_ldb(v.nblobs), v.t[0], v.t[1], v.t[2], v.t[3], v.per};
     if (v.nblobs <= kBlobRegs) {
-      if (task < blob_tasks(v.X, v.Y, v.Z)) blob_task(ba, task);
+      if (task < blob_tasks(v.X, v.Y, v.Z, v.per)) blob_task(ba, task);
     } else {
       blob_points(ba, (size_t)local * blockDim.x + threadIdx.x, (size_t)v.nblocks * blockDim.x);
     }
     return;
   }
-  if (task < gyroid_tasks(v.X, v.Y, v.Z)) {
-    const GyroidArgs ga{v.out, v.X, v.Y, v.Z, v.t[0], v.t[1], v.t[2], v.t[3], v.t[4], v.t[5], v.seed, v.amp};
+  if (task < gyroid_tasks(v.X, v.Y, v.Z, v.per)) {
+    const GyroidArgs ga{v.out, v.X, v.Y, v.Z, v.t[0], v.t[1], v.t[2], v.t[3], v.t[4], v.t[5], v.seed, v.amp, v.per};
     gyroid_task(ga, task);
   }
 }
