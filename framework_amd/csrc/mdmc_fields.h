@@ -11,8 +11,10 @@
  *   - Gaussian blobs are evaluated in separable form
  *       f(x,y,z) = sum_b ((a_b * ex_b[x]) * ey_b[y]) * ez_b[z]
  *     with 1-D factors ex_b[x] = exp(-(x-c_b)^2 / (2 sigma_b^2)) computed once
- *     on the host in double precision; the 3-D sum is plain double multiplies
- *     and adds in a fixed order (no FMA), then rounded to float;
+ *     on the host in double precision; blob by blob, in order, the product
+ *     (a_b * ex_b[x]) * ey_b[y] is rounded and then multiplied by ez_b[z] and
+ *     added to the sum in one fused multiply-add (C99 fma, the GPU's FMA:
+ *     both correctly rounded), then the sum is rounded to float;
  *   - the gyroid is sx[x]*cy[y] + sy[y]*cz[z] + sz[z]*cx[x] plus hash noise,
  *     again from host-computed 1-D sin/cos tables and fixed-order double math.
  *

@@ -1970,8 +1970,9 @@ __device__ __forceinline__ PlaneTask plane_task(uint32_t X, uint32_t Y, uint32_t
 }
 
 // Blob sums: a lane keeps amp[b] * ex[b][x] for every blob in registers and,
-// row by row, adds ((amp*ex) * ey) * ez blob by blob, the same operations in
-// the same order as the CPU generator.  Blobs go in groups of 8 (one group
+// row by row, adds ((amp*ex) * ey) * ez blob by blob with one FMA each
+// (fma((amp*ex) * ey, ez, sum)), the same operations in the same order as the
+// CPU generator.  Blobs go in groups of 8 (one group
 // of row factors is one wide scalar load); the zero entries that pad the
 // last group add +0.0 to a sum that starts at +0.0 and so can never be -0.0,
 // which leaves it unchanged.  Needs ldb == kBlobRegs.  (Keeping the products
@@ -2009,7 +2010,7 @@ __device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
           fz[i] = ez[8 * g + i];
         }
 #pragma unroll
-        for (uint32_t i = 0; i < 8; ++i) acc = acc + (ax[8 * g + i] * fy[i]) * fz[i];
+        for (uint32_t i = 0; i < 8; ++i) acc = __builtin_fma(ax[8 * g + i] * fy[i], fz[i], acc);
       }
     }
     if (t.x < a.X) out[(size_t)rr * a.X] = (float)acc;
@@ -2025,8 +2026,8 @@ __device__ __forceinline__ void blob_points(const BlobArgs &a, size_t i0, size_t
     const uint32_t y = (uint32_t)(r % a.Y), z = (uint32_t)(r / a.Y);
     double acc = 0.0;
     for (uint32_t b = 0; b < a.nblobs; ++b)
-      acc = acc + ((a.amp[b] * a.ex[(size_t)b * a.X + x]) * a.ey[(size_t)y * a.ldb + b]) *
-                      a.ez[(size_t)z * a.ldb + b];
+      acc = __builtin_fma((a.amp[b] * a.ex[(size_t)b * a.X + x]) * a.ey[(size_t)y * a.ldb + b],
+                          a.ez[(size_t)z * a.ldb + b], acc);
     a.out[id] = (float)acc;
   }
 }

@@ -398,7 +398,8 @@ double mdmc_server_now(mdmc_server *srv);
 
 /* Sum of `nblobs` Gaussian blobs (centres uniform in the grid, sigma uniform
  * in [sigma_lo, sigma_hi], amplitude uniform in [amp_lo, amp_hi]) from a
- * SplitMix64 stream `seed`. */
+ * SplitMix64 stream `seed`; in double precision, each blob's term added with
+ * one fused multiply-add, rounded to float at the end. */
 int mdmc_field_blobs_device(mdmc_ctx *ctx, float *d_out, const uint32_t dims[3], uint64_t seed,
                             uint32_t nblobs, double sigma_lo, double sigma_hi, double amp_lo,
                             double amp_hi, void *stream);

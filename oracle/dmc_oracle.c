@@ -674,8 +674,8 @@ void oracle_field_blobs(float *out, const uint32_t dims[3], uint64_t seed, uint3
       for (x = 0; x < dims[0]; ++x) {
         double acc = 0.0;
         for (b = 0; b < nblobs; ++b)
-          acc = acc + ((amp[b] * ex[(size_t)b * dims[0] + x]) * ey[(size_t)b * dims[1] + y]) *
-                          ez[(size_t)b * dims[2] + z];
+          acc = fma((amp[b] * ex[(size_t)b * dims[0] + x]) * ey[(size_t)b * dims[1] + y],
+                    ez[(size_t)b * dims[2] + z], acc);
         out[((size_t)z * dims[1] + y) * dims[0] + x] = (float)acc;
       }
   free(ex); free(ey); free(ez); free(amp);
