@@ -258,6 +258,11 @@ def main() -> int:
                       "workload": "512^3 blobs (the headline field), voxel units (box 0..511)"}
         log(f"FP64 twin: {fp64_error}")
 
+    # smoothing on the headline mesh (after the FP64 check, which reads the
+    # unsmoothed vertices)
+    smooth = bench_smooth(ctx, torch, field, ISO, verts, quads, counts, V)
+    smooth["workload"] = "512^3 blobs (the headline field)"
+    log(f"smoothing: {smooth}")
     host = None if args.skip_host else bench_host(fa, field.cpu().numpy(), V, Q)
     dense = None if args.skip_dense else bench_dense(fa, torch)
     batched = None if args.skip_batched else bench_batched(fa, torch, tune=not args.no_tune)
@@ -292,6 +297,7 @@ def main() -> int:
                      "kernel_avg_ms": per_kernel},
         "cpu_baseline": cpu,
         "vertex_error_vs_fp64": fp64_error,
+        "smoothing": smooth,
         "host_path": host,
         "dense": dense,
         "batched": batched,
@@ -352,13 +358,49 @@ def bench_host(fa, fnp, V: int, Q: int, reps: int = 5) -> dict:
         t_ooc.append(time.perf_counter() - t)
     assert np.array_equal(qo, q) and np.array_equal(vo.view(np.uint32), v.view(np.uint32))
     ms_ooc = float(np.median(t_ooc)) * 1e3
+    # the OBJ writer (mdmc_write_obj, dump_obj's format) on that mesh
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "mesh.obj")
+        fa.write_obj(path, v, q)
+        t_obj = []
+        for _ in range(3):
+            t = time.perf_counter()
+            fa.write_obj(path, v, q)
+            t_obj.append(time.perf_counter() - t)
+        obj_bytes = os.path.getsize(path)
+    ms_obj = float(np.median(t_obj)) * 1e3
     out = {"entry": "mdmc_run (host field in, host mesh out)", "ms_per_extraction": ms,
            "gvoxels_per_s": N / (ms * 1e-3) / 1e9, "h2d_bytes": 4 * N, "d2h_bytes": 12 * V + 16 * Q,
            "transfer_gbs": (4 * N + 12 * V + 16 * Q) / (ms * 1e-3) / 1e9,
            "out_of_core": {"device_budget_bytes": budget, "slab_cell_planes": S, "ms_per_extraction": ms_ooc,
-                           "gvoxels_per_s": N / (ms_ooc * 1e-3) / 1e9, "identical_to_in_core": True}}
+                           "gvoxels_per_s": N / (ms_ooc * 1e-3) / 1e9, "identical_to_in_core": True},
+           "obj_writer": {"bytes": obj_bytes, "ms": ms_obj, "mb_per_s": obj_bytes / (ms_obj * 1e-3) / 1e6,
+                          "vertices": V, "triangles": 2 * Q}}
     log(f"host path: {out}")
     return out
+
+
+def bench_smooth(ctx, torch, field, iso: float, verts, quads, counts, V: int, iters: int = 3) -> dict:
+    """Vertex smoothing (num_smooth, mdmc_smooth_device) on the mesh of one
+    extraction: `iters` iterations (each k_smooth then k_reaverage) timed with
+    HIP events on the launching stream.  Algorithmic bytes per iteration: every
+    vertex read once and written once, 24 V."""
+    ctx.extract_into(field, iso, verts, quads, counts=counts)
+    stream = torch.cuda.current_stream()
+    ctx.smooth(verts, quads, 1, counts=counts)  # warm
+    ctx.extract_into(field, iso, verts, quads, counts=counts)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    ctx.smooth(verts, quads, iters, counts=counts)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    assert int(counts[3]) == 0
+    ms = t0.elapsed_time(t1) / iters
+    return {"iterations": iters, "ms_per_iteration": ms, "algorithmic_bytes_per_iteration": 24.0 * V,
+            "achieved_gbs": 24.0 * V / (ms * 1e-3) / 1e9}
 
 
 def bench_dense(fa, torch, n: int = 1024, steps: int = 5) -> dict:
@@ -399,6 +441,7 @@ def bench_dense(fa, torch, n: int = 1024, steps: int = 5) -> dict:
            "algorithmic_bytes": B, "achieved_gbs": B / (ms * 1e-3) / 1e9,
            "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in prof.items() if k != "other"}}
+    out["smoothing"] = bench_smooth(ctx, torch, f, 0.0, verts, quads, counts, V)
     log(f"dense: {out}")
     ctx.close()
     del f, verts, quads

@@ -1419,8 +1419,14 @@ __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8,
 // Position of an active cell from its segment.
 struct CellPos {
   uint32_t xl, cfg, lut2, x, j, k, vb, qb;
+  uint32_t seg_vb;  // the segment's first vertex
+  uint32_t nact;    // the segment's active cells
+  float y1, z1;     // coordinates of point row j + 1 and plane k + 1
 };
 
+// As in the emit, what depends only on the segment (its bases and count, the
+// row's coordinates) comes through scalar loads for the lanes in the wave's
+// first segment and through vector loads for the others.
 __device__ __forceinline__ CellPos cell_pos(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
                                             uint64_t &rec) {
   CellPos c;
@@ -1433,10 +1439,32 @@ __device__ __forceinline__ CellPos cell_pos(const EmitArgs &a, const EmitVol &vo
   c.j = row % vol.g.cy;
   c.k = row / vol.g.cy;
   c.x = xt * a.cap + c.xl;
-  const uint2 vq = a.seg_vq[s];
+  const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+  uint32_t s0u = s0;
+  asm volatile("" : "+s"(s0u));
+  const uint32_t row0 = (s0u - vol.seg_off) / vol.XT;
+  uint2 vq = const_u2(a.seg_vq, s0u);
+  uint32_t nact = const_u2x(a.seg_counts, s0u) & 0xFFFFu;
+  float y1 = as_const(vol.yt_c)[row0 % vol.g.cy + 1u], z1 = as_const(vol.zt_c)[row0 / vol.g.cy + 1u];
+  if (s != s0) {
+    vq = a.seg_vq[s];
+    nact = a.seg_counts[s].x & 0xFFFFu;
+    y1 = vol.yt_c[c.j + 1u];
+    z1 = vol.zt_c[c.k + 1u];
+  }
+  c.seg_vb = vq.x;
+  c.nact = nact;
+  c.y1 = y1;
+  c.z1 = z1;
   c.vb = vq.x + rec_lvb(rec) - vol.vbase;
   c.qb = vq.y + rec_lqb(rec) - vol.qbase;
   return c;
+}
+
+// the same from a copy of c_cfg_info in LDS
+__device__ __forceinline__ uint32_t iso_count_lds(const uint16_t *s_info, uint32_t cfg, uint32_t lut2) {
+  const uint32_t info = s_info[cfg];
+  return lut2 ? ((info >> 3) & 7u) : (info & 7u);
 }
 
 __device__ __forceinline__ uint32_t iso_count(uint32_t cfg, uint32_t lut2) {
@@ -1479,17 +1507,23 @@ __device__ __forceinline__ bool inside_tri(P2 p0, P2 p1, P2 p2, P2 t, float &al,
   return (-1e-4 < (double)be) && (-1e-4 < (double)ga) && ((double)(be + ga) < 1.0 + 1e-4);
 }
 
+// v[t] for t in 0..3 by selects (no indexed register array)
+__device__ __forceinline__ float sel4(uint32_t t, const float (&v)[4]) {
+  const float lo = (t & 1u) ? v[1] : v[0], hi = (t & 1u) ? v[3] : v[2];
+  return (t & 2u) ? hi : lo;
+}
+
 __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank) {
   uint64_t rec;
   const CellPos c = cell_pos(a, vol, s, rank, rec);
   const FieldDesc &g = vol.g;
-  if (c.x >= g.cx || rank >= (a.seg_counts[s].x & 0xFFFFu)) { atomicOr(vol.status, kStatusInternal); return; }
+  if (c.x >= g.cx || rank >= c.nact) { atomicOr(vol.status, kStatusInternal); return; }
   const uint32_t bip = owned_bipolar(c.cfg);
   const bool xin = c.x + 1 < g.cx, yin = c.j + 1 < g.cy, zin = c.k + 1 < g.cz;
   const uint32_t has = ((bip & 1u) && xin && yin ? 1u : 0u) | ((bip & 2u) && xin && zin ? 2u : 0u) |
                        ((bip & 4u) && yin && zin ? 4u : 0u);
   if (!has) return;
-  const float x1 = vol.xt_c[c.x + 1], y1 = vol.yt_c[c.j + 1], z1 = vol.zt_c[c.k + 1];
+  const float x1 = vol.xt_c[c.x + 1], y1 = c.y1, z1 = c.z1;
   uint32_t q = c.qb, ev = c.vb + iso_count(c.cfg, c.lut2);
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
@@ -1500,14 +1534,20 @@ __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vo
     const int4 rq = reinterpret_cast<const int4 *>(vol.quads)[q++];
     const int32_t iv[4] = {rq.x, rq.y, rq.z, rq.w};
     const int c0 = l == 2 ? 1 : 0, c1 = l == 0 ? 1 : 2;
+    // the ring's four iso vertices, read once (k_smooth writes edge points
+    // only, so they do not change while it runs)
+    float vx[4], vy[4], vz[4];
     P2 P[4];
     bool bad = false;
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       bad |= iv[p] < 0 || (uint64_t)iv[p] >= vol.vcap;
       const float *vp = vol.verts + 3ull * (uint32_t)(bad ? 0 : iv[p]);
-      P[p].x = vp[c0];
-      P[p].y = vp[c1];
+      vx[p] = vp[0];
+      vy[p] = vp[1];
+      vz[p] = vp[2];
+      P[p].x = c0 == 0 ? vx[p] : vy[p];
+      P[p].y = c1 == 1 ? vy[p] : vz[p];
     }
     if (bad) { atomicOr(vol.status, kStatusInternal); continue; }
     // calc_quadrilateral_signs (dmc.cu:1749-1780)
@@ -1541,12 +1581,10 @@ __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vo
     } else {
       continue;
     }
-    const float *va = vol.verts + 3ull * (uint32_t)iv[ta], *vb = vol.verts + 3ull * (uint32_t)iv[tb],
-                *vc = vol.verts + 3ull * (uint32_t)iv[tc];
     float *E = vol.verts + 3ull * e_vtx;
-    const float r0 = al * va[0] + be * vb[0] + ga * vc[0];
-    const float r1 = al * va[1] + be * vb[1] + ga * vc[1];
-    const float r2 = al * va[2] + be * vb[2] + ga * vc[2];
+    const float r0 = al * sel4(ta, vx) + be * sel4(tb, vx) + ga * sel4(tc, vx);
+    const float r1 = al * sel4(ta, vy) + be * sel4(tb, vy) + ga * sel4(tc, vy);
+    const float r2 = al * sel4(ta, vz) + be * sel4(tb, vz) + ga * sel4(tc, vz);
     E[0] = r0;
     E[1] = r1;
     E[2] = r2;
@@ -1560,17 +1598,17 @@ __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vo
 // group from -0.0 and divided by the count; groups with no point keep their
 // value.
 __device__ __forceinline__ void reaverage_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank,
-                                               const uint64_t *s_lab, float *s_acc) {
+                                               const uint64_t *s_lab, const uint16_t *s_info, float *s_acc) {
   uint64_t rec;
   const CellPos c = cell_pos(a, vol, s, rank, rec);
   const FieldDesc &g = vol.g;
   const uint32_t cap = a.cap;
-  if (c.x >= g.cx || c.xl >= cap || rank >= (a.seg_counts[s].x & 0xFFFFu)) {
+  if (c.x >= g.cx || c.xl >= cap || rank >= c.nact) {
     atomicOr(vol.status, kStatusInternal);
     return;
   }
   const uint64_t labels = s_lab[c.lut2 * 256 + c.cfg];
-  const uint32_t niso = iso_count(c.cfg, c.lut2);
+  const uint32_t niso = iso_count_lds(s_info, c.cfg, c.lut2);
   // owner cell (7 = this cell) and the owner's edge shift (0: e6, 1: e9, 2: e10)
   constexpr uint8_t OWN_N[12] = {5, 2, 2, 4, 3, 1, 7, 0, 1, 7, 7, 0};
   constexpr uint8_t OWN_L[12] = {2, 1, 2, 1, 0, 0, 0, 0, 2, 1, 2, 1};
@@ -1582,33 +1620,65 @@ __device__ __forceinline__ void reaverage_cell(const EmitArgs &a, const EmitVol 
     const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
     if (lab != 0xFu && (NEED[e] & have) == NEED[e] && OWN_N[e] < 7) need |= 1u << OWN_N[e];
   }
+  // Owners 1 (0,-1,0), 2 (0,0,-1) and 5 (0,-1,-1) are found through their
+  // segments' group maps.  An owner needed here contains a crossing edge, so
+  // it is active, and so are the other cells around that edge: owners 0
+  // (-1,0,0), 3 (-1,-1,0) and 4 (-1,0,-1) are the active cells just before
+  // this cell, owner 1 and owner 2 in their rows (when x is the first cell
+  // of its x-tile they lie in the previous x-tile's segments and are found
+  // through its group maps instead).  Each record's x field confirms the
+  // cell found.  A lookup the cell does not need reads this cell's segment,
+  // so that every load is in bounds.
   const bool wrap = c.xl == 0;  // -x lies in the previous x-tile's segment
-  const uint32_t sx = wrap ? s - 1u : s, xlm = wrap ? cap - 1u : c.xl - 1u;
+  const uint32_t xlm = wrap ? cap - 1u : c.xl - 1u;
   const uint32_t dy = vol.XT, dz = vol.XT * g.cy;
-  const uint32_t ns0[6] = {sx, s - dy, s - dz, sx - dy, sx - dz, s - dy - dz};
-  const uint32_t nx0[6] = {xlm, c.xl, c.xl, xlm, xlm, c.xl};
-  uint32_t ns[6], nxl[6], vq0[6];
-  uint4 gm[6];
+  const uint32_t need1 = need & 0x0Au, need2 = need & 0x14u, need5 = need & 0x20u, needw = need & 0x19u;
+  const uint32_t ns[6] = {wrap ? s - 1u : s, need1 ? s - dy : s, need2 ? s - dz : s,
+                          wrap ? s - 1u - dy : s - dy, wrap ? s - 1u - dz : s - dz, need5 ? s - dy - dz : s};
+  const uint32_t G = cap >> 6;
+  const uint4 g1 = a.gmap[(size_t)ns[1] * G + (c.xl >> 6)], g2 = a.gmap[(size_t)ns[2] * G + (c.xl >> 6)];
+  const uint4 g5 = a.gmap[(size_t)ns[5] * G + (c.xl >> 6)];
+  uint32_t vq0[6];
+  vq0[1] = a.seg_vq[ns[1]].x;
+  vq0[2] = a.seg_vq[ns[2]].x;
+  vq0[5] = a.seg_vq[ns[5]].x;
+  const uint32_t bit = c.xl & 63u;
+  const uint64_t below = (1ull << bit) - 1ull;
+  const uint64_t m1 = (uint64_t)g1.x | ((uint64_t)g1.y << 32), m2 = (uint64_t)g2.x | ((uint64_t)g2.y << 32);
+  const uint64_t m5 = (uint64_t)g5.x | ((uint64_t)g5.y << 32);
+  const uint32_t r1 = g1.z + (uint32_t)__popcll(m1 & below), r2 = g2.z + (uint32_t)__popcll(m2 & below);
+  const uint32_t r5 = g5.z + (uint32_t)__popcll(m5 & below);
+  uint32_t nr[6] = {rank - 1u, r1, r2, r1 - 1u, r2 - 1u, r5};
+  bool inmask[6] = {rank > 0u, ((m1 >> bit) & 1ull) != 0, ((m2 >> bit) & 1ull) != 0, r1 > 0u, r2 > 0u,
+                    ((m5 >> bit) & 1ull) != 0};
+  vq0[0] = c.seg_vb;
+  vq0[3] = vq0[1];
+  vq0[4] = vq0[2];
+  if (wrap && needw) {  // owners 0, 3, 4: the last cells of the previous x-tile's segments
+    const uint32_t gw = (cap - 1u) >> 6, bw = (cap - 1u) & 63u;
+    const uint64_t bwl = (1ull << bw) - 1ull;
 #pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const bool use = (need >> n) & 1u;
-    ns[n] = use ? ns0[n] : s;
-    nxl[n] = use ? nx0[n] : c.xl;
-    gm[n] = a.gmap[(size_t)ns[n] * (cap >> 6) + (nxl[n] >> 6)];
-    vq0[n] = a.seg_vq[ns[n]].x;
+    for (int n = 0; n < 6; ++n) {
+      if (n != 0 && n != 3 && n != 4) continue;
+      const uint32_t sn = ((need >> n) & 1u) ? ns[n] : s;  // in bounds when not needed
+      const uint4 gx = a.gmap[(size_t)sn * G + gw];
+      const uint64_t mx = (uint64_t)gx.x | ((uint64_t)gx.y << 32);
+      nr[n] = gx.z + (uint32_t)__popcll(mx & bwl);
+      inmask[n] = ((mx >> bw) & 1ull) != 0;
+      vq0[n] = a.seg_vq[sn].x;
+    }
   }
+  const uint32_t nx[6] = {xlm, c.xl, c.xl, xlm, xlm, c.xl};
   const uint32_t *slot_lo = reinterpret_cast<const uint32_t *>(a.slots);
   uint32_t own_base[8], own_bip[8];  // first edge point of each owner, its bipolar bits
   bool ok = true;
 #pragma unroll
   for (int n = 0; n < 6; ++n) {
-    const uint64_t mask = (uint64_t)gm[n].x | ((uint64_t)gm[n].y << 32);
-    const uint32_t bit = nxl[n] & 63u;
-    const uint32_t r = gm[n].z + (uint32_t)__popcll(mask & ((1ull << bit) - 1ull));
-    const bool act = ((mask >> bit) & 1ull) && r < cap;
-    if (((need >> n) & 1u) && !act) ok = false;
-    const uint32_t m = slot_lo[2 * ((size_t)ns[n] * cap + (act ? r : rank))];
-    own_base[n] = vq0[n] + rec_lvb(m) - vol.vbase + iso_count(rec_cfg(m), rec_lut2(m));
+    const bool use = (need >> n) & 1u;
+    const bool ok_r = use && inmask[n] && nr[n] < cap;
+    const uint32_t m = slot_lo[2 * ((size_t)(ok_r ? ns[n] : s) * cap + (ok_r ? nr[n] : rank))];
+    if (use && !(ok_r && rec_x(m) == nx[n])) ok = false;
+    own_base[n] = vq0[n] + rec_lvb(m) - vol.vbase + iso_count_lds(s_info, rec_cfg(m), rec_lut2(m));
     own_bip[n] = owned_bipolar(rec_cfg(m));
   }
   own_base[6] = 0;
@@ -1672,11 +1742,13 @@ __global__ __launch_bounds__(kEmitThreads) void k_smooth(EmitArgs a) {
 
 __global__ __launch_bounds__(kEmitThreads) void k_reaverage(EmitArgs a) {
   __shared__ uint64_t s_lab[2 * 256];
+  __shared__ uint16_t s_info[256];  // c_cfg_info: seven lookups per cell
   __shared__ float s_acc[12 * kEmitThreads];
   const Totals tot = *a.totals;
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
   if (!workgroup_has_chunks(num_chunks)) return;  // (uniform: before the barrier)
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
+  for (uint32_t i = threadIdx.x; i < 256; i += kEmitThreads) s_info[i] = c_cfg_info[i];
   __syncthreads();
   const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
@@ -1684,7 +1756,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_reaverage(EmitArgs a) {
   for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
     uint32_t s, rank;
     if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
-    reaverage_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
+    reaverage_cell(a, a.v, s, rank, s_lab, s_info, s_acc + threadIdx.x);
   }
 }
 

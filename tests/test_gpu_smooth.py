@@ -33,6 +33,11 @@ def _fields():
         ("blobs", oracle.field_blobs((50, 44, 38), 3, 9, sigma=(3.0, 9.0)), 0.1, None),
         ("box", oracle.field_sphere((33, 30, 35), (16.0, 14.2, 17.9), 11.1), 0.0,
          ((-5.0, -2.5, 1.0), (5.0, 7.5, 3.0))),
+        # rows longer than one x-tile: cells at the start of a tile find their
+        # -x neighbours in the previous tile's segment
+        ("wide_gyroid", oracle.field_gyroid((300, 20, 18), 7.0, 11, 0.3), 0.0, None),
+        ("wide_blobs", oracle.field_blobs((520, 16, 14), 6, 32, sigma=(4.0, 10.0)), 0.1, None),
+        ("wide_blobs2", oracle.field_blobs((520, 16, 14), 7, 28, sigma=(3.0, 12.0)), 0.05, None),
     ]
 
 
