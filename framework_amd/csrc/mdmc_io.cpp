@@ -10,6 +10,7 @@
 // default priority.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
@@ -164,6 +165,41 @@ void staged_release() {
 
 }  // namespace mdmc
 
+namespace {
+
+// OBJ text of vertices [i0, i1) or of the two triangles of quads [i0, i1),
+// one line (or two) per element, into dst; returns the bytes written.
+constexpr size_t kObjLineMax = 128;  // a vertex line or a quad's two lines fit
+
+size_t obj_vertices(char *dst, const float *v, uint64_t i0, uint64_t i1) {
+  size_t used = 0;
+  // "v x y z" with six significant digits (the reference writes floats
+  // through std::ostream at its default precision, dmc.cu:2238-2246)
+  for (uint64_t i = i0; i < i1; ++i)
+    used += (size_t)std::snprintf(dst + used, kObjLineMax, "v %g %g %g\n", (double)v[3 * i], (double)v[3 * i + 1],
+                                  (double)v[3 * i + 2]);
+  return used;
+}
+
+size_t obj_faces(char *dst, const int32_t *quads, uint64_t i0, uint64_t i1) {
+  size_t used = 0;
+  // two triangles (q0,q1,q2), (q2,q3,q0) per quad, 1-based indices
+  for (uint64_t q = i0; q < i1; ++q) {
+    const uint32_t a = (uint32_t)quads[4 * q] + 1u, b = (uint32_t)quads[4 * q + 1] + 1u,
+                   c = (uint32_t)quads[4 * q + 2] + 1u, d = (uint32_t)quads[4 * q + 3] + 1u;
+    used += (size_t)std::snprintf(dst + used, kObjLineMax,
+                                  "f %" PRIu32 " %" PRIu32 " %" PRIu32 "\nf %" PRIu32 " %" PRIu32 " %" PRIu32 "\n", a,
+                                  b, c, c, d, a);
+  }
+  return used;
+}
+
+}  // namespace
+
+// Formatting the text is what costs (a few hundred ns per line), so chunks of
+// lines are formatted by several threads, a round of chunks at a time (memory
+// stays at a round's buffers however large the mesh), and written in order:
+// the bytes are those of one thread writing line after line.
 extern "C" int mdmc_write_obj(const char *path, const float *vertices, uint64_t num_vertices, const int32_t *quads,
                               uint64_t num_quads) {
   if (!path || (num_vertices && !vertices) || (num_quads && !quads))
@@ -174,34 +210,33 @@ extern "C" int mdmc_write_obj(const char *path, const float *vertices, uint64_t 
       return mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: quad index outside the vertex array");
   FILE *f = std::fopen(path, "wb");
   if (!f) return mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: cannot open the output file");
-  std::vector<char> buf(1u << 20);
-  size_t used = 0;
+  constexpr uint64_t kLines = 16384;  // elements per chunk
+  const uint64_t nvc = (num_vertices + kLines - 1) / kLines, nqc = (num_quads + kLines - 1) / kLines;
+  const uint64_t chunks = nvc + nqc;
+  const unsigned hw = std::thread::hardware_concurrency();
+  const uint64_t T = std::max<uint64_t>(1, std::min<uint64_t>({16, hw ? hw : 1, chunks}));
+  std::vector<std::vector<char>> buf(T, std::vector<char>(kLines * kObjLineMax));
+  std::vector<size_t> used(T, 0);
+  auto format = [&](uint64_t c, uint64_t t) {
+    if (c < nvc)
+      used[t] = obj_vertices(buf[t].data(), vertices, c * kLines, std::min(num_vertices, (c + 1) * kLines));
+    else
+      used[t] = obj_faces(buf[t].data(), quads, (c - nvc) * kLines, std::min(num_quads, (c - nvc + 1) * kLines));
+  };
   bool ok = true;
-  auto flush = [&]() {
-    if (used && std::fwrite(buf.data(), 1, used, f) != used) ok = false;
-    used = 0;
-  };
-  auto room = [&](size_t need) {
-    if (buf.size() - used < need) flush();
-  };
-  // "v x y z" with six significant digits (the reference writes floats
-  // through std::ostream at its default precision, dmc.cu:2238-2246)
-  for (uint64_t i = 0; i < num_vertices && ok; ++i) {
-    room(128);
-    const int n = std::snprintf(buf.data() + used, 128, "v %g %g %g\n", (double)vertices[3 * i],
-                                (double)vertices[3 * i + 1], (double)vertices[3 * i + 2]);
-    used += (size_t)n;
+  for (uint64_t c0 = 0; c0 < chunks && ok; c0 += T) {
+    const uint64_t n = std::min(T, chunks - c0);
+    std::vector<std::thread> th;
+    try {
+      for (uint64_t t = 1; t < n; ++t) th.emplace_back(format, c0 + t, t);
+    } catch (...) {
+      for (uint64_t t = 1 + th.size(); t < n; ++t) format(c0 + t, t);  // no thread: format here
+    }
+    format(c0, 0);
+    for (std::thread &x : th) x.join();
+    for (uint64_t t = 0; t < n && ok; ++t)
+      if (used[t] && std::fwrite(buf[t].data(), 1, used[t], f) != used[t]) ok = false;
   }
-  // two triangles (q0,q1,q2), (q2,q3,q0) per quad, 1-based indices
-  for (uint64_t q = 0; q < num_quads && ok; ++q) {
-    room(128);
-    const uint32_t a = (uint32_t)quads[4 * q] + 1u, b = (uint32_t)quads[4 * q + 1] + 1u,
-                   c = (uint32_t)quads[4 * q + 2] + 1u, d = (uint32_t)quads[4 * q + 3] + 1u;
-    const int n = std::snprintf(buf.data() + used, 128, "f %" PRIu32 " %" PRIu32 " %" PRIu32 "\nf %" PRIu32
-                                " %" PRIu32 " %" PRIu32 "\n", a, b, c, c, d, a);
-    used += (size_t)n;
-  }
-  flush();
   if (std::fclose(f) != 0) ok = false;
   return ok ? MDMC_OK : mdmc::set_error(MDMC_ERR_INVALID, "mdmc_write_obj: write to the output file failed");
 }

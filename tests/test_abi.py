@@ -59,16 +59,19 @@ def test_cpp_shim_header_compiles_and_links(tmp_path):
                    check=True)
 
 
-def test_obj_writers_agree(tmp_path):
+@pytest.mark.parametrize("nv,nq", [(257, 300), (70001, 50003)])
+def test_obj_writers_agree(tmp_path, nv, nq):
     """mdmc_write_obj over quads and the shim's dump_obj over the triangles the
-    shim's run_dmc makes from them give the same bytes; faces are 1-based."""
+    shim's run_dmc makes from them give the same bytes; faces are 1-based.  The
+    larger mesh spans several of the chunks mdmc_write_obj formats on separate
+    threads (16384 vertices or quads each), so their order is checked too."""
     import numpy as np
 
     rng = np.random.default_rng(3)
-    v = (rng.standard_normal((257, 3)) * [1.0, 300.0, 1e-3]).astype(np.float32)
+    v = (rng.standard_normal((nv, 3)) * [1.0, 300.0, 1e-3]).astype(np.float32)
     v[5] = [0.1, 1e7, -0.0]
     v[9, 1] = np.float32(np.nan)
-    q = rng.integers(0, 257, (300, 4)).astype(np.int32)
+    q = rng.integers(0, nv, (nq, 4)).astype(np.int32)
     (tmp_path / "v.f32").write_bytes(v.tobytes())
     (tmp_path / "q.i32").write_bytes(q.tobytes())
     src = os.path.join(ROOT, "tests", "golden", "obj_check.cpp")
@@ -82,8 +85,9 @@ def test_obj_writers_agree(tmp_path):
     a, b = (tmp_path / "a.obj").read_bytes(), (tmp_path / "b.obj").read_bytes()
     assert a == b
     lines = a.decode().splitlines()
-    assert len(lines) == 257 + 600
-    assert lines[0].startswith("v ") and lines[257] == "f %d %d %d" % (q[0, 0] + 1, q[0, 1] + 1, q[0, 2] + 1)
+    assert len(lines) == nv + 2 * nq
+    assert lines[0].startswith("v ") and lines[nv] == "f %d %d %d" % (q[0, 0] + 1, q[0, 1] + 1, q[0, 2] + 1)
+    assert lines[-1] == "f %d %d %d" % (q[-1, 2] + 1, q[-1, 3] + 1, q[-1, 0] + 1)
     fa.write_obj(str(tmp_path / "c.obj"), v, q)
     assert (tmp_path / "c.obj").read_bytes() == a
 
