@@ -1513,29 +1513,70 @@ __device__ __forceinline__ float sel4(uint32_t t, const float (&v)[4]) {
   return (t & 2u) ? hi : lo;
 }
 
-__device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vol, uint32_t s, uint32_t rank) {
-  uint64_t rec;
-  const CellPos c = cell_pos(a, vol, s, rank, rec);
+// One smoothing pass over the owned edge points of a chunk's cells (found:
+// this lane has a cell).  The chunk's quads are spread over its lanes, one
+// quad per lane per round: the lanes' quad counts are prefix-summed and lane
+// L of round r takes quad r*64 + L from the lane that owns it (a shuffle
+// search over the prefix).  Taking each cell's up-to-three quads on its own
+// lane would run three divergent rounds in nearly every surface-dense chunk,
+// where most cells own one quad.  Every lane takes part in the shuffles.
+__device__ __forceinline__ void smooth_chunk(const EmitArgs &a, const EmitVol &vol, bool found, uint32_t s,
+                                             uint32_t rank) {
   const FieldDesc &g = vol.g;
-  if (c.x >= g.cx || rank >= c.nact) { atomicOr(vol.status, kStatusInternal); return; }
-  const uint32_t bip = owned_bipolar(c.cfg);
-  const bool xin = c.x + 1 < g.cx, yin = c.j + 1 < g.cy, zin = c.k + 1 < g.cz;
-  const uint32_t has = ((bip & 1u) && xin && yin ? 1u : 0u) | ((bip & 2u) && xin && zin ? 2u : 0u) |
-                       ((bip & 4u) && yin && zin ? 4u : 0u);
-  if (!has) return;
-  const float x1 = vol.xt_c[c.x + 1], y1 = c.y1, z1 = c.z1;
-  uint32_t q = c.qb, ev = c.vb + iso_count(c.cfg, c.lut2);
+  uint32_t has = 0, bip = 0, qb = 0, ev0 = 0;
+  float x1 = 0.0f, y1 = 0.0f, z1 = 0.0f;
+  if (found) {
+    uint64_t rec;
+    const CellPos c = cell_pos(a, vol, s, rank, rec);
+    if (c.x >= g.cx || rank >= c.nact) {
+      atomicOr(vol.status, kStatusInternal);
+    } else {
+      bip = owned_bipolar(c.cfg);
+      const bool xin = c.x + 1 < g.cx, yin = c.j + 1 < g.cy, zin = c.k + 1 < g.cz;
+      has = ((bip & 1u) && xin && yin ? 1u : 0u) | ((bip & 2u) && xin && zin ? 2u : 0u) |
+            ((bip & 4u) && yin && zin ? 4u : 0u);
+      if (has) x1 = vol.xt_c[c.x + 1];
+      y1 = c.y1;
+      z1 = c.z1;
+      qb = c.qb;
+      ev0 = c.vb + iso_count(c.cfg, c.lut2);
+    }
+  }
+  const uint32_t lane = lane_id();
+  const uint32_t cnt = (uint32_t)__popc(has);
+  const uint32_t incl = wave_incl_scan(cnt), off = incl - cnt;
+  const uint32_t K = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+  for (uint32_t base = 0; base < K; base += kWave) {
+    const uint32_t jq = base + lane;
+    // owner: the highest lane whose prefix is <= jq (for jq < K it owns jq)
+    uint32_t m = 0;
 #pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    const uint32_t e_vtx = ev;
-    if ((bip >> l) & 1u) ++ev;
-    if (!((has >> l) & 1u)) continue;
-    if ((uint64_t)q >= vol.qcap || (uint64_t)e_vtx >= vol.vcap) { atomicOr(vol.status, kStatusInternal); return; }
-    const int4 rq = reinterpret_cast<const int4 *>(vol.quads)[q++];
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+      const uint32_t cand = m + step;
+      const uint32_t oc = (uint32_t)__shfl((int)off, (int)(cand & 63u), kWave);
+      if (cand < 64u && oc <= jq) m = cand;
+    }
+    const uint32_t o_off = (uint32_t)__shfl((int)off, (int)m, kWave);
+    const uint32_t o_has = (uint32_t)__shfl((int)has, (int)m, kWave);
+    const uint32_t o_bip = (uint32_t)__shfl((int)bip, (int)m, kWave);
+    const uint32_t o_qb = (uint32_t)__shfl((int)qb, (int)m, kWave);
+    const uint32_t o_ev0 = (uint32_t)__shfl((int)ev0, (int)m, kWave);
+    const float o_x1 = __shfl(x1, (int)m, kWave), o_y1 = __shfl(y1, (int)m, kWave), o_z1 = __shfl(z1, (int)m, kWave);
+    if (jq >= K) continue;
+    // the owner's k-th quad is on its k-th set bit l of `has`
+    const uint32_t kq = jq - o_off;
+    uint32_t h = o_has;
+    if (kq >= 1u) h &= h - 1u;
+    if (kq >= 2u) h &= h - 1u;
+    const uint32_t l = (uint32_t)__builtin_ctz(h);
+    const uint32_t q = o_qb + kq;
+    const uint32_t e_vtx = o_ev0 + (uint32_t)__popc(o_bip & ((1u << l) - 1u));
+    if ((uint64_t)q >= vol.qcap || (uint64_t)e_vtx >= vol.vcap) { atomicOr(vol.status, kStatusInternal); continue; }
+    const int4 rq = reinterpret_cast<const int4 *>(vol.quads)[q];
     const int32_t iv[4] = {rq.x, rq.y, rq.z, rq.w};
-    const int c0 = l == 2 ? 1 : 0, c1 = l == 0 ? 1 : 2;
     // the ring's four iso vertices, read once (k_smooth writes edge points
-    // only, so they do not change while it runs)
+    // only, so they do not change while it runs); the edge's plane: x-y for
+    // e6 (l = 0), x-z for e9, y-z for e10
     float vx[4], vy[4], vz[4];
     P2 P[4];
     bool bad = false;
@@ -1546,8 +1587,8 @@ __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vo
       vx[p] = vp[0];
       vy[p] = vp[1];
       vz[p] = vp[2];
-      P[p].x = c0 == 0 ? vx[p] : vy[p];
-      P[p].y = c1 == 1 ? vy[p] : vz[p];
+      P[p].x = l == 2u ? vy[p] : vx[p];
+      P[p].y = l == 0u ? vy[p] : vz[p];
     }
     if (bad) { atomicOr(vol.status, kStatusInternal); continue; }
     // calc_quadrilateral_signs (dmc.cu:1749-1780)
@@ -1571,7 +1612,7 @@ __device__ __forceinline__ void smooth_cell(const EmitArgs &a, const EmitVol &vo
       s0 = npos < 2 ? ipos : ineg;  // the concave corner
     }
     const uint32_t s1 = (s0 + 2) & 3;
-    const P2 o = l == 0 ? P2{x1, y1} : (l == 1 ? P2{x1, z1} : P2{y1, z1});
+    const P2 o = l == 0u ? P2{o_x1, o_y1} : (l == 1u ? P2{o_x1, o_z1} : P2{o_y1, o_z1});
     float al, be, ga;
     uint32_t ta, tb, tc;
     if (inside_tri(P[s0], P[(s0 + 1) & 3], P[s1], o, al, be, ga)) {
@@ -1734,9 +1775,9 @@ __global__ __launch_bounds__(kEmitThreads) void k_smooth(EmitArgs a) {
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
   const ChunkIter it = chunk_iter(num_chunks, wave);
   for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
-    uint32_t s, rank;
-    if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
-    smooth_cell(a, a.v, s, rank);
+    uint32_t s = 0, rank = 0;
+    const bool found = locate_cell(a, tot, num_ne, ch, lane, s, rank);
+    smooth_chunk(a, a.v, found, s, rank);
   }
 }
 
