@@ -2106,27 +2106,41 @@ __device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
       for (uint32_t i = 0; i < 8; ++i) ax[8 * g + i] = amp[8 * g + i] * a.ex[(size_t)(8 * g + i) * a.X + xc];
     }
   }
+  // two rows per pass: two independent sums (each a serial chain of FMAs)
+  // and one wait for the scalar loads of both rows' factors per blob group
   float *out = a.out + t.r0 * a.X + t.x;
-  for (uint32_t rr = 0; rr < t.nr; ++rr) {
-    uint32_t y, z;
-    row_yz(t, a.Y, rr, y, z);
-    const uniform_dptr ey = as_uniform(a.ey + (size_t)y * kBlobRegs);
-    const uniform_dptr ez = as_uniform(a.ez + (size_t)z * kBlobRegs);
-    double acc = 0.0;
+  for (uint32_t rr = 0; rr < t.nr; rr += 2) {
+    const bool two = rr + 1u < t.nr;
+    uint32_t ya, za, yb, zb;
+    row_yz(t, a.Y, rr, ya, za);
+    row_yz(t, a.Y, two ? rr + 1u : rr, yb, zb);
+    const uniform_dptr eya = as_uniform(a.ey + (size_t)ya * kBlobRegs);
+    const uniform_dptr eza = as_uniform(a.ez + (size_t)za * kBlobRegs);
+    const uniform_dptr eyb = as_uniform(a.ey + (size_t)yb * kBlobRegs);
+    const uniform_dptr ezb = as_uniform(a.ez + (size_t)zb * kBlobRegs);
+    double acca = 0.0, accb = 0.0;
 #pragma unroll
     for (uint32_t g = 0; g < kBlobRegs / 8; ++g) {
       if (8 * g < a.nblobs) {
-        double fy[8], fz[8];
+        double fya[8], fza[8], fyb[8], fzb[8];
 #pragma unroll
         for (uint32_t i = 0; i < 8; ++i) {
-          fy[i] = ey[8 * g + i];
-          fz[i] = ez[8 * g + i];
+          fya[i] = eya[8 * g + i];
+          fza[i] = eza[8 * g + i];
+          fyb[i] = eyb[8 * g + i];
+          fzb[i] = ezb[8 * g + i];
         }
 #pragma unroll
-        for (uint32_t i = 0; i < 8; ++i) acc = __builtin_fma(ax[8 * g + i] * fy[i], fz[i], acc);
+        for (uint32_t i = 0; i < 8; ++i) {
+          acca = __builtin_fma(ax[8 * g + i] * fya[i], fza[i], acca);
+          accb = __builtin_fma(ax[8 * g + i] * fyb[i], fzb[i], accb);
+        }
       }
     }
-    if (t.x < a.X) out[(size_t)rr * a.X] = (float)acc;
+    if (t.x < a.X) {
+      out[(size_t)rr * a.X] = (float)acca;
+      if (two) out[(size_t)(rr + 1u) * a.X] = (float)accb;
+    }
   }
 }
 
