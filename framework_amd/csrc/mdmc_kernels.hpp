@@ -1735,23 +1735,42 @@ __device__ __forceinline__ void reaverage_cell(const EmitArgs &a, const EmitVol 
       s_acc[(3 * q + 2) * T] = -0.0f;
     }
   }
-  uint32_t cnt = 0;
+  if ((uint64_t)c.vb + niso > vol.vcap) { atomicOr(vol.status, kStatusVertexOverflow); return; }
+  // every edge point's index first, then all the loads (vertex 0, which
+  // exists, for edges that do not contribute), then the sums in edge order:
+  // the loads are in flight together instead of one edge at a time
+  uint32_t use = 0, pidx[12];
 #pragma unroll
   for (int e = 0; e < 12; ++e) {
+    pidx[e] = 0;
     const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
     if (lab == 0xFu || (NEED[e] & have) != NEED[e]) continue;
     const uint32_t on = OWN_N[e], ol = OWN_L[e];
     if (!((own_bip[on] >> ol) & 1u)) { ok = false; continue; }
     const uint32_t idx = own_base[on] + (uint32_t)__popc(own_bip[on] & ((1u << ol) - 1u));
     if ((uint64_t)idx >= vol.vcap) { ok = false; continue; }
-    const float *pnt = vol.verts + 3ull * idx;
+    pidx[e] = idx;
+    use |= 1u << e;
+  }
+  float px[12], py[12], pz[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    const float *pnt = vol.verts + 3ull * pidx[e];
+    px[e] = pnt[0];
+    py[e] = pnt[1];
+    pz[e] = pnt[2];
+  }
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    if (!((use >> e) & 1u)) continue;
+    const uint32_t lab = (uint32_t)(labels >> (4 * e)) & 0xFu;
     float *acc = s_acc + 3 * lab * T;
-    acc[0] = acc[0] + pnt[0];
-    acc[T] = acc[T] + pnt[1];
-    acc[2 * T] = acc[2 * T] + pnt[2];
+    acc[0] = acc[0] + px[e];
+    acc[T] = acc[T] + py[e];
+    acc[2 * T] = acc[2 * T] + pz[e];
     cnt += 1u << (8 * lab);
   }
-  if ((uint64_t)c.vb + niso > vol.vcap) { atomicOr(vol.status, kStatusVertexOverflow); return; }
   float *out = vol.verts + 3ull * c.vb;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
