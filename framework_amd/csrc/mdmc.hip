@@ -1899,17 +1899,17 @@ int mdmc_batch_generate_device(mdmc_ctx *ctx, const mdmc_gen_spec *specs, uint32
     const uint64_t npts = (uint64_t)X * Y * Z;
     G.block_off = blocks;
     uint64_t nb64 = 0;
-    // rows or planes per task: the most that still gives the chip 8 waves per
-    // SIMD (a small field in tasks of 16 rows is bound by the rows' latency)
+    // planes per task: the most that still gives the chip 8 waves per SIMD (a
+    // small field in tasks of 16 planes is bound by the planes' latency)
     const bool blobs = g.kind == MDMC_FIELD_BLOBS;
     const uint64_t min_tasks = (uint64_t)ctx->num_cus * 32;
     uint32_t per = kGenPer;
-    while (npts && per > 1 && (blobs ? blob_tasks(X, Y, Z, per) : gyroid_tasks(X, Y, Z, per)) < min_tasks) per >>= 1;
+    while (npts && per > 1 && gen_tasks(X, Y, Z, per) < min_tasks) per >>= 1;
     G.per = per;
     if (npts && blobs && g.nblobs > kBlobRegs)
       nb64 = std::min<uint64_t>((npts + 255) / 256, 4096);
     else if (npts)
-      nb64 = ((blobs ? blob_tasks(X, Y, Z, per) : gyroid_tasks(X, Y, Z, per)) + kGenWaves - 1) / kGenWaves;
+      nb64 = (gen_tasks(X, Y, Z, per) + kGenWaves - 1) / kGenWaves;
     if (nb64 + blocks > 0x7FFFFFFFull) return fail(MDMC_ERR_TOO_LARGE, "fields too large for the generator grid");
     G.nblocks = (uint32_t)nb64;
     blocks += G.nblocks;

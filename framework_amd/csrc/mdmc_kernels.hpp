@@ -2022,17 +2022,17 @@ struct BlobArgs {
   float *out;
   uint32_t X, Y, Z, nblobs, ldb;
   const double *ex, *ey, *ez, *amp;
-  uint32_t per;  // rows per task
+  uint32_t per;  // planes per task
 };
 
 // Generators work in wave tasks of 64 consecutive columns x; a lane owns one
-// column and the loop over the task's rows is uniform over the wave, so
-// per-row values are scalar loads and the stores of a row are one contiguous
-// 256-byte run.  Blob tasks take `per` consecutive rows r = y + Y*z; gyroid
-// tasks take one row y in `per` consecutive planes z (what depends on x and y
-// only is then computed once per task).  Consecutive tasks take consecutive
-// column blocks first.  `per` is at most kGenPer, fewer for a field too small
-// to give the chip enough waves (a task's rows run one after another).
+// column, and a task takes one row y in `per` consecutive planes z.  The loop
+// over the task's planes is uniform over the wave, so per-plane values are
+// scalar loads, the stores of a plane are one contiguous 256-byte run, and
+// what depends on x and y only is computed once per task.  Consecutive tasks
+// take consecutive column blocks, then rows, then plane blocks.  `per` is at
+// most kGenPer, fewer for a field too small to give the chip enough waves (a
+// task's planes run one after another).
 constexpr uint32_t kGenPer = 16;
 constexpr uint32_t kGenThreads = 256;
 constexpr uint32_t kGenWaves = kGenThreads / kWave;
@@ -2043,10 +2043,7 @@ __host__ __device__ inline uint32_t blob_ldb(uint32_t nblobs) {
   return nblobs <= kBlobRegs ? kBlobRegs : nblobs;
 }
 
-__host__ __device__ inline uint64_t blob_tasks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per) {
-  return (uint64_t)((X + 63u) / 64u) * (((uint64_t)Y * Z + per - 1u) / per);
-}
-__host__ __device__ inline uint64_t gyroid_tasks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per) {
+__host__ __device__ inline uint64_t gen_tasks(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per) {
   return (uint64_t)((X + 63u) / 64u) * Y * ((Z + per - 1u) / per);
 }
 
@@ -2055,36 +2052,8 @@ __host__ __device__ inline uint64_t gyroid_tasks(uint32_t X, uint32_t Y, uint32_
 typedef const __attribute__((address_space(4))) double *uniform_dptr;
 __device__ __forceinline__ uniform_dptr as_uniform(const double *p) { return (uniform_dptr)p; }
 
-// A blob task: the lane's column x and rows r0 .. r0 + nr - 1, the first as
-// (y0, z0); row r0 + i is (y0 + i, z0) carried into z.
-struct RowTask {
-  uint32_t x, nr, y0, z0;
-  uint64_t r0;
-};
-
-__device__ __forceinline__ RowTask row_task(uint32_t X, uint32_t Y, uint32_t Z, uint32_t per, uint64_t task) {
-  const uint32_t ncc = (X + 63u) / 64u;
-  RowTask t;
-  t.x = (uint32_t)(task % ncc) * 64u + (threadIdx.x & 63u);
-  t.r0 = (task / ncc) * per;
-  const uint64_t nrows = (uint64_t)Y * Z;
-  t.nr = t.r0 < nrows ? (uint32_t)min<uint64_t>(per, nrows - t.r0) : 0u;
-  t.y0 = (uint32_t)(t.r0 % Y);
-  t.z0 = (uint32_t)(t.r0 / Y);
-  return t;
-}
-
-__device__ __forceinline__ void row_yz(const RowTask &t, uint32_t Y, uint32_t i, uint32_t &y, uint32_t &z) {
-  y = t.y0 + i;
-  z = t.z0;
-  while (y >= Y) {
-    y -= Y;
-    ++z;
-  }
-}
-
-// A gyroid task: the lane's column x, its row y, and its planes
-// z0 .. z0 + nz - 1 (nz = 0 past the last task).
+// A task: the lane's column x, its row y, and its planes z0 .. z0 + nz - 1
+// (nz = 0 past the last task).
 struct PlaneTask {
   uint32_t x, y, z0, nz;
 };
@@ -2101,65 +2070,58 @@ __device__ __forceinline__ PlaneTask plane_task(uint32_t X, uint32_t Y, uint32_t
   return t;
 }
 
-// Blob sums: a lane keeps amp[b] * ex[b][x] for every blob in registers and,
-// row by row, adds ((amp*ex) * ey) * ez blob by blob with one FMA each
-// (fma((amp*ex) * ey, ez, sum)), the same operations in the same order as the
-// CPU generator.  Blobs go in groups of 8 (one group
-// of row factors is one wide scalar load); the zero entries that pad the
-// last group add +0.0 to a sum that starts at +0.0 and so can never be -0.0,
-// which leaves it unchanged.  Needs ldb == kBlobRegs.  (Keeping the products
-// with ey per task instead, over planes of one row, issues a third fewer
-// FP64 operations but ran no faster.)
+// Blob sums: a lane keeps (amp[b] * ex[b][x]) * ey[y][b] for every blob in
+// registers and, plane by plane, adds it times ez[z][b] blob by blob with one
+// FMA each (fma((amp*ex) * ey, ez, sum)), the same operations in the same
+// order as the CPU generator.  Four planes go per pass (four independent FMA
+// chains); blobs go in groups of 8 (one group of a plane's factors is one
+// wide scalar load).  The zero entries that pad the last group add +0.0 to a
+// sum that starts at +0.0 and so can never be -0.0, which leaves it
+// unchanged.  Needs ldb == kBlobRegs.
 __device__ __forceinline__ void blob_task(const BlobArgs &a, uint64_t task) {
-  const RowTask t = row_task(a.X, a.Y, a.Z, a.per, task);
-  if (t.nr == 0) return;
-  const uint32_t xc = t.x < a.X ? t.x : a.X - 1u;
+  constexpr uint32_t P = 4;
+  const PlaneTask t = plane_task(a.X, a.Y, a.Z, a.per, task);
+  if (t.x >= a.X || t.nz == 0) return;
   const uniform_dptr amp = as_uniform(a.amp);
-  double ax[kBlobRegs];
+  const uniform_dptr ey = as_uniform(a.ey + (size_t)t.y * kBlobRegs);
+  double txy[kBlobRegs];
 #pragma unroll
   for (uint32_t g = 0; g < kBlobRegs / 8; ++g) {
 #pragma unroll
-    for (uint32_t i = 0; i < 8; ++i) ax[8 * g + i] = 0.0;
+    for (uint32_t i = 0; i < 8; ++i) txy[8 * g + i] = 0.0;
     if (8 * g < a.nblobs) {
 #pragma unroll
-      for (uint32_t i = 0; i < 8; ++i) ax[8 * g + i] = amp[8 * g + i] * a.ex[(size_t)(8 * g + i) * a.X + xc];
+      for (uint32_t i = 0; i < 8; ++i)
+        txy[8 * g + i] = (amp[8 * g + i] * a.ex[(size_t)(8 * g + i) * a.X + t.x]) * ey[8 * g + i];
     }
   }
-  // two rows per pass: two independent sums (each a serial chain of FMAs)
-  // and one wait for the scalar loads of both rows' factors per blob group
-  float *out = a.out + t.r0 * a.X + t.x;
-  for (uint32_t rr = 0; rr < t.nr; rr += 2) {
-    const bool two = rr + 1u < t.nr;
-    uint32_t ya, za, yb, zb;
-    row_yz(t, a.Y, rr, ya, za);
-    row_yz(t, a.Y, two ? rr + 1u : rr, yb, zb);
-    const uniform_dptr eya = as_uniform(a.ey + (size_t)ya * kBlobRegs);
-    const uniform_dptr eza = as_uniform(a.ez + (size_t)za * kBlobRegs);
-    const uniform_dptr eyb = as_uniform(a.ey + (size_t)yb * kBlobRegs);
-    const uniform_dptr ezb = as_uniform(a.ez + (size_t)zb * kBlobRegs);
-    double acca = 0.0, accb = 0.0;
+  const uint64_t plane = (uint64_t)a.X * a.Y;
+  float *out = a.out + (uint64_t)t.z0 * plane + (uint64_t)t.y * a.X + t.x;
+  for (uint32_t kz = 0; kz < t.nz; kz += P) {
+    // planes past the task's last repeat it (summed, not stored)
+    uniform_dptr ez[P];
+#pragma unroll
+    for (uint32_t j = 0; j < P; ++j) ez[j] = as_uniform(a.ez + (size_t)(t.z0 + min(kz + j, t.nz - 1u)) * kBlobRegs);
+    double acc[P];
+#pragma unroll
+    for (uint32_t j = 0; j < P; ++j) acc[j] = 0.0;
 #pragma unroll
     for (uint32_t g = 0; g < kBlobRegs / 8; ++g) {
       if (8 * g < a.nblobs) {
-        double fya[8], fza[8], fyb[8], fzb[8];
+        double f[P][8];
 #pragma unroll
-        for (uint32_t i = 0; i < 8; ++i) {
-          fya[i] = eya[8 * g + i];
-          fza[i] = eza[8 * g + i];
-          fyb[i] = eyb[8 * g + i];
-          fzb[i] = ezb[8 * g + i];
-        }
+        for (uint32_t j = 0; j < P; ++j)
 #pragma unroll
-        for (uint32_t i = 0; i < 8; ++i) {
-          acca = __builtin_fma(ax[8 * g + i] * fya[i], fza[i], acca);
-          accb = __builtin_fma(ax[8 * g + i] * fyb[i], fzb[i], accb);
-        }
+          for (uint32_t i = 0; i < 8; ++i) f[j][i] = ez[j][8 * g + i];
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i)
+#pragma unroll
+          for (uint32_t j = 0; j < P; ++j) acc[j] = __builtin_fma(txy[8 * g + i], f[j][i], acc[j]);
       }
     }
-    if (t.x < a.X) {
-      out[(size_t)rr * a.X] = (float)acca;
-      if (two) out[(size_t)(rr + 1u) * a.X] = (float)accb;
-    }
+#pragma unroll
+    for (uint32_t j = 0; j < P; ++j)
+      if (kz + j < t.nz) out[(size_t)(kz + j) * plane] = (float)acc[j];
   }
 }
 
@@ -2225,7 +2187,7 @@ __device__ __forceinline__ void gyroid_task(const GyroidArgs &a, uint64_t task) 
 struct GenVol {
   float *out;
   uint32_t X, Y, Z, kind;  // kind 1: blobs, 2: gyroid
-  uint32_t nblobs, block_off, nblocks, per;  // per: rows (blobs) or planes (gyroid) per task
+  uint32_t nblobs, block_off, nblocks, per;  // per: planes per task
   const double *t[6];      // blobs: ex, ey, ez, amp; gyroid: sx, cx, sy, cy, sz, cz
   uint64_t seed;
   double amp;              // gyroid noise amplitude
@@ -2243,13 +2205,13 @@ __global__ __launch_bounds__(kGenThreads) void k_field_batch(const GenVol *vols,
   if (v.kind == 1) {
     const BlobArgs ba{v.out, v.X, v.Y, v.Z, v.nblobs, blob_ldb(v.nblobs), v.t[0], v.t[1], v.t[2], v.t[3], v.per};
     if (v.nblobs <= kBlobRegs) {
-      if (task < blob_tasks(v.X, v.Y, v.Z, v.per)) blob_task(ba, task);
+      if (task < gen_tasks(v.X, v.Y, v.Z, v.per)) blob_task(ba, task);
     } else {
       blob_points(ba, (size_t)local * blockDim.x + threadIdx.x, (size_t)v.nblocks * blockDim.x);
     }
     return;
   }
-  if (task < gyroid_tasks(v.X, v.Y, v.Z, v.per)) {
+  if (task < gen_tasks(v.X, v.Y, v.Z, v.per)) {
     const GyroidArgs ga{v.out, v.X, v.Y, v.Z, v.t[0], v.t[1], v.t[2], v.t[3], v.t[4], v.t[5], v.seed, v.amp, v.per};
     gyroid_task(ga, task);
   }
