@@ -5,7 +5,7 @@ The tunable launch shapes are context options (MDMC_OPT_* in include/mdmc.h):
   single volume  MDMC_OPT_PT_ROWS              cell rows per classify tile (7, 15)
                  MDMC_OPT_PT_PLANES            cell planes per classify tile
                  MDMC_OPT_PT_BAND              columns per XCD group
-                 MDMC_OPT_EMIT_BLOCKS_PER_CU   emit grid
+                 MDMC_OPT_EMIT_BLOCKS_PER_CU   persistent emit grid
   batches        MDMC_OPT_BATCH_PLANES         cell planes per batch classify tile
                  MDMC_OPT_CLASSIFY_BLOCKS_PER_CU  persistent classify grid (0: a workgroup per tile)
                  MDMC_OPT_EMIT_BLOCKS_PER_CU   emit grid
@@ -30,7 +30,7 @@ from .mdmc import (OPT_BATCH_PLANES, OPT_CLASSIFY_BLOCKS_PER_CU, OPT_EMIT_BLOCKS
                    OPT_PT_PLANES, OPT_PT_ROWS, MDMCError)
 
 SINGLE_SPACE = {OPT_PT_ROWS: [0, 15], OPT_PT_PLANES: [0, 1, 2, 3], OPT_PT_BAND: [0, 2, 4],
-                OPT_EMIT_BLOCKS_PER_CU: [0, 8, 32, 256]}
+                OPT_EMIT_BLOCKS_PER_CU: [0, 16]}
 BATCH_SPACE = {OPT_BATCH_PLANES: [0, 1, 2, 4], OPT_CLASSIFY_BLOCKS_PER_CU: [0, 4, 8, 12],
                OPT_EMIT_BLOCKS_PER_CU: [0, 8, 32, 256]}
 OPTION_NAMES = {OPT_BATCH_PLANES: "batch_planes", OPT_CLASSIFY_BLOCKS_PER_CU: "classify_blocks_per_cu",

@@ -282,6 +282,7 @@ struct mdmc_ctx {
   bool captured = false;
   unsigned long long capture_id = 0;
   bool capturing_now = false;  // the current call's stream is capturing
+  bool tickets_zero = false;   // the emit's ticket counters are zero (a count ran after the last emit)
   // size the scratch exactly (no headroom): set for the out-of-core
   // context, which keeps to a device budget
   bool exact_scratch = false;
@@ -364,7 +365,7 @@ ScratchLayout scratch_layout(const Plan &p) {
   L.ne = take((p.nsegs + 1) * sizeof(uint2));
   L.chunk = take(p.max_chunks * sizeof(uint32_t));
   L.state = take((size_t)p.ntiles * 4 * sizeof(unsigned long long));
-  L.ticket = take(sizeof(uint32_t));
+  L.ticket = take(kTicketBytes);
   L.totals = take(sizeof(Totals));
   L.slots = take(p.nslots * sizeof(uint64_t));
   L.gmap = take(p.nsegs * (p.cap / 64) * sizeof(uint4));
@@ -433,6 +434,16 @@ CoordArgs coord_args(const Plan &p, const Scratch &s, const float mn[3], const f
                    mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
 }
 
+// Grid of the single-volume emit (k_emit): its persistent workgroups, a
+// multiple of 8 (one chunk ticket counter per XCD), no more than one wave per
+// chunk of the capacity needs.
+uint32_t emit_blocks(const mdmc_ctx *ctx, uint64_t max_chunks) {
+  const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitPersistPerCu;
+  const uint64_t want = (max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
+  const uint64_t b = std::min<uint64_t>(want, (uint64_t)ctx->num_cus * (uint64_t)bpc);
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8, (b + 7) / 8 * 8), 0x7FFFFFF8ull);
+}
+
 // box (optional): xyz_min, xyz_max; with it the scan also fills the
 // coordinate tables the emit reads.
 int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
@@ -476,6 +487,7 @@ int launch_count(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
   }
   if (rc != MDMC_OK) return fail(MDMC_ERR_INVALID, "no classify kernel for this tile shape");
   HIP_TRY(hipGetLastError());
+  ctx->tickets_zero = true;
 
   ScanArgs sa{};
   sa.seg_counts = s.seg_counts;
@@ -546,11 +558,12 @@ int launch_emit(mdmc_ctx *ctx, const Plan &p, const float *d_field, float iso,
       std::memcpy(ctx->coords_box + 3, mx, 3 * sizeof(float));
       ctx->coords_valid = true;
     }
-    // one wave per chunk of 64 active cells, at most kEmitBlocksPerCu workgroups per CU
-    const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
-    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitBlocksPerCu;
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
-    hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, st, ea);
+    ea.emit_ctr = s.ticket;
+    // the count zeroed the ticket counters; an emit after another (or one
+    // being captured, whose replays may follow one) zeroes them first
+    if (!ctx->tickets_zero || ctx->capturing_now) hipLaunchKernelGGL(k_zero_tickets, dim3(1), dim3(64), 0, st, s.ticket);
+    hipLaunchKernelGGL(k_emit, dim3(emit_blocks(ctx, p.max_chunks)), dim3(kEmitThreads), 0, st, ea);
+    ctx->tickets_zero = false;
   }
   HIP_TRY(hipGetLastError());
   ctx->emitted = true;
@@ -1760,10 +1773,9 @@ int mdmc_run_out_of_core(const float *field, const uint32_t dims[3], const float
     ea.v.status = reinterpret_cast<uint32_t *>(d_counts) + 3;
     hipLaunchKernelGGL(k_coords, dim3((uint32_t)std::min<uint64_t>((p.ncoord + 255) / 256, 1024)), dim3(256), 0,
                        nullptr, co);
-    const uint64_t want = (p.max_chunks + kEmitThreads / 64 - 1) / (kEmitThreads / 64);
-    const int bpc = ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] > 0 ? ctx->opt[MDMC_OPT_EMIT_BLOCKS_PER_CU] : kEmitBlocksPerCu;
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ctx->num_cus * bpc));
-    hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kEmitThreads), 0, nullptr, ea);
+    ea.emit_ctr = sc.ticket;  // zeroed by this slab's count
+    hipLaunchKernelGGL(k_emit, dim3(emit_blocks(ctx, p.max_chunks)), dim3(kEmitThreads), 0, nullptr, ea);
+    ctx->tickets_zero = false;
     if (hipGetLastError() != hipSuccess) return bail(MDMC_ERR_HIP, "slab emit launch");
     ctx->emitted = false;  // the slab mesh is not the context's mesh
     if (hipMemcpy(&hc, d_counts, sizeof(hc), hipMemcpyDeviceToHost) != hipSuccess || hc.status)

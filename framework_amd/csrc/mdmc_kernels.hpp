@@ -174,6 +174,17 @@ __device__ __forceinline__ uint32_t use_variant1(const FieldDesc &g, const uint1
   return (info_tab[ncfg] >> 6) & 1u;
 }
 
+// The scan's ticket word opens a region of the scratch of kTicketBytes: word 0
+// is the scan's tile ticket, and word kEmitCtr * (x + 1) the emit's chunk
+// ticket counter of XCD x (k_emit), each on a 256-byte line of its own (the
+// atomics on one line serialise).  The classify zeroes all nine.
+constexpr uint32_t kEmitCtr = 64;
+constexpr uint32_t kTicketBytes = 9 * kEmitCtr * 4;
+
+__device__ __forceinline__ void zero_tickets(uint32_t *t, uint32_t tid) {
+  if (tid < 9u) t[tid * kEmitCtr] = 0u;
+}
+
 struct ClassifyArgs {
   FieldDesc g;
   uint32_t XT, NRB, NKC, Kc;
@@ -185,7 +196,7 @@ struct ClassifyArgs {
   uint32_t scan_state_words;
   uint2 *seg_pad;         // segment totals past the last segment, up to a whole scan tile
   uint32_t seg_pad_n;     // (zeroed here, read by the scan)
-  uint32_t *scan_ticket;
+  uint32_t *scan_ticket;  // the ticket region, zeroed here
   Totals *totals;
   uint32_t *counts_out;   // user mdmc_counts (4 words), zeroed here
 };
@@ -202,7 +213,7 @@ __global__ __launch_bounds__(64) void k_classify(ClassifyArgs a) {
     for (uint32_t i = lane; i < a.scan_state_words; i += kWave) a.scan_state[i] = 0ull;
     for (uint32_t i = lane; i < a.seg_pad_n; i += kWave) a.seg_pad[i] = make_uint2(0u, 0u);
     if (lane < 8) reinterpret_cast<uint32_t *>(a.totals)[lane] = 0u;
-    if (lane == 0) *a.scan_ticket = 0u;
+    zero_tickets(a.scan_ticket, lane);
     if (a.counts_out != nullptr && lane < 4) a.counts_out[lane] = 0u;
   }
   __syncthreads();
@@ -431,7 +442,7 @@ __device__ __forceinline__ void zero_scan_state(const ClassifyArgs &a, uint32_t 
   for (uint32_t i = tid; i < a.scan_state_words; i += nthreads) a.scan_state[i] = 0ull;
   for (uint32_t i = tid; i < a.seg_pad_n; i += nthreads) a.seg_pad[i] = make_uint2(0u, 0u);
   if (tid < 8) reinterpret_cast<uint32_t *>(a.totals)[tid] = 0u;
-  if (tid == 0) *a.scan_ticket = 0u;
+  zero_tickets(a.scan_ticket, tid);
 }
 
 // One plane tile (x-tile xt, row block rb, plane block kb) of the volume `a`.
@@ -1061,6 +1072,7 @@ struct EmitArgs {
   const uint2 *ne;
   const uint32_t *chunk_ne;
   const Totals *totals;
+  uint32_t *emit_ctr;  // k_emit's chunk tickets: the ticket region (zero at launch)
 };
 
 // Arrays seen through the constant address space: loads at wave-uniform
@@ -1379,27 +1391,77 @@ __device__ __forceinline__ bool workgroup_has_chunks(uint32_t n) {
   return it.next < it.end;
 }
 
+// Single-volume emit grid: a persistent grid of this many workgroups per CU
+// (MDMC_OPT_EMIT_BLOCKS_PER_CU overrides it), the most that are resident at
+// once at 8 waves per SIMD.
+constexpr int kEmitPersistPerCu = 8;
+
+// Chunk tickets of the persistent emit.  Workgroups are dispatched
+// round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), so the
+// workgroups b with b % 8 == x take the contiguous chunk range [x n / 8,
+// (x + 1) n / 8) in tickets of one chunk per wave: ticket t is chunks
+// [lo + 4 t, lo + 4 t + 4).  Workgroup b's first ticket is its slot b / 8 (no
+// atomic), its later ones come from counter x, offset by the number of
+// slots; the next ticket is asked for while the current one's chunks run.
+// The chunks in flight on an XCD stay one contiguous window that moves
+// through its range in order, so neighbouring chunks, which share field rows
+// and the records of neighbour segments, meet in its L2 while both are in
+// flight.  The placement only shapes locality: any mapping of workgroups to
+// XCDs takes every chunk exactly once.
+struct EmitTickets {
+  uint32_t lo, hi;
+};
+
+__device__ __forceinline__ EmitTickets emit_tickets(uint32_t n) {
+  const uint32_t x = blockIdx.x & 7u;
+  return EmitTickets{(uint32_t)((uint64_t)n * x / 8u), (uint32_t)((uint64_t)n * (x + 1u) / 8u)};
+}
+
 // 8 waves per SIMD: the emit is latency-bound, the register cap costs a few
-// scalar spills only
+// scalar spills only.  The grid is a multiple of 8 workgroups, and the ticket
+// counters are zero at launch (the classify zeroes them; the host zeroes them
+// again before an emit that follows another).
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_emit(EmitArgs a) {
+  constexpr uint32_t wpb = kEmitThreads / kWave;
   __shared__ uint64_t s_lab[2 * 256];
   __shared__ float s_acc[12 * kEmitThreads];
+  __shared__ uint32_t s_ticket[2];
   const Totals tot = *a.totals;
   // Counts are clamped to buffer capacities so that no access can leave the
   // scratch buffers even if the count pass produced inconsistent data.
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
-  if (!workgroup_has_chunks(num_chunks)) return;  // (uniform: before the barrier)
+  const EmitTickets et = emit_tickets(num_chunks);
+  const uint32_t slots = gridDim.x >> 3;
+  uint64_t first = et.lo + (uint64_t)(blockIdx.x >> 3) * wpb;
+  if (first >= et.hi) return;  // (uniform: before the barrier)
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
+  uint32_t *ctr = a.emit_ctr + kEmitCtr * ((blockIdx.x & 7u) + 1u);
   __syncthreads();
   const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
-  const ChunkIter it = chunk_iter(num_chunks, wave);
-  for (uint32_t ch = it.next; ch < it.end; ch += it.step) {
-    uint32_t s, rank;
-    if (!locate_cell(a, tot, num_ne, ch, lane, s, rank)) continue;
-    emit_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
+  // when the first tickets cover every chunk there is nothing to ask for
+  const bool dynamic = et.lo + (uint64_t)slots * wpb < et.hi;
+  for (uint32_t it = 0;; ++it) {
+    // the next ticket is asked for before this one's chunks run and published
+    // after them; two slots, because thread 0 writes the next iteration's
+    // only after every thread has passed this iteration's barrier
+    uint32_t next = 0;
+    if (dynamic && threadIdx.x == 0) next = atomicAdd(ctr, 1u);
+    const uint64_t ch = first + wave;
+    if (ch < et.hi) {
+      uint32_t s, rank;
+      if (locate_cell(a, tot, num_ne, (uint32_t)ch, lane, s, rank)) emit_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
+    }
+    if (!dynamic) break;
+    if (threadIdx.x == 0) s_ticket[it & 1u] = next;
+    __syncthreads();
+    first = et.lo + ((uint64_t)slots + s_ticket[it & 1u]) * wpb;
+    if (first >= et.hi) break;
   }
 }
+
+// Zeroes the emit's ticket counters (one workgroup of 64 threads).
+__global__ void k_zero_tickets(uint32_t *t) { zero_tickets(t, threadIdx.x); }
 
 
 // ---------------------------------------------------------------------------

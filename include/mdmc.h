@@ -314,7 +314,8 @@ int mdmc_ctx_set_scratch(mdmc_ctx *ctx, void *d_scratch, uint64_t bytes);
 enum {
   MDMC_OPT_BATCH_PLANES = 0,           /* cell planes per batch classify tile */
   MDMC_OPT_CLASSIFY_BLOCKS_PER_CU = 1, /* persistent batch classify grid (0: a workgroup per tile) */
-  MDMC_OPT_EMIT_BLOCKS_PER_CU = 2,     /* emit grid: workgroups per CU (single and batch; default 128) */
+  MDMC_OPT_EMIT_BLOCKS_PER_CU = 2,     /* emit grid, workgroups per CU: single-volume persistent grid
+                                          (default 8), batch and smoothing at most (default 128) */
   MDMC_OPT_PT_PLANES = 3,              /* cell planes per single-volume classify tile */
   MDMC_OPT_PT_BAND = 4,                /* single-volume classify: columns per XCD group */
   MDMC_OPT_PT_ROWS = 5                 /* cell rows per single-volume classify tile: 7 or 15 */

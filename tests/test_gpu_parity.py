@@ -203,6 +203,37 @@ def test_world_box_through_both_coordinate_paths(ctx, torch_cuda):
     assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), oracle.run(f, 0.0, mn, mx))
 
 
+@pytest.mark.parametrize("emit_per_cu", [0, 1])
+def test_repeated_emits_of_one_count(ctx, torch_cuda, emit_per_cu):
+    """The emit takes chunks by ticket from counters that the count zeroes;
+    an emit that follows another without a count between zeroes them first.
+    Three emits of one count equal the oracle on a surface-dense field with
+    many more chunks than the emit's first tickets cover (with one workgroup
+    per CU most chunks come through the counters)."""
+    torch = torch_cuda
+    from framework_amd.mdmc import OPT_EMIT_BLOCKS_PER_CU
+
+    dims = (160, 150, 140)
+    ft = ctx.field_gyroid(dims, period=7.0, seed=4, noise_amp=0.05)
+    torch.cuda.synchronize()
+    f = ft.cpu().numpy()
+    mn, mx = (0.0, 0.0, 0.0), tuple(float(d - 1) for d in dims)
+    ref = oracle.run(f, 0.0, mn, mx)
+    assert ref["num_active"] > 64 * 8 * 1024  # more chunks than the first tickets of 256 CUs cover
+    ctx.set_option(OPT_EMIT_BLOCKS_PER_CU, emit_per_cu)
+    try:
+        ctx.count(ft, 0.0)
+        for rep in range(3):
+            verts = torch.full((ref["num_vertices"], 3), -1.0, device="cuda")
+            quads = torch.full((ref["num_quads"], 4), -1, dtype=torch.int32, device="cuda")
+            ctx.emit(ft, verts, quads, xyz_min=mn, xyz_max=mx)
+            torch.cuda.synchronize()
+            assert ctx.read_counts()["status"] == 0, rep
+            assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), ref)
+    finally:
+        ctx.set_option(OPT_EMIT_BLOCKS_PER_CU, 0)
+
+
 def test_field_generators_match_cpu(ctx, torch_cuda):
     dims = (70, 51, 33)
     gb = ctx.field_blobs(dims, seed=11, nblobs=9)
