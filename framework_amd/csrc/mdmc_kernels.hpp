@@ -1396,31 +1396,18 @@ __device__ __forceinline__ bool workgroup_has_chunks(uint32_t n) {
 // once at 8 waves per SIMD.
 constexpr int kEmitPersistPerCu = 8;
 
-// Chunk tickets of the persistent emit.  Workgroups are dispatched
-// round-robin over the 8 XCDs (workgroup b runs on XCD b % 8), so the
-// workgroups b with b % 8 == x take the contiguous chunk range [x n / 8,
-// (x + 1) n / 8) in tickets of one chunk per wave: ticket t is chunks
-// [lo + 4 t, lo + 4 t + 4).  Workgroup b's first ticket is its slot b / 8 (no
-// atomic), its later ones come from counter x, offset by the number of
-// slots; the next ticket is asked for while the current one's chunks run.
-// The chunks in flight on an XCD stay one contiguous window that moves
-// through its range in order, so neighbouring chunks, which share field rows
-// and the records of neighbour segments, meet in its L2 while both are in
-// flight.  The placement only shapes locality: any mapping of workgroups to
-// XCDs takes every chunk exactly once.
-struct EmitTickets {
-  uint32_t lo, hi;
-};
-
-__device__ __forceinline__ EmitTickets emit_tickets(uint32_t n) {
-  const uint32_t x = blockIdx.x & 7u;
-  return EmitTickets{(uint32_t)((uint64_t)n * x / 8u), (uint32_t)((uint64_t)n * (x + 1u) / 8u)};
-}
-
 // 8 waves per SIMD: the emit is latency-bound, the register cap costs a few
-// scalar spills only.  The grid is a multiple of 8 workgroups, and the ticket
-// counters are zero at launch (the classify zeroes them; the host zeroes them
-// again before an emit that follows another).
+// scalar spills only.  Chunks come by ticket (emit_tickets, mdmc_tilemap.h):
+// workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), so the chunks in flight on an XCD stay one contiguous window
+// that moves through its range in order, and neighbouring chunks, which share
+// field rows and the records of neighbour segments, meet in its L2 while both
+// are in flight; the next ticket is asked for before the current one's
+// chunks run and read after them, so its latency hides under the work.  The
+// placement only shapes locality: any mapping of workgroups to XCDs takes
+// every chunk exactly once.  The grid is a multiple of 8 workgroups, and the
+// ticket counters are zero at launch (the classify zeroes them; the host
+// zeroes them again before an emit that follows another).
 __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_emit(EmitArgs a) {
   constexpr uint32_t wpb = kEmitThreads / kWave;
   __shared__ uint64_t s_lab[2 * 256];
@@ -1430,32 +1417,29 @@ __global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_waves_per_eu(8,
   // Counts are clamped to buffer capacities so that no access can leave the
   // scratch buffers even if the count pass produced inconsistent data.
   const uint32_t num_chunks = (uint32_t)min((uint64_t)tot.num_chunks, a.max_chunks);
-  const EmitTickets et = emit_tickets(num_chunks);
-  const uint32_t slots = gridDim.x >> 3;
-  uint64_t first = et.lo + (uint64_t)(blockIdx.x >> 3) * wpb;
+  const EmitTickets et = emit_tickets(num_chunks, blockIdx.x & 7u, gridDim.x, wpb);
+  uint64_t first = emit_ticket_chunk(et, blockIdx.x >> 3, wpb);
   if (first >= et.hi) return;  // (uniform: before the barrier)
   for (uint32_t i = threadIdx.x; i < 512; i += kEmitThreads) s_lab[i] = (&c_label[0][0])[i];
   uint32_t *ctr = a.emit_ctr + kEmitCtr * ((blockIdx.x & 7u) + 1u);
   __syncthreads();
   const uint32_t lane = lane_id(), wave = wave_id();
   const uint32_t num_ne = min(tot.num_nonempty, a.nsegs);
-  // when the first tickets cover every chunk there is nothing to ask for
-  const bool dynamic = et.lo + (uint64_t)slots * wpb < et.hi;
   for (uint32_t it = 0;; ++it) {
     // the next ticket is asked for before this one's chunks run and published
     // after them; two slots, because thread 0 writes the next iteration's
     // only after every thread has passed this iteration's barrier
     uint32_t next = 0;
-    if (dynamic && threadIdx.x == 0) next = atomicAdd(ctr, 1u);
+    if (et.dynamic && threadIdx.x == 0) next = atomicAdd(ctr, 1u);
     const uint64_t ch = first + wave;
     if (ch < et.hi) {
       uint32_t s, rank;
       if (locate_cell(a, tot, num_ne, (uint32_t)ch, lane, s, rank)) emit_cell(a, a.v, s, rank, s_lab, s_acc + threadIdx.x);
     }
-    if (!dynamic) break;
+    if (!et.dynamic) break;
     if (threadIdx.x == 0) s_ticket[it & 1u] = next;
     __syncthreads();
-    first = et.lo + ((uint64_t)slots + s_ticket[it & 1u]) * wpb;
+    first = emit_ticket_chunk(et, (uint64_t)et.slots + s_ticket[it & 1u], wpb);
     if (first >= et.hi) break;
   }
 }

@@ -1,4 +1,5 @@
-// Workgroup -> tile map of the plane-tile classify (k_classify_pt).
+// Workgroup -> tile map of the plane-tile classify (k_classify_pt), and the
+// chunk tickets of the persistent emit (k_emit).
 //
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
 // b % 8).  A column is one x-tile of one row block; columns form groups of B.
@@ -54,6 +55,29 @@ MDMC_HD inline PtTile pt_tile_of(uint32_t b, uint32_t XT, uint32_t NRB, uint32_t
   t.kb = kb;
   return t;
 }
+
+// Chunk tickets of the persistent emit (k_emit).  The workgroups b with
+// b % 8 == x (XCD x) take the contiguous chunk range [x n / 8, (x + 1) n / 8)
+// in tickets of one chunk per wave: ticket t is chunks [lo + W t, lo + W t +
+// W) for W waves per workgroup.  Workgroup b's first ticket is its slot b / 8
+// (no atomic), its later ones are slots + c for the values c of counter x,
+// taken only when the first tickets do not already cover the range
+// (dynamic).  A workgroup stops at the first ticket that starts at or past
+// hi.  G, the number of workgroups, is a multiple of 8.
+struct EmitTickets {
+  uint32_t lo, hi, slots;
+  bool dynamic;
+};
+MDMC_HD inline EmitTickets emit_tickets(uint32_t n, uint32_t x, uint32_t G, uint32_t W) {
+  EmitTickets e;
+  e.lo = (uint32_t)((uint64_t)n * x / 8u);
+  e.hi = (uint32_t)((uint64_t)n * (x + 1u) / 8u);
+  e.slots = G >> 3;
+  e.dynamic = e.lo + (uint64_t)e.slots * W < e.hi;
+  return e;
+}
+// first chunk of ticket t
+MDMC_HD inline uint64_t emit_ticket_chunk(const EmitTickets &e, uint64_t t, uint32_t W) { return e.lo + t * W; }
 
 }  // namespace mdmc
 

@@ -91,3 +91,16 @@ def test_classify_tile_map_covers_every_tile_once(tmp_path):
                     os.path.join(ROOT, "tests", "golden", "tile_map_check.cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
     assert out.strip() == "OK", out
+
+
+def test_emit_tickets_take_every_chunk_once(tmp_path):
+    """The persistent emit's chunk tickets (mdmc_tilemap.h), simulated on the
+    CPU in k_emit's control flow with the workgroups advanced in random order
+    over 4000 random chunk counts, grids and wave counts: every chunk exactly
+    once, none past the end, every workgroup stops, bounded counters
+    (tests/golden/emit_ticket_check.cpp)."""
+    exe = tmp_path / "emit_ticket_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "framework_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "golden", "emit_ticket_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300).stdout
+    assert out.strip() == "OK", out
