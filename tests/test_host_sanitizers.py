@@ -1,6 +1,6 @@
 """Host-side checks built with the compilers' sanitizers (CPU only): the
-device-memory pool and the classify tile map under AddressSanitizer and
-UndefinedBehaviorSanitizer, the oracle under both, and a stress emulation of
+device-memory pool, the classify tile map and the emit's chunk tickets under
+AddressSanitizer and UndefinedBehaviorSanitizer, the oracle under both, and a stress emulation of
 the scan's decoupled look-back protocol (plain and under ThreadSanitizer)."""
 from __future__ import annotations
 
@@ -24,7 +24,7 @@ def _run(cmd, exe, args=(), timeout=600):
     assert res.returncode == 0 and res.stdout.strip().endswith("OK"), res.stdout[-2000:] + res.stderr[-4000:]
 
 
-@pytest.mark.parametrize("src", ["pool_check.cpp", "tile_map_check.cpp"])
+@pytest.mark.parametrize("src", ["pool_check.cpp", "tile_map_check.cpp", "emit_ticket_check.cpp"])
 def test_host_checks_under_asan_ubsan(tmp_path, src):
     exe = str(tmp_path / src.replace(".cpp", ""))
     _run(["g++", "-std=c++17", *ASAN, "-I", CSRC, os.path.join(GOLDEN, src), "-o", exe], exe)
