@@ -24,7 +24,7 @@ ARCH = os.environ.get("MDMC_ARCH", "gfx950")
 
 SOURCES = [os.path.join(CSRC, "mdmc.hip"), os.path.join(CSRC, "mdmc_server.cpp"), os.path.join(CSRC, "mdmc_io.cpp")]
 HEADERS = [os.path.join(CSRC, f) for f in ("mdmc_kernels.hpp", "mdmc_internal.h", "dmc_luts.h",
-                                           "mdmc_fields.h", "mdmc_pool.h", "mdmc_io.h", "mdmc_tilemap.h")] + [os.path.join(ROOT, "include", "mdmc.h")]
+                                           "mdmc_fields.h", "mdmc_pool.h", "mdmc_io.h", "mdmc_tilemap.h", "mdmc_fmt.h")] + [os.path.join(ROOT, "include", "mdmc.h")]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",

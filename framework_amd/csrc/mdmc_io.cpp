@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/mdmc.h"
+#include "mdmc_fmt.h"
 #include "mdmc_io.h"
 
 namespace {
@@ -174,10 +175,19 @@ constexpr size_t kObjLineMax = 128;  // a vertex line or a quad's two lines fit
 size_t obj_vertices(char *dst, const float *v, uint64_t i0, uint64_t i1) {
   size_t used = 0;
   // "v x y z" with six significant digits (the reference writes floats
-  // through std::ostream at its default precision, dmc.cu:2238-2246)
-  for (uint64_t i = i0; i < i1; ++i)
-    used += (size_t)std::snprintf(dst + used, kObjLineMax, "v %g %g %g\n", (double)v[3 * i], (double)v[3 * i + 1],
-                                  (double)v[3 * i + 2]);
+  // through std::ostream at its default precision, dmc.cu:2238-2246): the
+  // bytes of printf's "v %g %g %g\n" (mdmc_fmt.h)
+  for (uint64_t i = i0; i < i1; ++i) {
+    char *o = dst + used;
+    size_t n = 0;
+    for (int c = 0; c < 3; ++c) {
+      o[n++] = c ? ' ' : 'v';
+      if (!c) o[n++] = ' ';
+      n += mdmc::fmt_g(o + n, v[3 * i + c]);
+    }
+    o[n++] = '\n';
+    used += n;
+  }
   return used;
 }
 
@@ -187,17 +197,25 @@ size_t obj_faces(char *dst, const int32_t *quads, uint64_t i0, uint64_t i1) {
   for (uint64_t q = i0; q < i1; ++q) {
     const uint32_t a = (uint32_t)quads[4 * q] + 1u, b = (uint32_t)quads[4 * q + 1] + 1u,
                    c = (uint32_t)quads[4 * q + 2] + 1u, d = (uint32_t)quads[4 * q + 3] + 1u;
-    used += (size_t)std::snprintf(dst + used, kObjLineMax,
-                                  "f %" PRIu32 " %" PRIu32 " %" PRIu32 "\nf %" PRIu32 " %" PRIu32 " %" PRIu32 "\n", a,
-                                  b, c, c, d, a);
+    const uint32_t idx[6] = {a, b, c, c, d, a};
+    char *o = dst + used;
+    size_t n = 0;
+    for (int k = 0; k < 6; ++k) {
+      if (k % 3 == 0) o[n++] = 'f';
+      o[n++] = ' ';
+      n += mdmc::fmt_u32(o + n, idx[k]);
+      if (k % 3 == 2) o[n++] = '\n';
+    }
+    used += n;
   }
   return used;
 }
 
 }  // namespace
 
-// Formatting the text is what costs (a few hundred ns per line), so chunks of
-// lines are formatted by several threads, a round of chunks at a time (memory
+// Formatting the text is what costs (about 100 ns per vertex line with the
+// integer formatter of mdmc_fmt.h, a few hundred through snprintf), so chunks
+// of lines are formatted by several threads, a round of chunks at a time (memory
 // stays at a round's buffers however large the mesh), and written in order:
 // the bytes are those of one thread writing line after line.
 extern "C" int mdmc_write_obj(const char *path, const float *vertices, uint64_t num_vertices, const int32_t *quads,

@@ -92,6 +92,18 @@ def test_obj_writers_agree(tmp_path, nv, nq):
     assert (tmp_path / "c.obj").read_bytes() == a
 
 
+def test_obj_number_text_matches_printf(tmp_path):
+    """The OBJ writer formats numbers itself (mdmc_fmt.h): its "%g" of a
+    float and its "%u" are byte for byte snprintf's, over every 997th float
+    bit pattern, dense sweeps around powers of ten and decade carries, and
+    random values (tests/golden/fmt_check.cpp)."""
+    exe = tmp_path / "fmt_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "framework_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "golden", "fmt_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300).stdout
+    assert out.strip() == "OK", out
+
+
 def test_obj_writer_rejects_out_of_range_quads(tmp_path):
     """A face naming a vertex outside the array is an error, and no file is made."""
     import numpy as np
