@@ -203,13 +203,14 @@ def test_world_box_through_both_coordinate_paths(ctx, torch_cuda):
     assert_same_mesh(verts.cpu().numpy(), quads.cpu().numpy(), oracle.run(f, 0.0, mn, mx))
 
 
-@pytest.mark.parametrize("emit_per_cu", [0, 1])
+@pytest.mark.parametrize("emit_per_cu", [0, 1, 3, 1000])
 def test_repeated_emits_of_one_count(ctx, torch_cuda, emit_per_cu):
     """The emit takes chunks by ticket from counters that the count zeroes;
     an emit that follows another without a count between zeroes them first.
     Three emits of one count equal the oracle on a surface-dense field with
     many more chunks than the emit's first tickets cover (with one workgroup
-    per CU most chunks come through the counters)."""
+    per CU most chunks come through the counters), at grids of 8, 1, 3 and
+    1000 workgroups per CU (the last capped by the chunk capacity)."""
     torch = torch_cuda
     from framework_amd.mdmc import OPT_EMIT_BLOCKS_PER_CU
 
